@@ -1,0 +1,189 @@
+"""Benchmark: steps/s and pair-evals/s of the per-step hot path at N = 262,144.
+
+Workload (one "step" = one velocity-Verlet step of the whole system; the
+neighbour list is rebuilt every 10 steps inside the timed region):
+262,144 atoms at reduced density 0.8 (L = 68.94), Lennard-Jones eps = sigma = 1
+cut at 2.5 plus cutoff Coulomb (q = +-1, k = 1, cut at 2.5), dt = 0.001, list
+radius 2.8 (skin 0.3), reflecting walls, FP32 state and pair maths.  Inputs are
+synthetic (seeded jittered lattice, Gaussian velocities at kT = 1) and resident
+in HBM before the timed region starts.
+
+Prints one JSON line.  Extra fields: the mixed-precision and FP64 forms on the
+same workload, the reference-key (masked Morton) form, the fused force
+kernel's roofline, and the threaded CPU baseline (the reference's CPU path,
+restated in C under oracle/, timed on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32
+FLOPS_PER_PAIR_LJC = 32        # per in-cutoff pair, LJ + Coulomb (+,-,*,/,sqrt,rsqrt,cmp = 1; FMA = 2)
+# Algorithmic HBM bytes of one fused FP32 force+integrate launch:
+#   8 B per unique listed pair (two 4-byte directed list entries)
+#   + 68 B per atom (16 B position read, 16 B velocity read + write,
+#     16 B drifted position write, 4 B list length)
+BYTES_PER_PAIR = 8
+BYTES_PER_ATOM = 68
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_spec(pkg, c, precision, key, energy_every=10, use_graph=True):
+    return pkg.SimSpec(duration=0, stepwidth=c["dt"], rc=c["rc"], rc_coul=c["rc_coul"],
+                       skin=c["skin"], rebuild_every=c["rebuild_every"], lo=0.0, hi=c["L"],
+                       precision=precision, morton=key, energy_every=energy_every,
+                       use_graph=use_graph, walls=True)
+
+
+def timed_run(pkg, sim, steps):
+    ctx = sim.ctx
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    sim.run(steps)
+    ms = ctx.timer_stop()
+    wall = time.perf_counter() - t0
+    return ms, wall
+
+
+def gpu_form(pkg, c, precision, key, warmup, steps):
+    sim = pkg.Simulation(c["n"], make_spec(pkg, c, precision, key))
+    sim.set_state(c["x"], c["v"], c["q"])
+    sim.run(warmup)
+    sim.energies()
+    ms, wall = timed_run(pkg, sim, steps)
+    rows = sim.energies()
+    pairs = float(rows[:, 4].mean()) if len(rows) else float("nan")
+    return sim, ms, wall, pairs
+
+
+def cpu_baseline(c, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the CPU restatement of the reference path
+    O.lib()
+    bp = O.BaseParams(1.0, 1.0, c["rc"], 1.0, c["rc_coul"], c["dt"], c["skin"], 0.0, c["L"], 4,
+                      c["rebuild_every"], 1, O.PRED_SCALAR, O.MORTON_MASKED, threads)
+    t0 = time.perf_counter()
+    _, _, times, pe = O.base_run(c["x"], c["v"], c["q"], bp, steps)
+    wall = time.perf_counter() - t0
+    return wall, times, pe
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed/FP64/masked forms")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the FP32 timed region (for profiler runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    c = pkg.synth.plain_scale(seed=2)
+    n = c["n"]
+    log("workload ljc-262144: N=%d L=%.4f" % (n, c["L"]))
+
+    # ---- headline: FP32 plain path, interleaved LBVH key ----
+    sim, ms, wall, pairs = gpu_form(pkg, c, "fp32", pkg.MORTON_INTERLEAVE, args.warmup, args.steps)
+    steps_per_s = args.steps / (ms / 1e3)
+    log("fp32: %.3f ms/step (%.1f steps/s), wall %.3f s, %.0f pairs in cutoff" %
+        (ms / args.steps, steps_per_s, wall, pairs))
+    st = sim.stats()
+    result = {
+        "metric": "steps/s + pair-evals/s at N=262k LJ; mixed vs plain FP64 @1e-5 force err",
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "ljc-262144", "n": n, "density": 0.8, "rc_lj": c["rc"],
+                   "rc_coul": c["rc_coul"], "skin": c["skin"], "dt": c["dt"],
+                   "rebuild_every": c["rebuild_every"], "lbvh_key": "interleave30",
+                   "atoms_per_leaf": 4, "precision": "fp32", "walls": True},
+        "pair_evals_per_s": round(pairs * steps_per_s, 1),
+        "pairs_in_cutoff_per_step": pairs,
+        "list_pairs": st["list_entries"] // 2,
+    }
+    if args.profile_only:
+        print(json.dumps(result), flush=True)
+        return
+
+    # ---- roofline of the fused force + integrate kernel (HIP events per launch) ----
+    sim.kernel_timing(1)
+    sim.run(200)
+    avg_ms, launches = sim.kernel_timing(0)
+    st = sim.stats()
+    plist = st["list_entries"] / 2.0
+    bytes_launch = BYTES_PER_PAIR * plist + BYTES_PER_ATOM * n
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    flops_launch = FLOPS_PER_PAIR_LJC * pairs
+    result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                          "traffic": None, "kernel": "k_step<float,float,float,1,0>",
+                          "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+                          "algorithmic_bytes_per_launch": int(bytes_launch),
+                          "fp32_tflops": round(flops_launch / (avg_ms * 1e-3) / 1e12, 2),
+                          "fp32_frac": round(flops_launch / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)}
+    log("force kernel: %.4f ms avg over %d launches, %.0f GB/s algorithmic" %
+        (avg_ms, launches, achieved))
+    sim.close()
+
+    # ---- precision forms and the reference-key form ----
+    if not args.no_extra:
+        extra = {}
+        k = min(args.steps, 300)
+        for prec in ("mixed", "fp64"):
+            s2, ms2, _, _ = gpu_form(pkg, c, prec, pkg.MORTON_INTERLEAVE, 20, k)
+            extra[prec + "_steps_per_s"] = round(k / (ms2 / 1e3), 3)
+            s2.close()
+            log("%s: %.3f ms/step" % (prec, ms2 / k))
+        extra["mixed_over_fp64"] = round(extra["mixed_steps_per_s"] / extra["fp64_steps_per_s"], 3)
+        s2, ms2, _, _ = gpu_form(pkg, c, "fp32", pkg.MORTON_MASKED, 10, 30)
+        extra["fp32_masked_key_steps_per_s"] = round(30 / (ms2 / 1e3), 3)
+        s2.close()
+        log("fp32 masked key: %.3f ms/step" % (ms2 / 30))
+        result["forms"] = extra
+
+    # ---- CPU baseline: the reference's threaded CPU path (C restatement) ----
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        wall, times, pe = cpu_baseline(c, args.cpu_steps, threads)
+        cpu_sps = args.cpu_steps / wall
+        result["cpu_baseline"] = {
+            "value": round(cpu_sps, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "ljc-262144, %d steps incl. %d list build(s) with the masked reference "
+                      "key, %d threads; wall %.2f s (build %.2f s: keys+sort %.2f, tree %.2f, "
+                      "traversal %.2f; forces+integration %.2f s)" % (
+                          args.cpu_steps, (args.cpu_steps + c["rebuild_every"] - 1) // c["rebuild_every"],
+                          threads, wall, times[0] + times[1] + times[2], times[0], times[1],
+                          times[2], times[3]),
+            "pair_evals_per_s": round(pe / wall, 1)}
+        result["gpu_over_cpu"] = round(steps_per_s / cpu_sps, 1)
+        log("cpu baseline: %.4f steps/s on %d threads" % (cpu_sps, threads))
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

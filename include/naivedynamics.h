@@ -1,0 +1,229 @@
+/*
+ * naivedynamics.h -- C ABI of libnaivehip.so, the gfx950 HIP implementation of
+ * NaiveDynamics.jl's per-step hot path (LBVH neighbour search, Lennard-Jones +
+ * Coulomb pair forces, velocity Verlet).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  "d_" arguments are device pointers
+ *    (allocate them with nd_malloc or any HIP allocator on the context's
+ *    device); "h_" arguments are host pointers.
+ *  - Every call returns an nd_status; on failure nd_last_error() describes it.
+ *    A Julia caller turns a non-zero status into error(), which is how the
+ *    reference reports bad input (BVHTraverse.jl:74-80, InexactError at :266).
+ *  - Atom and pair indices are 0-based.  Tree node numbers keep the
+ *    reference's 1-based GridKey numbering (leaves 1..nL, internal nodes
+ *    nL+1..2nL-1, root nL+1, 0 = sentinel); node k is stored in slot k-1.
+ *  - Calls on one context are stream-ordered on the context's HIP stream and
+ *    are not reentrant; use one context per host thread.  Calls documented as
+ *    synchronous return after the work has finished.
+ */
+#ifndef NAIVEDYNAMICS_H
+#define NAIVEDYNAMICS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ND_OK = 0,
+    ND_ERR_ARG = -1,      /* invalid argument or specification */
+    ND_ERR_DOMAIN = -2,   /* coordinate outside the quantisation domain / non-finite */
+    ND_ERR_CAPACITY = -3, /* output buffer too small; the required count is returned */
+    ND_ERR_HIP = -4,      /* HIP runtime failure */
+    ND_ERR_STATE = -5     /* call order violated (e.g. no tree built yet) */
+} nd_status;
+
+typedef enum { ND_PRED_SCALAR = 0, ND_PRED_SIMD = 1 } nd_predicate;
+typedef enum { ND_MORTON_MASKED = 0, ND_MORTON_INTERLEAVE = 1 } nd_morton_mode;
+typedef enum { ND_PREC_FP32 = 0, ND_PREC_MIXED = 1, ND_PREC_FP64 = 2 } nd_precision;
+
+/* GridKey{Float32,Int32} (BVHTraverse.jl:102-109), same 32-byte layout. */
+typedef struct {
+    float min[3];
+    float max[3];
+    int32_t left;
+    int32_t skip;
+} nd_node;
+
+/* Tuple{Int32,Int32,Float32} = (i, j, d) (BVHTraverse.jl:1314), 12 bytes. */
+typedef struct {
+    int32_t i;
+    int32_t j;
+    float d;
+} nd_pair;
+
+/* SpheresBVHSpecs{Float32,Int32} (BVHTraverse.jl:63-69). */
+typedef struct {
+    float neighbor_distance;
+    int32_t atom_count;
+    int32_t leaves_count;
+    int32_t branches_count;
+    int32_t atomsperleaf;
+} nd_bvh_spec;
+
+typedef struct nd_ctx nd_ctx;
+
+/* ---- context, device memory, timing (no reference counterpart: the Julia
+ *      shim needs them to stage host arrays, as NaiveKA.jl:507-510 does with
+ *      Adapt) -------------------------------------------------------------- */
+int nd_ctx_create(int32_t device, nd_ctx **out);
+int nd_ctx_destroy(nd_ctx *ctx);
+const char *nd_last_error(const nd_ctx *ctx);
+int nd_version(void);
+int nd_malloc(nd_ctx *ctx, size_t bytes, void **d_ptr);
+int nd_free(nd_ctx *ctx, void *d_ptr);
+int nd_memcpy_h2d(nd_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int nd_memcpy_d2h(nd_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int nd_memcpy_d2d(nd_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+int nd_memset(nd_ctx *ctx, void *d_dst, int32_t byte_value, size_t bytes);
+int nd_synchronize(nd_ctx *ctx);
+/* HIP events on the context's stream: nd_timer_start records, nd_timer_stop
+ * records, synchronises and returns the elapsed milliseconds. */
+int nd_timer_start(nd_ctx *ctx);
+int nd_timer_stop(nd_ctx *ctx, float *ms);
+
+/* ---- specification ------------------------------------------------------
+ * SpheresBVHSpecs(; neighbor_distance, atom_count, floattype=Float32,
+ * atomsperleaf) (BVHTraverse.jl:71-94): ND_ERR_ARG unless atomsperleaf
+ * divides atom_count and there are at least two leaves. */
+int nd_bvh_spec_init(nd_bvh_spec *spec, float neighbor_distance, int64_t atom_count,
+                     int64_t atomsperleaf);
+
+/* ---- neighbour-search stages (synchronous) ------------------------------ */
+
+/* mortoncodes! (BVHTraverse.jl:257-288).  ND_MORTON_MASKED reproduces the
+ * reference key q_d = floor(x_d / Float32(1/1023)), code = OR_d (q_d & M_d)
+ * bit for bit and fails with ND_ERR_DOMAIN where the reference throws
+ * InexactError.  ND_MORTON_INTERLEAVE is this library's 30-bit interleaved key
+ * over the box [lo, lo + 1024/scale) (lo3/scale3 may be NULL: the bounding
+ * box of the input is used).  d_xyz is N x 3 float (AoS). */
+int nd_mortoncodes(nd_ctx *ctx, const float *d_xyz, int32_t n, int32_t mode,
+                   const float *h_lo3, const float *h_scale3, int32_t *d_codes);
+
+/* sortperm(codes) + permute! (BVHTraverse.jl:570-573): stable ascending sort
+ * of signed Int32 keys; d_perm[s] = original index of sorted slot s. */
+int nd_sortperm(nd_ctx *ctx, const int32_t *d_codes, int32_t n, int32_t *d_codes_sorted,
+                int32_t *d_perm);
+
+/* leafcluster_primitives + bounding_volume_hierarchy!
+ * (BVHTraverse.jl:376-410, 704-860): leaf boxes of +-neighbor_distance/2 around
+ * the sorted atoms, Apetrei bottom-up build with ropes, exact box refit.
+ * d_nodes has 2*leaves_count-1 entries.  override_root=1 writes the root box
+ * as [0,1]^3 as the reference does (BVHTraverse.jl:856-858). */
+int nd_bounding_volume_hierarchy(nd_ctx *ctx, const float *d_xyz_sorted,
+                                 const int32_t *d_codes_sorted, const nd_bvh_spec *spec,
+                                 int32_t override_root, nd_node *d_nodes);
+
+/* leafneighbor_traverse (BVHTraverse.jl:1236-1323) with ND_PRED_SCALAR
+ * (s < r*r, :1021-1055) or simdneighbor_traverse (NaiveSIMD.jl:610-704) with
+ * ND_PRED_SIMD (sqrt(s) < r).  Pairs are (original index of the query-leaf
+ * atom, original index of the other atom, Float32 distance), in the order a
+ * single-threaded reference run emits them.  If cap < required, returns
+ * ND_ERR_CAPACITY with *count = required and writes nothing. */
+int nd_leafneighbor_traverse(nd_ctx *ctx, const nd_node *d_nodes, const float *d_xyz_sorted,
+                             const int32_t *d_index_sorted, const nd_bvh_spec *spec,
+                             int32_t predicate, nd_pair *d_out, int64_t cap, int64_t *count);
+
+/* ---- one-call neighbour list ---------------------------------------------
+ * leafbuild_traverse_bvh (BVHTraverse.jl:1423-1428) / simdbuild_traverse_bvh
+ * (NaiveSIMD.jl:706-710) / gpubvh_neighborlist (NaiveKA.jl:470-558): Morton,
+ * sort, tree, traversal on device.  Capacity protocol as above. The tree of the
+ * last call stays on the device for nd_tree_export. */
+int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
+                              int32_t predicate, int32_t morton_mode, nd_pair *d_out,
+                              int64_t cap, int64_t *count);
+/* Host-buffer form of the same call (stages h_xyz in and pairs out). */
+int nd_leafbuild_traverse_bvh_host(nd_ctx *ctx, const float *h_xyz, const nd_bvh_spec *spec,
+                                   int32_t predicate, int32_t morton_mode, nd_pair *h_out,
+                                   int64_t cap, int64_t *count);
+/* treedata of gpubvh_neighborlist (NaiveKA.jl:557) / leafTreeData
+ * (BVHTraverse.jl:596): nodes (2nL-1), sort permutation and sorted codes of
+ * the last nd_leafbuild_traverse_bvh call, copied to host (any may be NULL). */
+int nd_tree_export(nd_ctx *ctx, nd_node *h_nodes, int32_t *h_perm, int32_t *h_codes_sorted);
+
+/* threshold_pairs(unique_pairs(x), threshold) (AllToAll.jl:65-88): all i<j
+ * with sqrt(s) < threshold, in (i, j) lexicographic order.  O(N^2). */
+int nd_unique_threshold_pairs(nd_ctx *ctx, const float *d_xyz, int32_t n, float threshold,
+                              nd_pair *d_out, int64_t cap, int64_t *count);
+
+/* ---- forces over a pair list --------------------------------------------- */
+typedef struct {
+    float eps, sigma; /* Lennard-Jones */
+    float rc;         /* LJ cutoff */
+    float k_coul;     /* Coulomb constant */
+    float rc_coul;    /* Coulomb cutoff */
+    int32_t shift_lj, shift_coul; /* energy shift at the cutoff */
+} nd_force_params;
+
+/* force_lennardjones!(force, pairslist, position) (Forces.jl:15-45): zeroes
+ * d_force (N x 3 float) and adds the textbook LJ force of every listed pair
+ * with r < rc to both atoms (Newton's third law).  *h_energy (may be NULL)
+ * receives the pair energy in FP64. */
+int nd_force_lennardjones(nd_ctx *ctx, float *d_force, const nd_pair *d_pairs, int64_t npairs,
+                          const float *d_xyz, int32_t n, const nd_force_params *p,
+                          double *h_energy);
+/* force_coulomb!(force, pairslist, charge) (Forces.jl:56-66): textbook cutoff
+ * Coulomb k q_i q_j r_ij / r^3; positions are needed for the direction. */
+int nd_force_coulomb(nd_ctx *ctx, float *d_force, const nd_pair *d_pairs, int64_t npairs,
+                     const float *d_xyz, const float *d_charge, int32_t n,
+                     const nd_force_params *p, double *h_energy);
+/* sum_forces!(force, f1, f2) (Forces.jl:68-74). */
+int nd_sum_forces(nd_ctx *ctx, float *d_force, const float *d_f1, const float *d_f2, int32_t n);
+
+/* ---- simulation (simulate! / simulate_bvh!, Simulator.jl:154-256, 327-379) */
+typedef struct {
+    int32_t n;            /* atoms */
+    int32_t precision;    /* nd_precision */
+    int32_t morton_mode;  /* key for the simulation's LBVH (nd_morton_mode) */
+    int32_t atomsperleaf; /* LBVH leaf size (must divide n) */
+    int32_t rebuild_every;/* neighbour-list rebuild period in steps (>= 1) */
+    int32_t walls;        /* 1: reflecting walls at [lo, hi]^3 (boundary_reflect!) */
+    int32_t energy_every; /* record (KE, PE, virial, pairs) every k steps; 0 = never */
+    int32_t thermostat_every; /* velocity rescale period (rescale_velocity!); 0 = off */
+    int32_t use_graph;    /* 1: replay each rebuild window as a captured hipGraph */
+    int32_t max_neighbors;/* initial per-atom list capacity (0 = estimate) */
+    double eps, sigma, rc; /* LJ */
+    double k_coul, rc_coul;/* Coulomb (rc_coul = 0 disables) */
+    double skin;           /* list radius = max(rc, rc_coul) + skin */
+    double dt;
+    double lo, hi;         /* box for walls and the interleaved key */
+    double temperature, vdamp; /* thermostat target and damping gamma */
+} nd_sim_params;
+
+typedef struct {
+    int64_t steps;            /* steps integrated so far */
+    int64_t rebuilds;         /* neighbour-list builds so far */
+    int64_t list_entries;     /* directed entries in the current list */
+    int32_t max_neighbors;    /* largest per-atom count at the last build */
+    int32_t capacity;         /* current per-atom capacity */
+    int64_t pairs_in_cutoff;  /* unique pairs with r < cutoff at the last energy step */
+} nd_sim_stats;
+
+typedef struct nd_sim nd_sim;
+
+int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out);
+int nd_sim_destroy(nd_sim *sim);
+/* State in original atom order, FP64 host arrays (x, v: N x 3; q, m: N; q or m
+ * may be NULL for 0 and 1). */
+int nd_sim_set_state(nd_sim *sim, const double *h_x, const double *h_v, const double *h_q,
+                     const double *h_m);
+int nd_sim_get_state(nd_sim *sim, double *h_x, double *h_v, double *h_f);
+/* Advance nsteps velocity-Verlet steps (kick-drift-force-kick), rebuilding
+ * the neighbour list every rebuild_every steps.  Returns after completion. */
+int nd_sim_run(nd_sim *sim, int32_t nsteps);
+/* Energy rows (step, kinetic, potential, virial, pairs_in_cutoff) recorded
+ * since the last call; *rows receives the number written. */
+int nd_sim_energies(nd_sim *sim, double *h_rows, int32_t max_rows, int32_t *rows);
+int nd_sim_stats_get(nd_sim *sim, nd_sim_stats *out);
+/* Average device time per launch of the fused force kernel over the last
+ * nd_sim_run, measured with HIP events around each launch when enabled. */
+int nd_sim_kernel_timing(nd_sim *sim, int32_t enable, double *avg_force_ms,
+                         int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+// nd_internal.hpp -- shared declarations of libnaivehip (gfx950 HIP).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "naivedynamics.h"
+
+namespace nd {
+
+constexpr int kBlock = 256;  // 4 wave64s per workgroup
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define ND_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::nd::Error(ND_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define ND_REQUIRE(cond, msg)                                          \
+    do {                                                               \
+        if (!(cond)) throw ::nd::Error(ND_ERR_ARG, std::string(msg)); \
+    } while (0)
+
+// Grow-only device buffer owned by a context or simulation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n) {
+        if (n <= bytes && p) return;
+        if (p) ND_HIP(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        if (n == 0) n = 256;
+        ND_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Status word bits written by kernels (atomicOr).
+enum : int32_t { kStatusDomain = 1, kStatusCapacity = 2, kStatusTree = 4 };
+
+// Scratch for one radix sort of n (key, value) pairs.
+struct SortBufs {
+    DevBuf keys_alt, vals_alt, hist, flags;
+};
+
+}  // namespace nd
+
+struct nd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    nd::DevBuf status;  // int32 status word + int64 counters
+    nd::DevBuf codes, codes_sorted, perm, xyz_sorted, nodes, store, counts, offsets, xyz_in,
+        pairs_out, bbox;
+    nd::SortBufs sort;
+    // last tree built by nd_leafbuild_traverse_bvh
+    int32_t tree_n = 0, tree_apl = 0;
+    bool has_tree = false;
+};
+
+namespace nd {
+
+// ---- launch helpers implemented in the kernel translation units ----
+// All are asynchronous on ctx->stream.
+
+// nd_morton.hip
+void launch_morton(nd_ctx *ctx, const float *xyz, int32_t n, int32_t mode, const float lo[3],
+                   const float scale[3], int32_t *codes, int32_t *idx, int32_t *status);
+void launch_bbox(nd_ctx *ctx, const float *xyz, int32_t n, float *bbox6);
+void launch_fill_i32(hipStream_t s, int32_t *p, int64_t n, int32_t v);
+void launch_iota(hipStream_t s, int32_t *p, int32_t n);
+void radix_sort_pairs_auto(nd_ctx *ctx, int32_t *keys, int32_t *vals, int32_t n);
+void launch_gather_xyz(hipStream_t s, const float *xyz, const int32_t *perm, int32_t n,
+                       float *out);
+
+// nd_sort.hip: stable LSD radix sort of int32 keys (signed order) with int32
+// values; bit range [0, key_bits) of the sign-flipped key is sorted (pass
+// key_bits = 32 for a full sort).  Results land in keys/vals.
+void radix_sort_pairs(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
+                      int32_t first_bit, int32_t end_bit);
+size_t radix_scratch_blocks(int32_t n);
+
+// nd_lbvh.hip
+// status (may be null) receives kStatusTree if the codes break the build's
+// index invariants (only possible for caller-supplied codes with bit 31 set).
+void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
+                 const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
+                 nd_node *nodes, int32_t *store, int32_t *status);
+
+// nd_traverse.hip
+int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
+                      const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t predicate,
+                      nd_pair *out, int64_t cap);
+
+void set_error(nd_ctx *ctx, const std::string &m);
+int32_t read_status(nd_ctx *ctx);  // synchronises; returns and clears the status word
+void clear_status(nd_ctx *ctx);
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+    return (unsigned)((n + block - 1) / block);
+}
+
+}  // namespace nd
+
+namespace nd {
+
+// Float32(1/1023) (BVHTraverse.jl:184): the quotient rounded once to binary32.
+constexpr float kBinwidth = (float)(1.0 / 1023.0);
+// mortoncodes! masks (BVHTraverse.jl:241).
+constexpr int32_t kMagic0 = 0x09249249, kMagic1 = 0x12492492, kMagic2 = 0x24924924;
+
+// Spread the low 10 bits of x to bit positions 0, 3, 6, ..., 27.
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// q = round(Int32, v, RoundDown): defined for -2^31 <= v < 2^31 only.
+__device__ __forceinline__ bool floor_i32(float v, int32_t &q) {
+    if (!(v >= -2147483648.0f && v < 2147483648.0f)) return false;
+    q = (int32_t)floorf(v);
+    return true;
+}
+
+// 10-bit cell coordinate of x in the interleaved key: clamp(floor((x-lo)*s), 0, 1023).
+__device__ __forceinline__ uint32_t cell10(float x, float lo, float s) {
+    float t = (x - lo) * s;
+    if (!(t > 0.0f)) return 0u;  // also maps NaN to 0 (NaN input is flagged)
+    if (t >= 1023.0f) return 1023u;
+    return (uint32_t)floorf(t);
+}
+
+}  // namespace nd
+
+// Julia's Float32 min/max (ordering with -0 < +0; NaN propagates).
+__device__ __forceinline__ float jl_min(float x, float y) {
+    if (__builtin_isnan(x) || __builtin_isnan(y)) return x - y;
+    if (x < y) return x;
+    if (y < x) return y;
+    return __builtin_signbit(x) ? x : y;
+}
+__device__ __forceinline__ float jl_max(float x, float y) {
+    if (__builtin_isnan(x) || __builtin_isnan(y)) return x - y;
+    if (x > y) return x;
+    if (y > x) return y;
+    return __builtin_signbit(x) ? y : x;
+}
+
+#define ND_API_BEGIN try {
+#define ND_API_END(ctx)                                    \
+    }                                                      \
+    catch (const ::nd::Error &e) {                         \
+        if (ctx) ::nd::set_error(ctx, e.what());           \
+        return e.code;                                     \
+    }                                                      \
+    catch (const std::exception &e) {                      \
+        if (ctx) ::nd::set_error(ctx, e.what());           \
+        return ND_ERR_HIP;                                 \
+    }                                                      \
+    return ND_OK;

@@ -1,0 +1,1067 @@
+// nd_sim.hip -- the per-step hot path of simulate! / simulate_bvh!
+// (src/Simulator.jl:154-256, 327-379): LBVH neighbour list every k steps, then
+// one fused kernel per step that evaluates Lennard-Jones + Coulomb pair
+// forces and advances velocity Verlet (kick-drift-force-kick).
+//
+// HBM layout (all per-atom arrays in the current LBVH (Morton) order):
+//   pos[2]  float4 / double4 (x, y, z, charge), double-buffered: a step reads
+//           pos[cur] for every neighbour and writes the drifted atom to pos[alt]
+//   vel[2]  float4 / double4 (vx, vy, vz, 1/m), double-buffered for re-sorting
+//   frc     float4 / double4 force of the last completed step
+//   orig[2] int32 original atom index of each slot
+//   nbr     int32 [cap][n] neighbour slots, transposed so that the k-th
+//           neighbours of 64 consecutive atoms are one contiguous 256-byte row
+//   ncount  int32 [n]
+// Rebuild (every rebuild_every steps): key -> radix sort -> permute state ->
+// LBVH (leaf boxes +-r_list/2, nd_lbvh.hip) -> one lane per atom walks the
+// tree from the root and appends every slot j != i with |x_i - x_j|^2 <
+// r_list^2 to its list (full list: each pair appears in both atoms' lists, so
+// the force kernel needs no atomics and is deterministic).
+//
+// Precision forms: FP32 (float state and pair maths), MIXED (pair maths in
+// FP32 on FP64 coordinate differences, per-atom force / position / velocity
+// accumulation in FP64), FP64 (everything in double).  Energies and the virial
+// are reduced per workgroup in FP64 and summed with Neumaier compensation.
+#include <vector>
+
+#include "nd_internal.hpp"
+
+namespace {
+
+constexpr int kModeForce = 0;  // F(x_n), store F, energy at n
+constexpr int kModeFused = 1;  // F(x_n), finish v_n, [energy], start step: v_{n+1/2}, x_{n+1}
+constexpr int kModeFinal = 2;  // F(x_n), finish v_n, [energy], store F
+
+struct SimMisc {
+    int32_t maxcnt;
+    int32_t overflow;
+    int32_t status;
+    int32_t rowctr;
+    int64_t step;
+    int64_t partial_step;
+    double ke;
+    double pad[3];
+};
+
+struct StepParams {
+    int32_t n, cap, walls, pad;
+    float eps24, eps4, sigma2, rc2, rcc2, rcmax2, kc, elj_shift, ecoul_inv_rc;
+    float dtf, lof, hif;
+    double deps24, deps4, dsigma2, drc2, drcc2, drcmax2, dkc, delj_shift, decoul_inv_rc;
+    double dt, lo, hi;
+};
+
+template <class S> struct V4;
+template <> struct V4<float> { using T = float4; };
+template <> struct V4<double> { using T = double4; };
+
+__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rsq_(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
+
+template <class C> struct PP;
+template <> struct PP<float> {
+    __device__ static float eps24(const StepParams &p) { return p.eps24; }
+    __device__ static float eps4(const StepParams &p) { return p.eps4; }
+    __device__ static float sigma2(const StepParams &p) { return p.sigma2; }
+    __device__ static float rc2(const StepParams &p) { return p.rc2; }
+    __device__ static float rcc2(const StepParams &p) { return p.rcc2; }
+    __device__ static float rcmax2(const StepParams &p) { return p.rcmax2; }
+    __device__ static float kc(const StepParams &p) { return p.kc; }
+    __device__ static float eljs(const StepParams &p) { return p.elj_shift; }
+    __device__ static float ecs(const StepParams &p) { return p.ecoul_inv_rc; }
+};
+template <> struct PP<double> {
+    __device__ static double eps24(const StepParams &p) { return p.deps24; }
+    __device__ static double eps4(const StepParams &p) { return p.deps4; }
+    __device__ static double sigma2(const StepParams &p) { return p.dsigma2; }
+    __device__ static double rc2(const StepParams &p) { return p.drc2; }
+    __device__ static double rcc2(const StepParams &p) { return p.drcc2; }
+    __device__ static double rcmax2(const StepParams &p) { return p.drcmax2; }
+    __device__ static double kc(const StepParams &p) { return p.dkc; }
+    __device__ static double eljs(const StepParams &p) { return p.delj_shift; }
+    __device__ static double ecs(const StepParams &p) { return p.decoul_inv_rc; }
+};
+
+// Sum of 4 doubles over the workgroup; lane 0 of wave 0 holds the result.
+__device__ __forceinline__ void block_sum4(double v[4], double (*sh)[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_down(v[c], off, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh[w][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k][c];
+            v[c] = s;
+        }
+}
+
+// PosS: storage scalar of positions/velocities; CalcS: pair arithmetic;
+// AccS: per-atom accumulation.  FP32 = <float,float,float>,
+// MIXED = <double,float,double>, FP64 = <double,double,double>.
+template <class PosS, class CalcS, class AccS, int MODE, bool ENERGY>
+__global__ void __launch_bounds__(nd::kBlock)
+k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
+       typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
+       const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, StepParams P,
+       double *__restrict__ partials, SimMisc *__restrict__ misc) {
+    using P4 = typename V4<PosS>::T;
+    using Q = PP<CalcS>;
+    __shared__ double sh[nd::kBlock / 64][4];
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t n = P.n;
+    AccS fx = 0, fy = 0, fz = 0;
+    double pe = 0.0, vir = 0.0, ke = 0.0, np = 0.0;
+    if (i < n) {
+        const P4 pi = pos_in[i];
+        const CalcS qi = (CalcS)pi.w;
+        const CalcS rcmax2 = Q::rcmax2(P), rc2 = Q::rc2(P), rcc2 = Q::rcc2(P);
+        const CalcS sigma2 = Q::sigma2(P), eps24 = Q::eps24(P), eps4 = Q::eps4(P), kc = Q::kc(P);
+        int32_t cnt = ncount[i];
+        cnt = cnt < P.cap ? cnt : P.cap;
+        AccS epair = 0, vpair = 0;  // FP64 sums in the MIXED and FP64 forms
+        int32_t npair = 0;
+        const int32_t *col = nbr + i;
+#pragma unroll 4
+        for (int32_t k = 0; k < cnt; ++k) {
+            const int32_t j = col[(int64_t)k * n];
+            const P4 pj = pos_in[j];
+            const CalcS dx = (CalcS)(pi.x - pj.x);
+            const CalcS dy = (CalcS)(pi.y - pj.y);
+            const CalcS dz = (CalcS)(pi.z - pj.z);
+            const CalcS r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < rcmax2) {
+                const CalcS ir2 = rcp_(r2);
+                CalcS fs = 0;
+                if (r2 < rc2) {
+                    const CalcS sr2 = sigma2 * ir2;
+                    const CalcS sr6 = sr2 * sr2 * sr2;
+                    fs = eps24 * ir2 * sr6 * (sr6 + sr6 - (CalcS)1);
+                    if (ENERGY) epair += (AccS)(eps4 * sr6 * (sr6 - (CalcS)1) - Q::eljs(P));
+                }
+                if (r2 < rcc2) {
+                    const CalcS qq = kc * qi * (CalcS)pj.w;
+                    const CalcS ir = rsq_(r2);
+                    fs += qq * ir * ir2;
+                    if (ENERGY) epair += (AccS)(qq * (ir - Q::ecs(P)));
+                }
+                fx += (AccS)(fs * dx);
+                fy += (AccS)(fs * dy);
+                fz += (AccS)(fs * dz);
+                if (ENERGY) {
+                    vpair += (AccS)(fs * r2);
+                    npair += 1;
+                }
+            }
+        }
+        // each unordered pair is visited from both atoms: halve the pair sums
+        pe = 0.5 * (double)epair;
+        vir = 0.5 * (double)vpair;
+        np = 0.5 * (double)npair;
+
+        if (MODE == kModeForce || MODE == kModeFinal) {
+            P4 f;
+            f.x = (PosS)fx; f.y = (PosS)fy; f.z = (PosS)fz; f.w = 0;
+            frc[i] = f;
+        }
+        if (MODE != kModeForce) {
+            P4 v = vel[i];
+            const PosS h = (PosS)0.5 * (sizeof(PosS) == 4 ? (PosS)P.dtf : (PosS)P.dt) * v.w;
+            v.x += h * (PosS)fx;
+            v.y += h * (PosS)fy;
+            v.z += h * (PosS)fz;
+            if (ENERGY) ke = 0.5 / (double)v.w * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+            if (MODE == kModeFused) {
+                const PosS dt = sizeof(PosS) == 4 ? (PosS)P.dtf : (PosS)P.dt;
+                v.x += h * (PosS)fx;
+                v.y += h * (PosS)fy;
+                v.z += h * (PosS)fz;
+                P4 x = pi;
+                x.x += dt * v.x;
+                x.y += dt * v.y;
+                x.z += dt * v.z;
+                if (P.walls) {
+                    const PosS lo = sizeof(PosS) == 4 ? (PosS)P.lof : (PosS)P.lo;
+                    const PosS hi = sizeof(PosS) == 4 ? (PosS)P.hif : (PosS)P.hi;
+                    // boundary_reflect! (Simulator.jl:81-111)
+                    if (x.x < lo) { x.x = lo; v.x = -v.x; }
+                    if (x.x > hi) { x.x = hi; v.x = -v.x; }
+                    if (x.y < lo) { x.y = lo; v.y = -v.y; }
+                    if (x.y > hi) { x.y = hi; v.y = -v.y; }
+                    if (x.z < lo) { x.z = lo; v.z = -v.z; }
+                    if (x.z > hi) { x.z = hi; v.z = -v.z; }
+                }
+                pos_out[i] = x;
+            }
+            vel[i] = v;
+        } else if (ENERGY) {
+            const P4 v = vel[i];
+            ke = 0.5 / (double)v.w * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+        }
+    }
+    if (ENERGY) {
+        double acc[4] = {pe, vir, ke, np};
+        block_sum4(acc, sh);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) partials[4 * (int64_t)blockIdx.x + c] = acc[c];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (ENERGY) misc->partial_step = misc->step;
+        if (MODE == kModeFused) misc->step += 1;
+    }
+}
+
+// Kick-drift without a force evaluation (first step of a run, or after the
+// thermostat): v += dt/2 F/m; x += dt v; walls.  Optional velocity scale
+// beta = sqrt(1 + gamma (T0/T - 1)) from the last recorded kinetic energy.
+template <class PosS>
+__global__ void __launch_bounds__(nd::kBlock)
+k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
+             typename V4<PosS>::T *__restrict__ vel, const typename V4<PosS>::T *__restrict__ frc,
+             StepParams P, SimMisc *__restrict__ misc, int32_t rescale, double t0, double gamma) {
+    using P4 = typename V4<PosS>::T;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) {
+        P4 v = vel[i];
+        if (rescale) {
+            // rescale_velocity! (Simulator.jl:119-144) with T = 2 KE / (3 N k_B)
+            const double T = 2.0 * misc->ke / (3.0 * (double)P.n);
+            const double beta = T > 0.0 ? sqrt(fmax(0.0, 1.0 + gamma * (t0 / T - 1.0))) : 1.0;
+            v.x = (PosS)(v.x * beta);
+            v.y = (PosS)(v.y * beta);
+            v.z = (PosS)(v.z * beta);
+        }
+        const P4 f = frc[i];
+        const PosS dt = sizeof(PosS) == 4 ? (PosS)P.dtf : (PosS)P.dt;
+        const PosS h = (PosS)0.5 * dt * v.w;
+        v.x += h * f.x;
+        v.y += h * f.y;
+        v.z += h * f.z;
+        P4 x = pos_in[i];
+        x.x += dt * v.x;
+        x.y += dt * v.y;
+        x.z += dt * v.z;
+        if (P.walls) {
+            const PosS lo = sizeof(PosS) == 4 ? (PosS)P.lof : (PosS)P.lo;
+            const PosS hi = sizeof(PosS) == 4 ? (PosS)P.hif : (PosS)P.hi;
+            if (x.x < lo) { x.x = lo; v.x = -v.x; }
+            if (x.x > hi) { x.x = hi; v.x = -v.x; }
+            if (x.y < lo) { x.y = lo; v.y = -v.y; }
+            if (x.y > hi) { x.y = hi; v.y = -v.y; }
+            if (x.z < lo) { x.z = lo; v.z = -v.z; }
+            if (x.z > hi) { x.z = hi; v.z = -v.z; }
+        }
+        pos_out[i] = x;
+        vel[i] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) misc->step += 1;
+}
+
+// Neumaier-compensated sum of the per-workgroup partials; appends one energy
+// row (step, KE, PE, virial, pairs) and keeps KE for the thermostat.
+__global__ void __launch_bounds__(nd::kBlock)
+k_energy_finalize(const double *__restrict__ partials, int32_t nblocks, double *__restrict__ rows,
+                  int32_t max_rows, SimMisc *__restrict__ misc) {
+    __shared__ double ss[nd::kBlock][4], cc[nd::kBlock][4];
+    double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+    for (int32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double x = partials[4 * (int64_t)b + q];
+            const double t = s[q] + x;
+            c[q] += fabs(s[q]) >= fabs(x) ? (s[q] - t) + x : (x - t) + s[q];
+            s[q] = t;
+        }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { ss[threadIdx.x][q] = s[q]; cc[threadIdx.x][q] = c[q]; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double tot[4];
+    for (int q = 0; q < 4; ++q) {
+        double S = 0.0, C = 0.0;
+        for (int k = 0; k < (int)blockDim.x; ++k) {
+            const double x = ss[k][q];
+            const double t = S + x;
+            C += fabs(S) >= fabs(x) ? (S - t) + x : (x - t) + S;
+            S = t;
+            C += cc[k][q];
+        }
+        tot[q] = S + C;
+    }
+    const int32_t r = misc->rowctr;
+    if (r < max_rows) {
+        double *row = rows + 5 * (int64_t)r;
+        row[0] = (double)misc->partial_step;
+        row[1] = tot[2];
+        row[2] = tot[0];
+        row[3] = tot[1];
+        row[4] = tot[3];
+    }
+    misc->rowctr = r + 1;
+    misc->ke = tot[2];
+}
+
+// Key of every atom from its (possibly FP64) position rounded to FP32.
+template <class PosS>
+__global__ void __launch_bounds__(nd::kBlock)
+k_sim_keys(const typename V4<PosS>::T *__restrict__ pos, int32_t n, int32_t mode, float lo,
+           float scale, int32_t *__restrict__ codes, int32_t *__restrict__ idx,
+           SimMisc *__restrict__ misc) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { misc->maxcnt = 0; }
+    if (i >= n) return;
+    const auto p = pos[i];
+    const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
+    int32_t code;
+    bool ok;
+    if (mode == ND_MORTON_MASKED) {
+        int32_t qx = 0, qy = 0, qz = 0;
+        ok = nd::floor_i32(x / nd::kBinwidth, qx);
+        ok &= nd::floor_i32(y / nd::kBinwidth, qy);
+        ok &= nd::floor_i32(z / nd::kBinwidth, qz);
+        code = (qx & nd::kMagic0) | (qy & nd::kMagic1) | (qz & nd::kMagic2);
+    } else {
+        ok = __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
+        code = (int32_t)(nd::spread10(nd::cell10(x, lo, scale)) |
+                         (nd::spread10(nd::cell10(y, lo, scale)) << 1) |
+                         (nd::spread10(nd::cell10(z, lo, scale)) << 2));
+    }
+    codes[i] = code;
+    idx[i] = i;
+    if (!ok) atomicOr(&misc->status, nd::kStatusDomain);
+}
+
+// Gather the state into the new order; also writes the FP32 copy of the
+// positions that the tree build and traversal read.
+template <class PosS>
+__global__ void __launch_bounds__(nd::kBlock)
+k_permute(const int32_t *__restrict__ perm, int32_t n, const typename V4<PosS>::T *__restrict__ pos_in,
+          typename V4<PosS>::T *__restrict__ pos_out, const typename V4<PosS>::T *__restrict__ vel_in,
+          typename V4<PosS>::T *__restrict__ vel_out, const int32_t *__restrict__ orig_in,
+          int32_t *__restrict__ orig_out, float4 *__restrict__ pos32) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int32_t o = perm[s];
+    const auto p = pos_in[o];
+    pos_out[s] = p;
+    vel_out[s] = vel_in[o];
+    orig_out[s] = orig_in[o];
+    if (pos32) pos32[s] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+}
+
+// One lane per atom: stackless walk from the root over the +-r/2 boxes,
+// appending every slot j != i with |x_i - x_j|^2 < r^2 to column i.
+__global__ void __launch_bounds__(nd::kBlock)
+k_nlist(const float4 *__restrict__ pos, const nd_node *__restrict__ nodes, int32_t n, int32_t nL,
+        int32_t apl, float h, float rl2, int32_t cap, int32_t *__restrict__ nbr,
+        int32_t *__restrict__ ncount, SimMisc *__restrict__ misc) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t cnt = 0;
+    if (i < n) {
+        const float4 p = pos[i];
+        const float qmn0 = p.x - h, qmn1 = p.y - h, qmn2 = p.z - h;
+        const float qmx0 = p.x + h, qmx1 = p.y + h, qmx2 = p.z + h;
+        int32_t t = nL + 1;  // root
+        int32_t visits = 0;
+        while (t != 0) {
+            // each node is visited at most once per query (DFS order)
+            if (++visits > 2 * nL || t < 1 || t > 2 * nL - 1) {
+                atomicOr(&misc->status, nd::kStatusTree);
+                break;
+            }
+            const float4 *np4 = reinterpret_cast<const float4 *>(nodes + (t - 1));
+            const float4 a = np4[0];
+            const float4 b = np4[1];
+            const bool ov = (qmn0 < a.w) & (qmx0 > a.x) & (qmn1 < b.x) & (qmx1 > a.y) &
+                            (qmn2 < b.y) & (qmx2 > a.z);
+            const int32_t left = __float_as_int(b.z), skip = __float_as_int(b.w);
+            if (ov) {
+                if (left == 0) {
+                    const int32_t base = (t - 1) * apl;
+                    for (int32_t m = 0; m < apl; ++m) {
+                        const int32_t j = base + m;
+                        const float4 q = pos[j];
+                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                        const float s = dx * dx + dy * dy + dz * dz;
+                        if (j != i && s < rl2) {
+                            if (cnt < cap) nbr[(int64_t)cnt * n + i] = j;
+                            ++cnt;
+                        }
+                    }
+                    t = skip;
+                } else {
+                    t = left;
+                }
+            } else {
+                t = skip;
+            }
+        }
+        ncount[i] = cnt;
+    }
+    // wave-level max, one atomic per wave
+    int32_t m = cnt;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0) {
+        atomicMax(&misc->maxcnt, m);
+        if (m > cap) atomicOr(&misc->overflow, 1);
+    }
+}
+
+__global__ void k_misc_reset_counters(SimMisc *misc) {
+    if (threadIdx.x == 0) { misc->overflow = 0; misc->status = 0; misc->maxcnt = 0; }
+}
+
+}  // namespace
+
+struct nd_sim {
+    nd_ctx *ctx = nullptr;
+    nd_sim_params p{};
+    int32_t n = 0, nL = 0, apl = 0;
+    bool wide = false;  // FP64 storage (MIXED, FP64)
+    nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, codes, perm, nodes, store, nbr, ncount, misc,
+        partials, rows, snap_pos, snap_vel, snap_frc, snap_orig;
+    nd::SortBufs sort;
+    int cur = 0;   // position buffer (flips every drift and every re-sort)
+    int vcur = 0;  // velocity / original-index buffer (flips only on a re-sort)
+    int32_t cap = 0;
+    int32_t max_rows = 1 << 16;
+    int64_t step = 0, rebuilds = 0, last_build_step = -1;
+    bool have_force = false, list_valid = false, have_state = false;
+    float rlist = 0.0f;
+    StepParams sp{};
+    // graphs of one rebuild window, per buffer parity
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+    // kernel timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double force_ms = 0.0;
+    int64_t force_launches = 0;
+    int64_t last_list_entries = 0;
+    int32_t last_maxcnt = 0;
+    int64_t pairs_last = 0;
+};
+
+namespace {
+
+size_t pos_bytes(const nd_sim *s) { return (s->wide ? sizeof(double4) : sizeof(float4)) * (size_t)s->n; }
+
+void drop_graphs(nd_sim *s) {
+    for (auto &g : s->gexec)
+        if (g) { hipGraphExecDestroy(g); g = nullptr; }
+}
+
+void alloc_lists(nd_sim *s, int32_t cap) {
+    s->cap = cap;
+    s->sp.cap = cap;
+    s->nbr.reserve(sizeof(int32_t) * (size_t)cap * (size_t)s->n);
+    drop_graphs(s);
+}
+
+hipEvent_t next_event(nd_sim *s) {
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t e;
+        ND_HIP(hipEventCreate(&e));
+        s->ev_pool.push_back(e);
+    }
+    return s->ev_pool[s->ev_used++];
+}
+
+// ---- launch sequences (asynchronous) ----
+
+void launch_rebuild(nd_sim *s) {
+    hipStream_t st = s->ctx->stream;
+    const int32_t n = s->n;
+    const unsigned g = nd::grid_for(n);
+    SimMisc *misc = s->misc.as<SimMisc>();
+    const float lo = s->p.lo;
+    const float scale = s->p.hi > s->p.lo ? 1024.0f / (s->p.hi - s->p.lo) : 0.0f;
+    int32_t *codes = s->codes.as<int32_t>(), *perm = s->perm.as<int32_t>();
+    if (s->wide)
+        k_sim_keys<double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), n, s->p.morton_mode,
+                                                     lo, scale, codes, perm, misc);
+    else
+        k_sim_keys<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), n, s->p.morton_mode,
+                                                    lo, scale, codes, perm, misc);
+    // Both keys are < 2^30, so bits 30..31 of the sign-flipped key are constant.
+    nd::radix_sort_pairs(st, s->sort, codes, perm, n, 0, 30);
+    const int a = s->cur ^ 1, va = s->vcur ^ 1;
+    if (s->wide)
+        k_permute<double><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
+                                                    s->vel[s->vcur].as<double4>(), s->vel[va].as<double4>(),
+                                                    s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),
+                                                    s->pos32.as<float4>());
+    else
+        k_permute<float><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<float4>(), s->pos[a].as<float4>(),
+                                                   s->vel[s->vcur].as<float4>(), s->vel[va].as<float4>(),
+                                                   s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),
+                                                   nullptr);
+    s->cur = a;
+    s->vcur = va;
+    const float *p32 = s->wide ? reinterpret_cast<const float *>(s->pos32.p)
+                               : reinterpret_cast<const float *>(s->pos[s->cur].p);
+    nd::launch_lbvh(st, p32, 4, codes, s->nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
+                    s->store.as<int32_t>(), &misc->status);
+    k_nlist<<<g, nd::kBlock, 0, st>>>(reinterpret_cast<const float4 *>(p32), s->nodes.as<nd_node>(), n,
+                                      s->nL, s->apl, s->rlist / 2.0f, s->rlist * s->rlist, s->cap,
+                                      s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), misc);
+    ND_HIP(hipGetLastError());
+}
+
+template <class PosS, class CalcS, class AccS, int MODE, bool E>
+void launch_step_t(nd_sim *s) {
+    using P4 = typename V4<PosS>::T;
+    hipStream_t st = s->ctx->stream;
+    const unsigned g = nd::grid_for(s->n);
+    const int a = s->cur ^ 1;
+    k_step<PosS, CalcS, AccS, MODE, E><<<g, nd::kBlock, 0, st>>>(
+        s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
+        s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
+        s->misc.as<SimMisc>());
+}
+
+template <int MODE, bool E>
+void launch_step_prec(nd_sim *s) {
+    switch (s->p.precision) {
+        case ND_PREC_FP32: launch_step_t<float, float, float, MODE, E>(s); break;
+        case ND_PREC_MIXED: launch_step_t<double, float, double, MODE, E>(s); break;
+        default: launch_step_t<double, double, double, MODE, E>(s); break;
+    }
+}
+
+// One force-evaluating kernel for global step k, plus its energy row.
+void launch_force_step(nd_sim *s, int mode, int64_t k) {
+    const bool e = s->p.energy_every > 0 && (k % s->p.energy_every) == 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s->timing) {
+        e0 = next_event(s);
+        e1 = next_event(s);
+        ND_HIP(hipEventRecord(e0, s->ctx->stream));
+    }
+    if (mode == kModeForce) e ? launch_step_prec<kModeForce, true>(s) : launch_step_prec<kModeForce, false>(s);
+    else if (mode == kModeFused) e ? launch_step_prec<kModeFused, true>(s) : launch_step_prec<kModeFused, false>(s);
+    else e ? launch_step_prec<kModeFinal, true>(s) : launch_step_prec<kModeFinal, false>(s);
+    ND_HIP(hipGetLastError());
+    if (s->timing) ND_HIP(hipEventRecord(e1, s->ctx->stream));
+    if (mode == kModeFused) s->cur ^= 1;
+    if (e) {
+        k_energy_finalize<<<1, nd::kBlock, 0, s->ctx->stream>>>(s->partials.as<double>(),
+                                                                (int32_t)nd::grid_for(s->n),
+                                                                s->rows.as<double>(), s->max_rows,
+                                                                s->misc.as<SimMisc>());
+        ND_HIP(hipGetLastError());
+    }
+}
+
+void launch_kick_drift(nd_sim *s, bool rescale) {
+    hipStream_t st = s->ctx->stream;
+    const unsigned g = nd::grid_for(s->n);
+    const int a = s->cur ^ 1;
+    if (s->wide)
+        k_kick_drift<double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
+                                                       s->vel[s->vcur].as<double4>(), s->frc.as<double4>(), s->sp,
+                                                       s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp);
+    else
+        k_kick_drift<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), s->pos[a].as<float4>(),
+                                                      s->vel[s->vcur].as<float4>(), s->frc.as<float4>(), s->sp,
+                                                      s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp);
+    ND_HIP(hipGetLastError());
+    s->cur ^= 1;
+}
+
+bool thermo_step(const nd_sim *s, int64_t k) {
+    return s->p.thermostat_every > 0 && k > 0 && (k % s->p.thermostat_every) == 0;
+}
+
+// Mid-step state (x_k, v_{k-1/2}) -> (x_{k+1}, v_{k+1/2}) for step index k.
+void launch_middle_step(nd_sim *s, int64_t k) {
+    if (thermo_step(s, k)) {
+        // needs KE(v_k) before the kick: final-mode force, energy row, rescale + kick-drift
+        const int32_t ee = s->p.energy_every;
+        s->p.energy_every = 1;
+        launch_force_step(s, kModeFinal, k);
+        s->p.energy_every = ee;
+        launch_kick_drift(s, true);
+    } else {
+        launch_force_step(s, kModeFused, k);
+    }
+}
+
+void read_misc(nd_sim *s, SimMisc &m) {
+    ND_HIP(hipMemcpyAsync(&m, s->misc.p, sizeof(SimMisc), hipMemcpyDeviceToHost, s->ctx->stream));
+    ND_HIP(hipStreamSynchronize(s->ctx->stream));
+}
+
+// Eager rebuild with an immediate capacity check (used when the list is stale
+// at the start of a run): grows the per-atom capacity until everything fits.
+void rebuild_checked(nd_sim *s) {
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        const int cur0 = s->cur;
+        launch_rebuild(s);
+        SimMisc m;
+        read_misc(s, m);
+        if (m.status & nd::kStatusDomain)
+            throw nd::Error(ND_ERR_DOMAIN, "simulation: coordinate outside the key domain");
+        if (m.status & nd::kStatusTree)
+            throw nd::Error(ND_ERR_STATE, "simulation: malformed neighbour tree");
+        s->last_maxcnt = m.maxcnt;
+        if (!m.overflow) {
+            s->list_valid = true;
+            s->rebuilds++;
+            return;
+        }
+        // The permutation is valid; only the list overflowed.
+        (void)cur0;
+        alloc_lists(s, m.maxcnt + m.maxcnt / 4 + 8);
+        k_misc_reset_counters<<<1, 64, 0, s->ctx->stream>>>(s->misc.as<SimMisc>());
+    }
+    throw nd::Error(ND_ERR_CAPACITY, "simulation: neighbour list capacity could not be sized");
+}
+
+void snapshot(nd_sim *s) {
+    hipStream_t st = s->ctx->stream;
+    const size_t pb = pos_bytes(s);
+    s->snap_pos.reserve(pb);
+    s->snap_vel.reserve(pb);
+    s->snap_frc.reserve(pb);
+    s->snap_orig.reserve(sizeof(int32_t) * (size_t)s->n);
+    ND_HIP(hipMemcpyAsync(s->snap_pos.p, s->pos[s->cur].p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->snap_vel.p, s->vel[s->vcur].p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->snap_frc.p, s->frc.p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->snap_orig.p, s->orig[s->vcur].p, sizeof(int32_t) * (size_t)s->n,
+                          hipMemcpyDeviceToDevice, st));
+}
+
+void restore(nd_sim *s) {
+    hipStream_t st = s->ctx->stream;
+    const size_t pb = pos_bytes(s);
+    ND_HIP(hipMemcpyAsync(s->pos[s->cur].p, s->snap_pos.p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->vel[s->vcur].p, s->snap_vel.p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->frc.p, s->snap_frc.p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->orig[s->vcur].p, s->snap_orig.p, sizeof(int32_t) * (size_t)s->n,
+                          hipMemcpyDeviceToDevice, st));
+}
+
+bool graph_eligible(const nd_sim *s) {
+    if (!s->p.use_graph || s->timing) return false;
+    const int32_t W = s->p.rebuild_every;
+    if (s->p.energy_every > 0 && W % s->p.energy_every != 0) return false;
+    if (s->p.thermostat_every > 0 && W % s->p.thermostat_every != 0) return false;
+    return true;
+}
+
+// Window at k0 (k0 % W == 0): rebuild(x_k0) + middle steps k0 .. k0+W-1.
+void launch_window(nd_sim *s, int64_t k0) {
+    launch_rebuild(s);
+    s->rebuilds++;
+    s->last_build_step = k0;
+    for (int32_t o = 0; o < s->p.rebuild_every; ++o) launch_middle_step(s, k0 + o);
+}
+
+void run_window_graph(nd_sim *s, int64_t k0) {
+    const int parity = s->cur | (s->vcur << 1);
+    if (!s->gexec[parity]) {
+        hipStream_t st = s->ctx->stream;
+        hipGraph_t graph;
+        const int cur0 = s->cur, vcur0 = s->vcur;
+        const int64_t rb0 = s->rebuilds;
+        ND_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            launch_window(s, k0);
+        } catch (...) {
+            hipStreamEndCapture(st, &graph);
+            throw;
+        }
+        ND_HIP(hipStreamEndCapture(st, &graph));
+        ND_HIP(hipGraphInstantiate(&s->gexec[parity], graph, nullptr, nullptr, 0));
+        ND_HIP(hipGraphDestroy(graph));
+        s->cur = cur0;  // capture did not execute anything
+        s->vcur = vcur0;
+        s->rebuilds = rb0;
+    }
+    ND_HIP(hipGraphLaunch(s->gexec[parity], s->ctx->stream));
+    // Host-side bookkeeping of launch_window: the permutation swaps the state
+    // buffers once and every middle step swaps them once more (a fused step,
+    // or a final step followed by a kick-drift).
+    s->rebuilds++;
+    s->last_build_step = k0;
+    s->cur ^= (1 + s->p.rebuild_every) & 1;
+    s->vcur ^= 1;
+}
+
+void run_once(nd_sim *s, int32_t nsteps) {
+    const int32_t W = s->p.rebuild_every;
+    int64_t k = s->step;
+    if (!s->list_valid) {
+        // the stored force is in the order before this permutation: recompute it
+        rebuild_checked(s);
+        s->last_build_step = k;
+        s->have_force = false;
+    }
+    if (!s->have_force) {
+        launch_force_step(s, kModeForce, k);
+        s->have_force = true;
+    }
+    if (nsteps == 0) return;
+    // step k -> k+1 starts with a kick-drift from the stored force (after the
+    // velocity rescale if step k is a thermostat step; the previous run's
+    // final step recorded KE(v_k) for it)
+    launch_kick_drift(s, thermo_step(s, k));
+    const int64_t kend = k + nsteps;  // integer step reached at the end
+    int64_t j = k + 1;                // next force evaluation index
+    const bool use_graph = graph_eligible(s);
+    while (j < kend) {
+        if (j % W == 0) {
+            if (use_graph && j + W <= kend) {
+                run_window_graph(s, j);
+                j += W;
+                continue;
+            }
+            launch_rebuild(s);
+            s->rebuilds++;
+            s->last_build_step = j;
+        }
+        launch_middle_step(s, j);
+        ++j;
+    }
+    // final integer step: rebuild if due, force + finishing half kick
+    if (kend % W == 0 && s->last_build_step != kend) {
+        launch_rebuild(s);
+        s->rebuilds++;
+        s->last_build_step = kend;
+    }
+    if (thermo_step(s, kend)) {
+        const int32_t ee = s->p.energy_every;
+        s->p.energy_every = 1;
+        launch_force_step(s, kModeFinal, kend);
+        s->p.energy_every = ee;
+    } else {
+        launch_force_step(s, kModeFinal, kend);
+    }
+    s->step = kend;
+}
+
+void set_step_params(nd_sim *s) {
+    const nd_sim_params &p = s->p;
+    StepParams &q = s->sp;
+    q = StepParams{};
+    q.n = s->n;
+    q.cap = s->cap;
+    q.walls = p.walls;
+    const double eps = p.eps, sig = p.sigma, rc = p.rc, kc = p.k_coul, rcc = p.rc_coul;
+    const double rcmax = rc > rcc ? rc : rcc;
+    double eljs = 0.0;
+    if (rc > 0.0) {
+        const double c2 = sig * sig / (rc * rc), c6 = c2 * c2 * c2;
+        eljs = 4.0 * eps * c6 * (c6 - 1.0);
+    }
+    q.deps24 = 24.0 * eps; q.deps4 = 4.0 * eps; q.dsigma2 = sig * sig;
+    q.drc2 = rc * rc; q.drcc2 = rcc * rcc; q.drcmax2 = rcmax * rcmax; q.dkc = kc;
+    q.delj_shift = 0.0;  // plain (unshifted) LJ energy, as the textbook form
+    q.decoul_inv_rc = 0.0;
+    (void)eljs;
+    q.eps24 = (float)q.deps24; q.eps4 = (float)q.deps4; q.sigma2 = (float)q.dsigma2;
+    q.rc2 = (float)q.drc2; q.rcc2 = (float)q.drcc2; q.rcmax2 = (float)q.drcmax2; q.kc = (float)kc;
+    q.elj_shift = 0.0f; q.ecoul_inv_rc = 0.0f;
+    q.dt = p.dt; q.lo = p.lo; q.hi = p.hi;
+    q.dtf = p.dt; q.lof = p.lo; q.hif = p.hi;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && p && out, "nd_sim_create: null argument");
+    *out = nullptr;
+    ND_REQUIRE(p->n >= 2, "nd_sim_create: need at least two atoms");
+    ND_REQUIRE(p->atomsperleaf >= 1 && p->n % p->atomsperleaf == 0 && p->n / p->atomsperleaf >= 2,
+               "nd_sim_create: atomsperleaf must divide n and give >= 2 leaves");
+    ND_REQUIRE(p->precision >= ND_PREC_FP32 && p->precision <= ND_PREC_FP64, "nd_sim_create: precision");
+    ND_REQUIRE(p->morton_mode == ND_MORTON_MASKED || p->morton_mode == ND_MORTON_INTERLEAVE,
+               "nd_sim_create: key mode");
+    ND_REQUIRE(p->rebuild_every >= 1, "nd_sim_create: rebuild_every >= 1");
+    ND_REQUIRE(p->dt > 0.0f && p->rc >= 0.0f && p->rc_coul >= 0.0f && p->skin >= 0.0f,
+               "nd_sim_create: dt > 0, cutoffs and skin >= 0");
+    ND_REQUIRE(p->rc > 0.0f || p->rc_coul > 0.0f, "nd_sim_create: no interaction enabled");
+    ND_REQUIRE(p->hi > p->lo || (p->morton_mode == ND_MORTON_MASKED && !p->walls),
+               "nd_sim_create: box hi must exceed lo");
+    ND_HIP(hipSetDevice(ctx->device));
+    nd_sim *s = new nd_sim();
+    try {
+        s->ctx = ctx;
+        s->p = *p;
+        s->n = p->n;
+        s->apl = p->atomsperleaf;
+        s->nL = p->n / p->atomsperleaf;
+        s->wide = p->precision != ND_PREC_FP32;
+        // The list is built from FP32 coordinates; pad its radius by the FP32
+        // rounding of coordinates in the box (2^-20 relative) so that it stays
+        // a superset of the pairs inside the cutoff in every precision form.
+        const double rl = (p->rc > p->rc_coul ? p->rc : p->rc_coul) + p->skin;
+        const double span = std::fabs(p->lo) > std::fabs(p->hi) ? std::fabs(p->lo) : std::fabs(p->hi);
+        s->rlist = (float)(rl + (span + rl) * 9.5367431640625e-07);
+        const size_t pb = pos_bytes(s), n = (size_t)s->n;
+        for (int b = 0; b < 2; ++b) {
+            s->pos[b].reserve(pb);
+            s->vel[b].reserve(pb);
+            s->orig[b].reserve(sizeof(int32_t) * n);
+        }
+        s->frc.reserve(pb);
+        if (s->wide) s->pos32.reserve(sizeof(float4) * n);
+        s->codes.reserve(sizeof(int32_t) * n);
+        s->perm.reserve(sizeof(int32_t) * n);
+        s->nodes.reserve(sizeof(nd_node) * (size_t)(2 * s->nL - 1));
+        s->store.reserve(sizeof(int32_t) * (size_t)s->nL);
+        s->ncount.reserve(sizeof(int32_t) * n);
+        s->misc.reserve(sizeof(SimMisc));
+        s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
+        s->rows.reserve(sizeof(double) * 5 * (size_t)s->max_rows);
+        const int32_t ntiles = (int32_t)nd::radix_scratch_blocks(s->n);
+        s->sort.keys_alt.reserve(sizeof(int32_t) * n);
+        s->sort.vals_alt.reserve(sizeof(int32_t) * n);
+        s->sort.hist.reserve(sizeof(int32_t) * 256 * (size_t)ntiles);
+        ND_HIP(hipMemsetAsync(s->misc.p, 0, sizeof(SimMisc), ctx->stream));
+        int32_t cap = p->max_neighbors;
+        if (cap <= 0) {
+            const double vol = p->hi > p->lo ? (double)(p->hi - p->lo) * (p->hi - p->lo) * (p->hi - p->lo) : 0.0;
+            const double rho = vol > 0.0 ? (double)p->n / vol : 1.0;
+            const double est = 4.18879 * rho * (double)s->rlist * s->rlist * s->rlist;
+            cap = (int32_t)(est * 1.4 + 32.0);
+            if (cap > p->n - 1) cap = p->n - 1;
+            if (cap < 8) cap = 8;
+        }
+        alloc_lists(s, cap);
+        set_step_params(s);
+        ND_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        nd_sim_destroy(s);
+        throw;
+    }
+    *out = s;
+    ND_API_END(ctx)
+}
+
+int nd_sim_destroy(nd_sim *s) {
+    if (!s) return ND_OK;
+    if (s->ctx) {
+        hipSetDevice(s->ctx->device);
+        hipStreamSynchronize(s->ctx->stream);
+    }
+    drop_graphs(s);
+    for (auto e : s->ev_pool) hipEventDestroy(e);
+    nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
+                          &s->orig[1], &s->pos32, &s->codes, &s->perm, &s->nodes, &s->store,
+                          &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
+                          &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
+                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags};
+    for (auto *b : bufs) b->release();
+    delete s;
+    return ND_OK;
+}
+
+int nd_sim_set_state(nd_sim *s, const double *h_x, const double *h_v, const double *h_q,
+                     const double *h_m) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s && h_x && h_v, "nd_sim_set_state: null argument");
+    ND_HIP(hipSetDevice(ctx->device));
+    const size_t n = (size_t)s->n;
+    std::vector<int32_t> orig(n);
+    for (size_t i = 0; i < n; ++i) orig[i] = (int32_t)i;
+    s->cur = 0;
+    s->vcur = 0;
+    if (s->wide) {
+        std::vector<double4> x(n), v(n);
+        for (size_t i = 0; i < n; ++i) {
+            const double m = h_m ? h_m[i] : 1.0;
+            ND_REQUIRE(m > 0.0, "nd_sim_set_state: masses must be positive");
+            x[i] = make_double4(h_x[3 * i], h_x[3 * i + 1], h_x[3 * i + 2], h_q ? h_q[i] : 0.0);
+            v[i] = make_double4(h_v[3 * i], h_v[3 * i + 1], h_v[3 * i + 2], 1.0 / m);
+        }
+        ND_HIP(hipMemcpyAsync(s->pos[0].p, x.data(), sizeof(double4) * n, hipMemcpyHostToDevice, ctx->stream));
+        ND_HIP(hipMemcpyAsync(s->vel[0].p, v.data(), sizeof(double4) * n, hipMemcpyHostToDevice, ctx->stream));
+        ND_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float4> x(n), v(n);
+        for (size_t i = 0; i < n; ++i) {
+            const double m = h_m ? h_m[i] : 1.0;
+            ND_REQUIRE(m > 0.0, "nd_sim_set_state: masses must be positive");
+            x[i] = make_float4((float)h_x[3 * i], (float)h_x[3 * i + 1], (float)h_x[3 * i + 2],
+                               h_q ? (float)h_q[i] : 0.0f);
+            v[i] = make_float4((float)h_v[3 * i], (float)h_v[3 * i + 1], (float)h_v[3 * i + 2],
+                               (float)(1.0 / m));
+        }
+        ND_HIP(hipMemcpyAsync(s->pos[0].p, x.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
+        ND_HIP(hipMemcpyAsync(s->vel[0].p, v.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
+        ND_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    ND_HIP(hipMemcpyAsync(s->orig[0].p, orig.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    ND_HIP(hipMemsetAsync(s->misc.p, 0, sizeof(SimMisc), ctx->stream));
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    s->step = 0;
+    s->have_force = false;
+    s->list_valid = false;
+    s->have_state = true;
+    s->last_build_step = -1;
+    drop_graphs(s);
+    ND_API_END(ctx)
+}
+
+int nd_sim_get_state(nd_sim *s, double *h_x, double *h_v, double *h_f) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s, "nd_sim_get_state: null simulation");
+    ND_REQUIRE(s->have_state, "nd_sim_get_state: no state set");
+    ND_HIP(hipSetDevice(ctx->device));
+    const size_t n = (size_t)s->n;
+    std::vector<int32_t> orig(n);
+    ND_HIP(hipMemcpyAsync(orig.data(), s->orig[s->vcur].p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+    auto unpack = [&](const auto &x, const auto &v, const auto &f) {
+        ND_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t k = 0; k < n; ++k) {
+            const size_t o = (size_t)orig[k];
+            if (h_x) { h_x[3 * o] = x[k].x; h_x[3 * o + 1] = x[k].y; h_x[3 * o + 2] = x[k].z; }
+            if (h_v) { h_v[3 * o] = v[k].x; h_v[3 * o + 1] = v[k].y; h_v[3 * o + 2] = v[k].z; }
+            if (h_f) {
+                if (s->have_force) { h_f[3 * o] = f[k].x; h_f[3 * o + 1] = f[k].y; h_f[3 * o + 2] = f[k].z; }
+                else { h_f[3 * o] = h_f[3 * o + 1] = h_f[3 * o + 2] = 0.0; }
+            }
+        }
+    };
+    if (s->wide) {
+        std::vector<double4> x(n), v(n), f(n);
+        ND_HIP(hipMemcpyAsync(x.data(), s->pos[s->cur].p, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        ND_HIP(hipMemcpyAsync(v.data(), s->vel[s->vcur].p, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        ND_HIP(hipMemcpyAsync(f.data(), s->frc.p, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        unpack(x, v, f);
+    } else {
+        std::vector<float4> x(n), v(n), f(n);
+        ND_HIP(hipMemcpyAsync(x.data(), s->pos[s->cur].p, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        ND_HIP(hipMemcpyAsync(v.data(), s->vel[s->vcur].p, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        ND_HIP(hipMemcpyAsync(f.data(), s->frc.p, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        unpack(x, v, f);
+    }
+    ND_API_END(ctx)
+}
+
+int nd_sim_run(nd_sim *s, int32_t nsteps) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s && nsteps >= 0, "nd_sim_run: bad argument");
+    ND_REQUIRE(s->have_state, "nd_sim_run: call nd_sim_set_state first");
+    ND_HIP(hipSetDevice(ctx->device));
+    s->ev_used = 0;
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        // snapshot for a rerun if the neighbour list overflows mid-run
+        SimMisc m0;
+        read_misc(s, m0);
+        const int cur0 = s->cur, vcur0 = s->vcur;
+        const int64_t step0 = s->step, rb0 = s->rebuilds, lb0 = s->last_build_step;
+        const bool hf0 = s->have_force, lv0 = s->list_valid;
+        snapshot(s);
+        run_once(s, nsteps);
+        SimMisc m;
+        read_misc(s, m);
+        if (m.status & nd::kStatusDomain)
+            throw nd::Error(ND_ERR_DOMAIN, "simulation: coordinate outside the key domain");
+        if (m.status & nd::kStatusTree)
+            throw nd::Error(ND_ERR_STATE, "simulation: malformed neighbour tree");
+        s->last_maxcnt = m.maxcnt;
+        if (!m.overflow) {
+            if (s->timing && s->ev_used >= 2) {
+                // every force launch recorded an event pair
+                for (size_t e = 0; e + 1 < s->ev_used; e += 2) {
+                    float ms = 0.0f;
+                    ND_HIP(hipEventElapsedTime(&ms, s->ev_pool[e], s->ev_pool[e + 1]));
+                    s->force_ms += ms;
+                    s->force_launches++;
+                }
+            }
+            return ND_OK;
+        }
+        // overflow: grow, restore the snapshot and run again
+        s->cur = cur0;
+        s->vcur = vcur0;
+        restore(s);
+        alloc_lists(s, m.maxcnt + m.maxcnt / 4 + 8);
+        set_step_params(s);
+        m0.overflow = 0;
+        m0.maxcnt = 0;
+        ND_HIP(hipMemcpyAsync(s->misc.p, &m0, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
+        ND_HIP(hipStreamSynchronize(ctx->stream));
+        s->step = step0;
+        s->rebuilds = rb0;
+        s->last_build_step = lb0;
+        s->have_force = hf0;
+        s->list_valid = false;
+        (void)lv0;
+        s->ev_used = 0;
+    }
+    throw nd::Error(ND_ERR_CAPACITY, "nd_sim_run: neighbour list kept overflowing");
+    ND_API_END(ctx)
+}
+
+int nd_sim_energies(nd_sim *s, double *h_rows, int32_t max_rows, int32_t *rows) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s && rows, "nd_sim_energies: null argument");
+    SimMisc m;
+    read_misc(s, m);
+    int32_t r = m.rowctr < s->max_rows ? m.rowctr : s->max_rows;
+    if (r > max_rows) r = max_rows;
+    if (r > 0 && h_rows)
+        ND_HIP(hipMemcpyAsync(h_rows, s->rows.p, sizeof(double) * 5 * (size_t)r, hipMemcpyDeviceToHost, ctx->stream));
+    m.rowctr = 0;
+    ND_HIP(hipMemcpyAsync(s->misc.p, &m, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    *rows = r;
+    if (r > 0 && h_rows) s->pairs_last = (int64_t)h_rows[5 * (size_t)(r - 1) + 4];
+    ND_API_END(ctx)
+}
+
+int nd_sim_stats_get(nd_sim *s, nd_sim_stats *out) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s && out, "nd_sim_stats_get: null argument");
+    std::vector<int32_t> cnt((size_t)s->n);
+    int64_t entries = 0;
+    if (s->list_valid) {
+        ND_HIP(hipMemcpyAsync(cnt.data(), s->ncount.p, sizeof(int32_t) * (size_t)s->n, hipMemcpyDeviceToHost, ctx->stream));
+        ND_HIP(hipStreamSynchronize(ctx->stream));
+        for (int32_t c : cnt) entries += c;
+    }
+    out->steps = s->step;
+    out->rebuilds = s->rebuilds;
+    out->list_entries = entries;
+    out->max_neighbors = s->last_maxcnt;
+    out->capacity = s->cap;
+    out->pairs_in_cutoff = s->pairs_last;
+    ND_API_END(ctx)
+}
+
+int nd_sim_kernel_timing(nd_sim *s, int32_t enable, double *avg_force_ms, int64_t *launches) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s, "nd_sim_kernel_timing: null simulation");
+    if (avg_force_ms) *avg_force_ms = s->force_launches ? s->force_ms / (double)s->force_launches : 0.0;
+    if (launches) *launches = s->force_launches;
+    if (enable >= 0) {
+        s->timing = enable != 0;
+        s->force_ms = 0.0;
+        s->force_launches = 0;
+    }
+    ND_API_END(ctx)
+}
+
+}  // extern "C"

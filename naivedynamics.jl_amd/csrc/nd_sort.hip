@@ -1,0 +1,247 @@
+// nd_sort.hip -- K2: stable LSD radix sort of (int32 key, int32 value) pairs in
+// signed-key order, 8-bit digits, for sortperm (BVHTraverse.jl:570-573).
+//
+// Per pass: (1) per-tile digit histograms, (2) one exclusive scan over the
+// digit-major [digit][tile] table, (3) stable scatter.  The scatter ranks
+// each 64-key round of a wave with eight ballots (lanes holding the same
+// digit form a peer mask; rank = popcount of lower peers), carries running
+// per-wave digit counters in LDS, stages the tile in LDS in sorted order and
+// writes every digit run to global memory contiguously.
+#include "nd_internal.hpp"
+
+namespace {
+
+constexpr int kWaves = nd::kBlock / 64;     // 4
+constexpr int kRounds = 4;                  // 64-key rounds per wave
+constexpr int kTile = nd::kBlock * kRounds; // 1024 keys per workgroup
+
+__device__ __forceinline__ uint32_t digit_of(int32_t key, int shift) {
+    return (((uint32_t)key ^ 0x80000000u) >> shift) & 0xFFu;
+}
+
+__global__ void __launch_bounds__(nd::kBlock)
+k_radix_hist(const int32_t *__restrict__ keys, int32_t n, int shift, int32_t ntiles,
+             int32_t *__restrict__ hist) {
+    __shared__ int32_t cnt[256];
+    const int t = threadIdx.x;
+    cnt[t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        int64_t i = base + r * nd::kBlock + t;
+        if (i < n) atomicAdd(&cnt[digit_of(keys[i], shift)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)t * ntiles + blockIdx.x] = cnt[t];
+}
+
+// Exclusive scan of hist[0..total) in place with one workgroup of 1024 lanes.
+__global__ void __launch_bounds__(1024) k_radix_scan(int32_t *__restrict__ hist, int64_t total) {
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (total + 1023) / 1024;
+    const int64_t lo = t * per, hi = lo + per < total ? lo + per : total;
+    int32_t s = 0;
+    for (int64_t k = lo; k < hi; ++k) s += hist[k];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = part[t] - s;  // exclusive prefix of this lane's chunk
+    for (int64_t k = lo; k < hi; ++k) {
+        int32_t c = hist[k];
+        hist[k] = run;
+        run += c;
+    }
+}
+
+__global__ void __launch_bounds__(nd::kBlock)
+k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
+                int32_t n, int shift, int32_t ntiles, const int32_t *__restrict__ hist,
+                int32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out) {
+    __shared__ int32_t wcnt[kWaves][256];
+    __shared__ int32_t bpre[256];
+    __shared__ int32_t gofs[256];
+    __shared__ int32_t wsum[kWaves];
+    __shared__ int32_t sk[kTile];
+    __shared__ int32_t sv[kTile];
+
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int lane = t & 63;
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    const int32_t nvalid = (int32_t)((n - base) < kTile ? (n - base) : kTile);
+
+    gofs[t] = hist[(int64_t)t * ntiles + blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) wcnt[k][t] = 0;
+    __syncthreads();
+
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int32_t key[kRounds], val[kRounds], rank[kRounds];
+    uint32_t dig[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int32_t li = w * (64 * kRounds) + r * 64 + lane;  // position within the tile
+        const bool valid = li < nvalid;
+        key[r] = valid ? keys_in[base + li] : 0;
+        val[r] = valid ? vals_in[base + li] : 0;
+        const uint32_t d = digit_of(key[r], shift);
+        dig[r] = d;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        int32_t before = 0;
+        if (valid) before = wcnt[w][d];
+        rank[r] = before + __popcll(peers & lt);
+        if (valid && (peers & lt) == 0) wcnt[w][d] = before + __popcll(peers);
+    }
+    __syncthreads();
+
+    // Per digit: offsets of each wave's run and the block-level digit prefix.
+    int32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+        int32_t c = wcnt[k][t];
+        wcnt[k][t] = tot;
+        tot += c;
+    }
+    // exclusive scan of tot over the 256 digits
+    int32_t inc = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int32_t v = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += v;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int32_t wbase = 0;
+    for (int k = 0; k < w; ++k) wbase += wsum[k];
+    bpre[t] = wbase + inc - tot;
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int32_t li = w * (64 * kRounds) + r * 64 + lane;
+        if (li < nvalid) {
+            const int32_t pos = bpre[dig[r]] + wcnt[w][dig[r]] + rank[r];
+            sk[pos] = key[r];
+            sv[pos] = val[r];
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int32_t k = r * nd::kBlock + t;
+        if (k < nvalid) {
+            const int32_t kk = sk[k];
+            const uint32_t d = digit_of(kk, shift);
+            const int64_t g = (int64_t)gofs[d] + (k - bpre[d]);
+            if (g >= 0 && g < n) {
+                keys_out[g] = kk;
+                vals_out[g] = sv[k];
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(nd::kBlock)
+k_keybits(const int32_t *__restrict__ keys, int32_t n, uint32_t *__restrict__ out) {
+    uint32_t o = 0u, a = 0xFFFFFFFFu;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t k = (uint32_t)keys[i] ^ 0x80000000u;
+        o |= k;
+        a &= k;
+    }
+    atomicOr(&out[0], o);
+    atomicAnd(&out[1], a);
+}
+
+__global__ void k_keybits_init(uint32_t *out) {
+    if (threadIdx.x == 0) { out[0] = 0u; out[1] = 0xFFFFFFFFu; }
+}
+
+__global__ void __launch_bounds__(nd::kBlock) k_iota(int32_t *p, int32_t n) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+}  // namespace
+
+namespace nd {
+
+size_t radix_scratch_blocks(int32_t n) { return (size_t)((n + kTile - 1) / kTile); }
+
+void radix_sort_pairs(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
+                      int32_t first_bit, int32_t end_bit) {
+    if (n <= 1 || end_bit <= first_bit) return;
+    const int32_t ntiles = (int32_t)radix_scratch_blocks(n);
+    b.keys_alt.reserve(sizeof(int32_t) * (size_t)n);
+    b.vals_alt.reserve(sizeof(int32_t) * (size_t)n);
+    b.hist.reserve(sizeof(int32_t) * 256 * (size_t)ntiles);
+    int32_t *ki = keys, *vi = vals, *ko = b.keys_alt.as<int32_t>(), *vo = b.vals_alt.as<int32_t>();
+    int passes = 0;
+    for (int shift = first_bit; shift < end_bit; shift += 8) {
+        k_radix_hist<<<ntiles, kBlock, 0, s>>>(ki, n, shift, ntiles, b.hist.as<int32_t>());
+        k_radix_scan<<<1, 1024, 0, s>>>(b.hist.as<int32_t>(), (int64_t)256 * ntiles);
+        k_radix_scatter<<<ntiles, kBlock, 0, s>>>(ki, vi, n, shift, ntiles,
+                                                  b.hist.as<int32_t>(), ko, vo);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+        ++passes;
+    }
+    ND_HIP(hipGetLastError());
+    if (passes & 1) {
+        ND_HIP(hipMemcpyAsync(keys, ki, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        ND_HIP(hipMemcpyAsync(vals, vi, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+// Sorts keys/vals in place over only the 8-bit digit windows whose bits vary
+// (synchronises once to read the varying-bit mask).
+void radix_sort_pairs_auto(nd_ctx *ctx, int32_t *keys, int32_t *vals, int32_t n) {
+    if (n <= 1) return;
+    ctx->sort.flags.reserve(64);
+    uint32_t *fl = ctx->sort.flags.as<uint32_t>();
+    k_keybits_init<<<1, 64, 0, ctx->stream>>>(fl);
+    unsigned g = (unsigned)std::min<int64_t>(grid_for(n), 1024);
+    k_keybits<<<g, kBlock, 0, ctx->stream>>>(keys, n, fl);
+    uint32_t h[2];
+    ND_HIP(hipMemcpyAsync(h, fl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    const uint32_t vary = h[0] ^ h[1];
+    for (int shift = 0; shift < 32; shift += 8)
+        if ((vary >> shift) & 0xFFu) radix_sort_pairs(ctx->stream, ctx->sort, keys, vals, n, shift, shift + 8);
+}
+
+void launch_iota(hipStream_t s, int32_t *p, int32_t n) {
+    if (n > 0) k_iota<<<grid_for(n), kBlock, 0, s>>>(p, n);
+    ND_HIP(hipGetLastError());
+}
+
+}  // namespace nd
+
+extern "C" int nd_sortperm(nd_ctx *ctx, const int32_t *d_codes, int32_t n,
+                           int32_t *d_codes_sorted, int32_t *d_perm) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && d_codes && d_codes_sorted && d_perm && n >= 0, "nd_sortperm: bad argument");
+    ND_HIP(hipSetDevice(ctx->device));
+    if (d_codes_sorted != d_codes)
+        ND_HIP(hipMemcpyAsync(d_codes_sorted, d_codes, sizeof(int32_t) * (size_t)n,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    nd::launch_iota(ctx->stream, d_perm, n);
+    nd::radix_sort_pairs_auto(ctx, d_codes_sorted, d_perm, n);
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    ND_API_END(ctx)
+}

@@ -1,0 +1,516 @@
+/*
+ * refcpu.c -- CPU restatement of NaiveDynamics.jl's LBVH neighbour-search path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP library is checked
+ * against and the threaded CPU baseline that bench.py times.  Nothing in the
+ * product library links, loads or calls it.
+ *
+ * Every function restates the algorithm of the Julia reference (file:line
+ * citations below, relative to the reference repository root).  Arithmetic is
+ * plain IEEE-754 binary32 in the same operation order as the Julia code; build
+ * with -ffp-contract=off and without -ffast-math so no multiply-add is fused.
+ *
+ * Index conventions: positions, codes and permutations are 0-based.  Tree nodes
+ * keep the reference numbering: node k (1-based; leaves 1..nL, internal nodes
+ * nL+1..2nL-1, root nL+1, 0 = sentinel) lives in array slot k-1, and the
+ * left/skip fields hold 1-based node numbers exactly as GridKey does
+ * (src/Neighbors/BVHTraverse.jl:102-109).  Pair records hold 0-based original
+ * atom indices (the Julia reference is 1-based; add 1 to compare).
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "refcpu.h"
+
+/* ------------------------------------------------------------------------- */
+/* A4/A5: masked "Morton" code.  BVHTraverse.jl:184 (binwidth) and 237-288.   */
+/* ------------------------------------------------------------------------- */
+
+/* Float32(1/1023): the quotient is formed in Float64 and rounded once. */
+#define REF_BINWIDTH ((float)(1.0 / 1023.0))
+
+static const int32_t REF_MAGIC[3] = {0x09249249, 0x12492492, 0x24924924};
+
+float ref_binwidth(void) { return REF_BINWIDTH; }
+
+/* round(Int32, v, RoundDown) throws InexactError unless floor(v) fits Int32;
+ * for binary32 that is exactly -2^31 <= v < 2^31 (NaN fails both tests). */
+static inline int ref_floor_i32(float v, int32_t *q) {
+    if (!(v >= -2147483648.0f && v < 2147483648.0f)) return 0;
+    *q = (int32_t)floorf(v);
+    return 1;
+}
+
+/* mortoncodes! (BVHTraverse.jl:257-288): yin = q & M, yang = q | ~M and
+ * yin & yang == q & M, OR-ed over the three axes. */
+int ref_morton_masked(const float *xyz, int64_t n, int32_t *codes) {
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+            float v = xyz[3 * i + d] / REF_BINWIDTH;
+            int32_t q;
+            if (!ref_floor_i32(v, &q)) return REF_ERR_DOMAIN;
+            int32_t yin = q & REF_MAGIC[d];
+            int32_t yang = q | ~REF_MAGIC[d];
+            code |= (yin & yang);
+        }
+        codes[i] = code;
+    }
+    return REF_OK;
+}
+
+/* Performance-mode key (not in the reference): a true 30-bit Morton
+ * interleave of 10-bit cell coordinates q_d = clamp(floor((x_d-lo_d)*s_d)),
+ * s_d = fl32(1024 / (hi_d - lo_d)).  Bit 3k+d of the code is bit k of q_d. */
+static inline uint32_t ref_spread10(uint32_t v) {
+    uint32_t r = 0;
+    for (int k = 0; k < 10; ++k) r |= ((v >> k) & 1u) << (3 * k);
+    return r;
+}
+
+int ref_morton_interleave(const float *xyz, int64_t n, const float lo[3],
+                          const float scale[3], int32_t *codes) {
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+            float x = xyz[3 * i + d];
+            if (!isfinite(x)) return REF_ERR_DOMAIN;
+            float t = (x - lo[d]) * scale[d];
+            int32_t q = (t <= 0.0f) ? 0 : (t >= 1023.0f ? 1023 : (int32_t)floorf(t));
+            code |= ref_spread10((uint32_t)q) << d;
+        }
+        codes[i] = (int32_t)code;
+    }
+    return REF_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* A6: sortperm on signed Int32 codes (BVHTraverse.jl:570-573).  Julia's      */
+/* sortperm is stable, so ties keep ascending original index.  Stable LSD    */
+/* radix sort on the sign-flipped key, 8-bit digits.                         */
+/* ------------------------------------------------------------------------- */
+void ref_sortperm(const int32_t *codes, int64_t n, int32_t *perm) {
+    uint32_t *k0 = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+    uint32_t *k1 = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+    int32_t *p1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        k0[i] = (uint32_t)codes[i] ^ 0x80000000u;
+        perm[i] = (int32_t)i;
+    }
+    uint32_t *ks = k0, *kd = k1;
+    int32_t *ps = perm, *pd = p1;
+    for (int shift = 0; shift < 32; shift += 8) {
+        int64_t cnt[257];
+        memset(cnt, 0, sizeof(cnt));
+        for (int64_t i = 0; i < n; ++i) cnt[((ks[i] >> shift) & 0xFFu) + 1]++;
+        if (cnt[((ks[0] >> shift) & 0xFFu) + 1] == n) continue; /* constant digit */
+        for (int b = 0; b < 256; ++b) cnt[b + 1] += cnt[b];
+        for (int64_t i = 0; i < n; ++i) {
+            uint32_t b = (ks[i] >> shift) & 0xFFu;
+            int64_t o = cnt[b]++;
+            kd[o] = ks[i];
+            pd[o] = ps[i];
+        }
+        uint32_t *tk = ks; ks = kd; kd = tk;
+        int32_t *tp = ps; ps = pd; pd = tp;
+    }
+    if (ps != perm) memcpy(perm, ps, sizeof(int32_t) * (size_t)n);
+    free(k0); free(k1); free(p1);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Julia min/max for Float32 (Base: diff = x-y, pick by signbit(diff)):       */
+/* ordinary ordering with -0 < +0; NaN inputs give NaN.                       */
+/* ------------------------------------------------------------------------- */
+static inline float jl_min(float x, float y) {
+    if (isnan(x) || isnan(y)) return x - y;
+    if (x < y) return x;
+    if (y < x) return y;
+    return signbit(x) ? x : y;
+}
+static inline float jl_max(float x, float y) {
+    if (isnan(x) || isnan(y)) return x - y;
+    if (x > y) return x;
+    if (y > x) return y;
+    return signbit(x) ? y : x;
+}
+
+/* ------------------------------------------------------------------------- */
+/* A7: leafcluster_primitives (BVHTraverse.jl:376-410): leaf l covers sorted  */
+/* atoms l*apl .. l*apl+apl-1, box = [min(x - r/2), max(x + r/2)].            */
+/* ------------------------------------------------------------------------- */
+void ref_leaf_boxes(const float *xyz_sorted, int32_t nL, int32_t apl, float r,
+                    ref_node *nodes) {
+    const float h = r / 2.0f; /* neighbor_distance/2 in Float32 */
+    for (int32_t l = 0; l < nL; ++l) {
+        const float *p0 = xyz_sorted + 3 * (int64_t)l * apl;
+        float mn[3], mx[3];
+        for (int d = 0; d < 3; ++d) { mn[d] = p0[d] - h; mx[d] = p0[d] + h; }
+        for (int32_t a = 0; a < apl; ++a) {
+            const float *p = xyz_sorted + 3 * ((int64_t)l * apl + a);
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = jl_min(mn[d], p[d] - h);
+                mx[d] = jl_max(mx[d], p[d] + h);
+            }
+        }
+        ref_node *k = &nodes[l];
+        for (int d = 0; d < 3; ++d) { k->min[d] = mn[d]; k->max[d] = mx[d]; }
+        k->left = 0;
+        k->skip = 0;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* A8: exptdelta (BVHTraverse.jl:676-696), 1-based leaf i.                    */
+/* ------------------------------------------------------------------------- */
+static inline int32_t ref_delta(int32_t i, const int32_t *codes_sorted, int32_t nL,
+                                int32_t apl) {
+    if (i >= nL || i < 1) return INT32_MAX;
+    int32_t x = codes_sorted[(int64_t)(i - 1) * apl] ^ codes_sorted[(int64_t)i * apl];
+    /* x + (x==0)*(typemin + (i xor (i+1))) - 1, with Int32 wraparound */
+    uint32_t t = (uint32_t)x;
+    if (x == 0) t += (uint32_t)INT32_MIN + (uint32_t)(i ^ (i + 1));
+    t -= 1u;
+    return (int32_t)t;
+}
+
+int32_t ref_delta_public(int32_t i, const int32_t *codes_sorted, int32_t nL, int32_t apl) {
+    return ref_delta(i, codes_sorted, nL, apl);
+}
+
+/* ------------------------------------------------------------------------- */
+/* A9/A10: bvh_interior! (BVHTraverse.jl:704-846) and                         */
+/* bounding_volume_hierarchy! (BVHTraverse.jl:849-860).  Apetrei/ArborX       */
+/* bottom-up build: one walker per leaf, the first walker to reach an         */
+/* internal node stops, the second merges its sibling's box and climbs.      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    ref_node *keys;
+    _Atomic int32_t *store;
+    const int32_t *codes;
+    int32_t nL, apl;
+} build_ctx;
+
+#define NODE(k) (c->keys[(k) - 1])
+
+static void ref_bvh_interior(build_ctx *c, int32_t i) {
+    const int32_t nL = c->nL, apl = c->apl;
+    const int32_t *codes = c->codes;
+    int32_t rangel = i, ranger = i;
+    int32_t dell = ref_delta(rangel - 1, codes, nL, apl);
+    int32_t delr = ref_delta(ranger, codes, nL, apl);
+    float bmin[3], bmax[3];
+    for (int d = 0; d < 3; ++d) { bmin[d] = NODE(i).min[d]; bmax[d] = NODE(i).max[d]; }
+
+    /* leaf rope (BVHTraverse.jl:717-741) */
+    if (i == nL) {
+        NODE(i).skip = 0;
+    } else {
+        int32_t ir = i + 1;
+        NODE(i).skip = (delr < ref_delta(ir, codes, nL, apl)) ? ir : ir + nL;
+    }
+
+    for (;;) {
+        int32_t leftChild;
+        if (delr < dell) { /* isLeftChild: the sibling lies to the right */
+            leftChild = i;
+            int32_t split = ranger;
+            int32_t expected = 0;
+            atomic_compare_exchange_strong(&c->store[split - 1], &expected, rangel);
+            ranger = expected;
+            if (ranger == 0) return; /* first to arrive: stop */
+            delr = ref_delta(ranger, codes, nL, apl);
+            int32_t rightChild = split + 1;
+            if (rightChild != ranger) rightChild += nL;
+            for (int d = 0; d < 3; ++d) {
+                bmax[d] = jl_max(NODE(rightChild).max[d], bmax[d]);
+                bmin[d] = jl_min(NODE(rightChild).min[d], bmin[d]);
+            }
+        } else { /* right child: the sibling lies to the left */
+            int32_t split = rangel - 1;
+            int32_t expected = 0;
+            atomic_compare_exchange_strong(&c->store[split - 1], &expected, ranger);
+            rangel = expected;
+            if (rangel == 0) return;
+            dell = ref_delta(rangel - 1, codes, nL, apl);
+            leftChild = split;
+            if (leftChild != rangel) leftChild += nL;
+            for (int d = 0; d < 3; ++d) {
+                bmax[d] = jl_max(NODE(leftChild).max[d], bmax[d]);
+                bmin[d] = jl_min(NODE(leftChild).min[d], bmin[d]);
+            }
+        }
+        int32_t q = (delr < dell) ? ranger : rangel;
+        int32_t parent = q + nL;
+        ref_node *P = &NODE(parent);
+        P->left = leftChild;
+        for (int d = 0; d < 3; ++d) { P->min[d] = bmin[d]; P->max[d] = bmax[d]; }
+        if (ranger == nL) {
+            P->skip = 0;
+        } else {
+            int32_t rr = ranger + 1;
+            P->skip = (delr < ref_delta(rr, codes, nL, apl)) ? rr : rr + nL;
+        }
+        i = parent;
+        if (i == 1 + nL) return; /* reached the root */
+    }
+}
+
+typedef struct {
+    build_ctx *c;
+    int32_t lo, hi;
+} build_task;
+
+static void *build_worker(void *arg) {
+    build_task *t = (build_task *)arg;
+    for (int32_t i = t->lo; i <= t->hi; ++i) ref_bvh_interior(t->c, i);
+    return NULL;
+}
+
+int ref_build_tree(const float *xyz_sorted, const int32_t *codes_sorted, int32_t N,
+                   int32_t apl, float r, ref_node *nodes, int override_root,
+                   int nthreads) {
+    if (apl < 1 || N % apl != 0) return REF_ERR_ARG;
+    int32_t nL = N / apl;
+    if (nL < 2) return REF_ERR_ARG;
+    memset(nodes, 0, sizeof(ref_node) * (size_t)(2 * nL - 1));
+    ref_leaf_boxes(xyz_sorted, nL, apl, r, nodes);
+    _Atomic int32_t *store = (_Atomic int32_t *)calloc((size_t)(nL - 1), sizeof(int32_t));
+    build_ctx c = {nodes, store, codes_sorted, nL, apl};
+    if (nthreads <= 1) {
+        for (int32_t i = 1; i <= nL; ++i) ref_bvh_interior(&c, i);
+    } else {
+        /* @batch over 1:nL in contiguous chunks (BVHTraverse.jl:851) */
+        pthread_t th[256];
+        build_task tk[256];
+        if (nthreads > 256) nthreads = 256;
+        int32_t per = (nL + nthreads - 1) / nthreads;
+        int nt = 0;
+        for (int t = 0; t < nthreads; ++t) {
+            int32_t lo = 1 + t * per, hi = lo + per - 1;
+            if (hi > nL) hi = nL;
+            if (lo > hi) break;
+            tk[nt] = (build_task){&c, lo, hi};
+            pthread_create(&th[nt], NULL, build_worker, &tk[nt]);
+            ++nt;
+        }
+        for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
+    }
+    free((void *)store);
+    if (override_root) {
+        /* bounding_volume_hierarchy! overwrites the root box (BVHTraverse.jl:856-858) */
+        ref_node *root = &nodes[nL];
+        for (int d = 0; d < 3; ++d) { root->min[d] = 0.0f; root->max[d] = 1.0f; }
+    }
+    return REF_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* A12-A16: leafneighbor_traverse (BVHTraverse.jl:1236-1323) with the scalar */
+/* pair test (onecluster/twocluster_proximitytest!, :1021-1055: s < r*r) or  */
+/* the SIMD test (ext/NaiveSIMD.jl:507-608: sqrt(s) < r).                     */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    ref_pair *v;
+    int64_t n, cap;
+} pairvec;
+
+static inline void pv_push(pairvec *pv, int32_t i, int32_t j, float d) {
+    if (pv->n == pv->cap) {
+        pv->cap = pv->cap ? pv->cap * 2 : 1024;
+        pv->v = (ref_pair *)realloc(pv->v, sizeof(ref_pair) * (size_t)pv->cap);
+    }
+    pv->v[pv->n].i = i;
+    pv->v[pv->n].j = j;
+    pv->v[pv->n].d = d;
+    pv->n++;
+}
+
+/* s = sum((a - b).^2) with StaticArrays' left fold: ((dx*dx + dy*dy) + dz*dz). */
+static inline float ref_sqdist(const float *a, const float *b) {
+    float dx = a[0] - b[0];
+    float dy = a[1] - b[1];
+    float dz = a[2] - b[2];
+    float sx = dx * dx, sy = dy * dy, sz = dz * dz;
+    float s = sx + sy;
+    s = s + sz;
+    return s;
+}
+
+static inline int ref_accept(float s, float r, float r2, int predicate, float *d) {
+    if (predicate == REF_PRED_SCALAR) {
+        if (s < r2) { *d = sqrtf(s); return 1; }
+        return 0;
+    }
+    float q = sqrtf(s);
+    if (q < r) { *d = q; return 1; }
+    return 0;
+}
+
+static inline int ref_overlap(const ref_node *a, const ref_node *b) {
+    /* all((A.min .< B.max) .& (A.max .> B.min)) (BVHTraverse.jl:1094-1098) */
+    int ok = 1;
+    for (int d = 0; d < 3; ++d) ok &= (a->min[d] < b->max[d]) & (a->max[d] > b->min[d]);
+    return ok;
+}
+
+typedef struct {
+    const ref_node *nodes;
+    const float *xyz;
+    const int32_t *idx;
+    int32_t nL, apl;
+    float r, r2;
+    int predicate;
+} trav_ctx;
+
+static void ref_traverse_leaf(const trav_ctx *c, int32_t q, pairvec *out, int64_t *tests) {
+    const int32_t apl = c->apl;
+    const ref_node *ql = &c->nodes[q - 1];
+    const int64_t qlo = (int64_t)(q - 1) * apl;
+    float d;
+    for (int32_t a = 0; a < apl - 1; ++a)
+        for (int32_t b = a + 1; b < apl; ++b) {
+            float s = ref_sqdist(c->xyz + 3 * (qlo + a), c->xyz + 3 * (qlo + b));
+            if (ref_accept(s, c->r, c->r2, c->predicate, &d))
+                pv_push(out, c->idx[qlo + a], c->idx[qlo + b], d);
+        }
+    int32_t t = ql->skip;
+    while (t != 0) {
+        const ref_node *tn = &c->nodes[t - 1];
+        if (ref_overlap(ql, tn)) {
+            if (tn->left == 0) {
+                const int64_t tlo = (int64_t)(t - 1) * apl;
+                if (tests) (*tests)++;
+                for (int32_t a = 0; a < apl; ++a)
+                    for (int32_t b = 0; b < apl; ++b) {
+                        float s = ref_sqdist(c->xyz + 3 * (qlo + a), c->xyz + 3 * (tlo + b));
+                        if (ref_accept(s, c->r, c->r2, c->predicate, &d))
+                            pv_push(out, c->idx[qlo + a], c->idx[tlo + b], d);
+                    }
+                t = tn->skip;
+            } else {
+                t = tn->left;
+            }
+        } else {
+            t = tn->skip;
+        }
+    }
+}
+
+typedef struct {
+    const trav_ctx *c;
+    int chunk, nthreads;
+    pairvec out;
+    int64_t tests;
+} trav_task;
+
+static void *trav_worker(void *arg) {
+    trav_task *t = (trav_task *)arg;
+    /* for q in chunk:threads:nL (BVHTraverse.jl:1255-1256) */
+    for (int32_t q = t->chunk; q <= t->c->nL; q += t->nthreads)
+        ref_traverse_leaf(t->c, q, &t->out, &t->tests);
+    return NULL;
+}
+
+int64_t ref_traverse(const ref_node *nodes, const float *xyz_sorted,
+                     const int32_t *idx_sorted, int32_t N, int32_t apl, float r,
+                     int predicate, ref_pair *out, int64_t cap, int nthreads,
+                     int64_t *leaf_pair_tests) {
+    if (apl < 1 || N % apl != 0 || N / apl < 2) return REF_ERR_ARG;
+    trav_ctx c = {nodes, xyz_sorted, idx_sorted, N / apl, apl, r, r * r, predicate};
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    trav_task tk[256];
+    pthread_t th[256];
+    for (int t = 0; t < nthreads; ++t) {
+        tk[t] = (trav_task){&c, t + 1, nthreads, {NULL, 0, 0}, 0};
+        if (nthreads == 1) trav_worker(&tk[t]);
+        else pthread_create(&th[t], NULL, trav_worker, &tk[t]);
+    }
+    if (nthreads > 1)
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    /* reduce(vcat, neighbor_vec) (BVHTraverse.jl:1314) */
+    int64_t total = 0, tests = 0;
+    for (int t = 0; t < nthreads; ++t) { total += tk[t].out.n; tests += tk[t].tests; }
+    if (leaf_pair_tests) *leaf_pair_tests = tests;
+    if (out && total <= cap) {
+        int64_t o = 0;
+        for (int t = 0; t < nthreads; ++t) {
+            if (tk[t].out.n) memcpy(out + o, tk[t].out.v, sizeof(ref_pair) * (size_t)tk[t].out.n);
+            o += tk[t].out.n;
+        }
+    }
+    for (int t = 0; t < nthreads; ++t) free(tk[t].out.v);
+    return total;
+}
+
+/* ------------------------------------------------------------------------- */
+/* A11 + A16: leafTreeData + leafbuild_traverse_bvh in one call               */
+/* (BVHTraverse.jl:544-597, 1423-1428; ext/NaiveSIMD.jl:424-480, 706-710).   */
+/* ------------------------------------------------------------------------- */
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int64_t ref_leafbuild_traverse_bvh(const float *xyz, int32_t N, int32_t apl, float r,
+                                   int predicate, int morton_mode, const float lo[3],
+                                   const float scale[3], ref_pair *out, int64_t cap,
+                                   int nthreads, ref_tree_out *tree, double *timing) {
+    if (apl < 1 || N % apl != 0 || N / apl < 2) return REF_ERR_ARG;
+    int32_t nL = N / apl;
+    double t0 = now_s();
+    int32_t *codes = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    int rc = (morton_mode == REF_MORTON_MASKED) ? ref_morton_masked(xyz, N, codes)
+                                                : ref_morton_interleave(xyz, N, lo, scale, codes);
+    if (rc != REF_OK) { free(codes); return rc; }
+    int32_t *perm = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    ref_sortperm(codes, N, perm);
+    float *xs = (float *)malloc(sizeof(float) * 3 * (size_t)N);
+    int32_t *cs = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    for (int32_t s = 0; s < N; ++s) {
+        int32_t o = perm[s];
+        cs[s] = codes[o];
+        xs[3 * s + 0] = xyz[3 * o + 0];
+        xs[3 * s + 1] = xyz[3 * o + 1];
+        xs[3 * s + 2] = xyz[3 * o + 2];
+    }
+    ref_node *nodes = (ref_node *)malloc(sizeof(ref_node) * (size_t)(2 * nL - 1));
+    double t1 = now_s();
+    ref_build_tree(xs, cs, N, apl, r, nodes, 1, nthreads);
+    double t2 = now_s();
+    int64_t cnt = ref_traverse(nodes, xs, perm, N, apl, r, predicate, out, cap, nthreads, NULL);
+    double t3 = now_s();
+    if (tree) {
+        if (tree->nodes) memcpy(tree->nodes, nodes, sizeof(ref_node) * (size_t)(2 * nL - 1));
+        if (tree->perm) memcpy(tree->perm, perm, sizeof(int32_t) * (size_t)N);
+        if (tree->codes_sorted) memcpy(tree->codes_sorted, cs, sizeof(int32_t) * (size_t)N);
+    }
+    if (timing) { timing[0] = t1 - t0; timing[1] = t2 - t1; timing[2] = t3 - t2; }
+    free(codes); free(perm); free(xs); free(cs); free(nodes);
+    return cnt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* A25: unique_pairs + threshold_pairs (AllToAll.jl:65-88): all i<j in        */
+/* original order, d = sqrt(sum((a_i - a_j).^2)), keep d < threshold.         */
+/* ------------------------------------------------------------------------- */
+int64_t ref_bruteforce(const float *xyz, int32_t N, float r, ref_pair *out, int64_t cap) {
+    int64_t n = 0;
+    for (int32_t i = 0; i < N - 1; ++i)
+        for (int32_t j = i + 1; j < N; ++j) {
+            float d = sqrtf(ref_sqdist(xyz + 3 * (int64_t)i, xyz + 3 * (int64_t)j));
+            if (d < r) {
+                if (out && n < cap) { out[n].i = i; out[n].j = j; out[n].d = d; }
+                ++n;
+            }
+        }
+    return n;
+}

@@ -1,0 +1,168 @@
+"""GPU: forces and velocity-Verlet trajectories against the FP64 oracle.
+
+Tolerances (relative force error = ||F_gpu - F_64|| / ||F_64|| over all atoms):
+  FP32 form: 2e-4 at L ~ 17-69 (FP32 coordinate differences carry ~|x| * 6e-8)
+  MIXED form: 1e-5 (FP64 coordinate differences, FP32 pair maths, FP64 sums)
+  FP64 form: 1e-10
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(pkg, n, seed, coul=True):
+    L = pkg.synth.box_side(n)
+    x = pkg.synth.lattice_positions(n, L, seed)
+    v = pkg.synth.gaussian_velocities(n, 1.0, seed)
+    q = pkg.synth.balanced_charges(n, seed) if coul else None
+    return x, v, q, L
+
+
+def _spec(pkg, L, precision="fp32", steps=10, coul=True, **kw):
+    d = dict(duration=steps, stepwidth=0.001, rc=2.5, rc_coul=2.5 if coul else 0.0, skin=0.3,
+             rebuild_every=10, lo=0.0, hi=L, precision=precision, energy_every=1)
+    d.update(kw)
+    return pkg.SimSpec(**d)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-4), ("mixed", 1e-5), ("fp64", 1e-10)])
+def test_initial_forces_vs_oracle(pkg, O, precision, tol):
+    x, v, q, L = _cfg(pkg, 32768, 1)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision))
+    sim.set_state(x, v, q)
+    sim.run(0)
+    _, _, f = sim.get_state()
+    F, U, W, npairs = O.forces64(x, q, O.o64_params(rc_lj=2.5, rc_coul=2.5))
+    assert _rel(f, F) < tol, _rel(f, F)
+    e = sim.energies()
+    assert len(e) == 1 and e[0, 0] == 0
+    # MIXED sums FP32 pair terms in FP64: the virial's cancellations leave ~1e-7
+    etol = {"fp32": 1e-4, "mixed": 1e-6, "fp64": 1e-12}[precision]
+    assert abs(e[0, 2] - U) < etol * abs(U)
+    assert abs(e[0, 3] - W) < etol * abs(W)
+    # FP32 / MIXED test r^2 < rc^2 in single precision: a pair within an ulp
+    # of the cutoff may land on the other side
+    assert abs(int(e[0, 4]) - npairs) <= (0 if precision == "fp64" else 2)
+    sim.close()
+
+
+@pytest.mark.parametrize("precision,xtol", [("fp32", 2e-4), ("mixed", 1e-7), ("fp64", 1e-10)])
+def test_trajectory_vs_oracle(pkg, O, precision, xtol):
+    x, v, q, L = _cfg(pkg, 4096, 2)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision, steps=50))
+    sim.set_state(x, v, q)
+    sim.run(50)
+    xg, vg, fg = sim.get_state()
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 50,
+                               walls=True, lo=0.0, hi=L)
+    assert np.abs(xg - xr).max() < xtol
+    assert _rel(fg, fr) < (5e-4 if precision == "fp32" else 1e-5)
+    e = sim.energies()
+    assert len(e) == 51
+    ktol = {"fp32": 1e-4, "mixed": 1e-6, "fp64": 1e-10}[precision]
+    assert np.allclose(e[:, 1], E[:, 0], rtol=ktol)
+    sim.close()
+
+
+def test_graph_and_eager_identical(pkg):
+    x, v, q, L = _cfg(pkg, 8192, 3)
+    out = []
+    for g in (True, False):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", use_graph=g, energy_every=5))
+        sim.set_state(x, v, q)
+        for k in (7, 33, 40):  # windows start and end at different offsets
+            sim.run(k)
+        out.append(sim.get_state() + (sim.energies(),))
+        sim.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+def test_rebuild_period_and_walls(pkg, O):
+    x, v, q, L = _cfg(pkg, 4096, 4)
+    # the first lattice layers (0.54 from each x wall) move outwards at 25
+    # sigma/tau: they reach the wall within the 40 steps and are reflected
+    v = v.copy()
+    v[x[:, 0] < 0.8, 0] = -25.0
+    v[x[:, 0] > L - 0.8, 0] = 25.0
+    res = []
+    for every in (1, 10, 20):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp64", steps=40, rebuild_every=every,
+                                           skin=0.5 if every > 1 else 0.0))
+        sim.set_state(x, v, q)
+        sim.run(40)
+        xg, vg, fg = sim.get_state()
+        assert xg.min() >= 0.0 and xg.max() <= L
+        res.append(xg)
+        sim.close()
+    xr, _, _, _ = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 40,
+                             walls=True, lo=0.0, hi=L)
+    for xg in res:
+        assert np.abs(xg - xr).max() < 1e-8
+
+
+def test_list_overflow_retry(pkg):
+    x, v, q, L = _cfg(pkg, 4096, 5)
+    sims = []
+    for cap in (0, 5):
+        # cap = 5 forces the initial list to overflow and the run to resize
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp64", steps=25), max_neighbors=cap)
+        sim.set_state(x, v, q)
+        sim.run(25)
+        sims.append(sim.get_state())
+        st = sim.stats()
+        assert st["capacity"] >= st["max_neighbors"]
+        sim.close()
+    for a, b in zip(*sims):
+        assert np.abs(a - b).max() < 1e-12
+
+
+def test_thermostat_moves_temperature(pkg):
+    x, v, q, L = _cfg(pkg, 4096, 6, coul=False)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", coul=False, thermostat_every=10,
+                                       temperature=2.0, vDamp=1.0, energy_every=10))
+    sim.set_state(x, v * 0.5, None)
+    sim.run(100)
+    e = sim.energies()
+    T = 2 * e[:, 1] / (3 * len(x))
+    assert T[0] < 0.5 and abs(T[-1] - 2.0) < 0.5
+    sim.close()
+
+
+def test_full_size_forces_vs_oracle(pkg, O):
+    """N = 262,144 LJ + Coulomb after 20 steps: FP32 and MIXED forces vs FP64."""
+    c = pkg.synth.plain_scale(seed=7)
+    for precision, tol in (("fp32", 3e-4), ("mixed", 1e-5)):
+        sim = pkg.Simulation(c["n"], _spec(pkg, c["L"], precision, energy_every=10))
+        sim.set_state(c["x"], c["v"], c["q"])
+        sim.run(20)
+        xg, vg, fg = sim.get_state()
+        F, U, W, npairs = O.forces64(xg, c["q"], O.o64_params(rc_lj=2.5, rc_coul=2.5))
+        assert _rel(fg, F) < tol, (precision, _rel(fg, F))
+        e = sim.energies()
+        assert int(e[-1, 4]) == npairs
+        sim.close()
+
+
+def test_pair_list_force_api(pkg, O):
+    x, v, q, L = _cfg(pkg, 4096, 8)
+    xf = x.astype(np.float32)
+    spec = pkg.SpheresBVHSpecs(neighbor_distance=2.5, atom_count=len(x), atomsperleaf=4)
+    pairs = pkg.leafbuild_traverse_bvh(xf, spec, morton=pkg.MORTON_INTERLEAVE)
+    f_lj = np.zeros((len(x), 3), np.float32)
+    f_c = np.zeros((len(x), 3), np.float32)
+    e_lj = pkg.force_lennardjones_(f_lj, pairs, xf, rc=2.5)
+    e_c = pkg.force_coulomb_(f_c, pairs, q, xf, rc=2.5)
+    f = np.zeros_like(f_lj)
+    pkg.sum_forces_(f, f_lj, f_c)
+    x64 = xf.astype(np.float64)
+    F, U, W, _ = O.forces64(x64, q, O.o64_params(rc_lj=2.5, rc_coul=2.5))
+    assert _rel(f, F) < 1e-4
+    assert abs(e_lj + e_c - U) < 1e-4 * abs(U)
+    Flj, Ulj, _, _ = O.forces64(x64, None, O.o64_params(rc_lj=2.5))
+    assert _rel(f_lj, Flj) < 1e-4 and abs(e_lj - Ulj) < 1e-4 * abs(Ulj)
