@@ -39,11 +39,16 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+# Leaf size of the simulation's internal LBVH (a tuning parameter of this
+# implementation; the reference neighbour-list entry points keep their own).
+SIM_ATOMS_PER_LEAF = 16
+
+
 def make_spec(pkg, c, precision, key, energy_every=10, use_graph=True):
     return pkg.SimSpec(duration=0, stepwidth=c["dt"], rc=c["rc"], rc_coul=c["rc_coul"],
                        skin=c["skin"], rebuild_every=c["rebuild_every"], lo=0.0, hi=c["L"],
                        precision=precision, morton=key, energy_every=energy_every,
-                       use_graph=use_graph, walls=True)
+                       use_graph=use_graph, walls=True, atomsperleaf=SIM_ATOMS_PER_LEAF)
 
 
 def timed_run(pkg, sim, steps):
@@ -66,6 +71,33 @@ def gpu_form(pkg, c, precision, key, warmup, steps):
     rows = sim.energies()
     pairs = float(rows[:, 4].mean()) if len(rows) else float("nan")
     return sim, ms, wall, pairs
+
+
+def copy_calibration(pkg):
+    """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of
+    this kernel give the counters' scale factors."""
+    ctx = pkg.default_context()
+    nb = 256 << 20
+    a = pkg.DeviceArray(ctx, (nb,), np.uint8)
+    b = pkg.DeviceArray(ctx, (nb,), np.uint8)
+    a.zero()
+    for _ in range(5):
+        pkg._lib.check(pkg.library().nd_memcpy_d2d(ctx.handle, b.ptr, a.ptr, nb), ctx, "copy")
+    ctx.synchronize()
+    log("copied 5 x %d bytes" % nb)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused force kernel from the committed
+    counter summary (profiles/force_step_ljc262144_pmc.md), or None."""
+    path = os.path.join(ROOT, "profiles", "force_step_ljc262144_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(c, steps, threads):
@@ -91,6 +123,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the FP32 timed region (for profiler runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 5 x 256 MiB device-to-device (counter calibration run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -99,6 +133,9 @@ def main():
 
     import __graft_entry__ as ge
     pkg = ge.load_package()
+    if args.copy_calibration:
+        copy_calibration(pkg)
+        return
     c = pkg.synth.plain_scale(seed=2)
     n = c["n"]
     log("workload ljc-262144: N=%d L=%.4f" % (n, c["L"]))
@@ -120,7 +157,7 @@ def main():
         "config": {"workload": "ljc-262144", "n": n, "density": 0.8, "rc_lj": c["rc"],
                    "rc_coul": c["rc_coul"], "skin": c["skin"], "dt": c["dt"],
                    "rebuild_every": c["rebuild_every"], "lbvh_key": "interleave30",
-                   "atoms_per_leaf": 4, "precision": "fp32", "walls": True},
+                   "atoms_per_leaf": SIM_ATOMS_PER_LEAF, "precision": "fp32", "walls": True},
         "pair_evals_per_s": round(pairs * steps_per_s, 1),
         "pairs_in_cutoff_per_step": pairs,
         "list_pairs": st["list_entries"] // 2,
@@ -129,10 +166,12 @@ def main():
         print(json.dumps(result), flush=True)
         return
 
-    # ---- roofline of the fused force + integrate kernel (HIP events per launch) ----
-    sim.kernel_timing(1)
-    sim.run(200)
-    avg_ms, launches = sim.kernel_timing(0)
+    # ---- roofline of the fused force + integrate kernel: 20 rebuild windows,
+    # each timing 9 back-to-back fused launches between two HIP events ----
+    W = c["rebuild_every"]
+    times = [sim.time_fused_window(W) for _ in range(20)]
+    avg_ms = float(np.mean(times))
+    launches = 20 * (W - 1)
     st = sim.stats()
     plist = st["list_entries"] / 2.0
     bytes_launch = BYTES_PER_PAIR * plist + BYTES_PER_ATOM * n
@@ -140,7 +179,7 @@ def main():
     flops_launch = FLOPS_PER_PAIR_LJC * pairs
     result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                          "traffic": None, "kernel": "k_step<float,float,float,1,0>",
+                          "traffic": measured_traffic(), "kernel": "k_step<float,float,float,1,0>",
                           "avg_launch_ms": round(avg_ms, 5), "launches": launches,
                           "algorithmic_bytes_per_launch": int(bytes_launch),
                           "fp32_tflops": round(flops_launch / (avg_ms * 1e-3) / 1e12, 2),

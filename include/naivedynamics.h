@@ -222,6 +222,11 @@ int nd_sim_stats_get(nd_sim *sim, nd_sim_stats *out);
  * nd_sim_run, measured with HIP events around each launch when enabled. */
 int nd_sim_kernel_timing(nd_sim *sim, int32_t enable, double *avg_force_ms,
                          int64_t *launches);
+/* Advances nsteps (2..rebuild_every) steps as one rebuild window: rebuild,
+ * force, kick-drift, then nsteps-1 fused force+integrate launches back to
+ * back between two HIP events, then the final half kick.  *avg_fused_ms is
+ * the event time divided by nsteps-1 (the fused kernel's average duration). */
+int nd_sim_time_fused_window(nd_sim *sim, int32_t nsteps, double *avg_fused_ms);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,8 @@ size_t radix_scratch_blocks(int32_t n);
 // nd_lbvh.hip
 // status (may be null) receives kStatusTree if the codes break the build's
 // index invariants (only possible for caller-supplied codes with bit 31 set).
+// store must hold lbvh_scratch_ints(nL) int32.
+inline size_t lbvh_scratch_ints(int32_t nL) { return 4 * ((size_t)nL + 16); }
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
                  nd_node *nodes, int32_t *store, int32_t *status);

@@ -1,13 +1,23 @@
-// nd_lbvh.hip -- K3: leaf boxes fused with the Apetrei bottom-up LBVH build
+// nd_lbvh.hip -- K3: leaf boxes + Apetrei bottom-up LBVH build + box refit
 // (leafcluster_primitives, BVHTraverse.jl:376-410; exptdelta :676-696;
 // bvh_interior! :704-846; bounding_volume_hierarchy! :849-860).
 //
-// One lane per leaf.  A lane writes its leaf node, then climbs: at every
-// internal node the first lane to arrive parks its range end in store[] with
-// an agent-scope compare-and-swap and stops; the second lane (which sees the
-// first lane's node through the CAS's acquire half) merges the sibling box,
-// writes the parent node and continues.  The tree topology depends only on
-// the sorted codes, so it is identical to a serial build.
+// Topology pass, one lane per leaf: the lane writes its leaf node (box of
+// its atoms +- r/2, rope), then climbs.  At every internal node the first
+// lane to arrive parks its range end in store[] with a compare-and-swap and
+// stops; the second lane receives the sibling's range end from the CAS,
+// writes the parent's left child and rope, records the parent's leaf range
+// and continues.  The range end carried by the CAS is the only datum passed
+// between lanes, so a relaxed agent-scope CAS suffices: no lane reads memory
+// another lane wrote in this kernel, and no cache writeback/invalidate is
+// needed per level.  The topology depends only on the sorted codes, so it is
+// identical to the reference's serial or threaded build.
+//
+// Refit pass, one wave per internal node: the node's box is the min/max of
+// the leaf boxes over its contiguous leaf range.  That equals the reference's
+// pairwise bottom-up merge bit for bit (Julia's min/max on non-NaN floats,
+// -0 ordered below +0, is associative and commutative), and needs no
+// inter-node ordering.
 #include "nd_internal.hpp"
 
 namespace {
@@ -30,38 +40,18 @@ __device__ __forceinline__ int32_t rope(int32_t r, int32_t delr, const int32_t *
     return (delr < delta(n1, codes, nL, apl)) ? n1 : n1 + nL;
 }
 
-__device__ __forceinline__ void store_node(nd_node *nodes, int32_t k, const float mn[3],
-                                           const float mx[3], int32_t left, int32_t skip) {
-    float4 *p = reinterpret_cast<float4 *>(nodes + (k - 1));
-    p[0] = make_float4(mn[0], mn[1], mn[2], mx[0]);
-    p[1] = make_float4(mx[1], mx[2], __int_as_float(left), __int_as_float(skip));
-}
-
-__device__ __forceinline__ void merge_node(const nd_node *nodes, int32_t k, float mn[3],
-                                           float mx[3]) {
-    const float4 *p = reinterpret_cast<const float4 *>(nodes + (k - 1));
-    const float4 a = p[0];
-    const float4 b = p[1];
-    // max.(keys[child].max, bounding) / min.(keys[child].min, bounding)
-    mx[0] = jl_max(a.w, mx[0]);
-    mx[1] = jl_max(b.x, mx[1]);
-    mx[2] = jl_max(b.y, mx[2]);
-    mn[0] = jl_min(a.x, mn[0]);
-    mn[1] = jl_min(a.y, mn[1]);
-    mn[2] = jl_min(a.z, mn[2]);
-}
-
-__device__ __forceinline__ int32_t cas_acq_rel(int32_t *p, int32_t desired) {
+__device__ __forceinline__ int32_t cas_relaxed(int32_t *p, int32_t desired) {
     int32_t expected = 0;
-    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_ACQ_REL,
-                                         __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return expected;  // old value: 0 if this lane arrived first
 }
 
 __global__ void __launch_bounds__(nd::kBlock)
-k_lbvh(const float *__restrict__ xyz, int32_t stride, const int32_t *__restrict__ codes,
-       int32_t nL, int32_t apl, float h, int32_t override_root, nd_node *nodes,
-       int32_t *store, int32_t *status) {
+k_lbvh_topology(const float *__restrict__ xyz, int32_t stride, const int32_t *__restrict__ codes,
+                int32_t nL, int32_t apl, float h, nd_node *__restrict__ nodes,
+                int32_t *__restrict__ store, int2 *__restrict__ ranges,
+                int32_t *__restrict__ status) {
     int32_t i = 1 + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i > nL) return;
 
@@ -78,11 +68,14 @@ k_lbvh(const float *__restrict__ xyz, int32_t stride, const int32_t *__restrict_
             mx[d] = jl_max(mx[d], p[d] + h);
         }
     }
-
     int32_t rangel = i, ranger = i;
     int32_t dell = delta(rangel - 1, codes, nL, apl);
     int32_t delr = delta(ranger, codes, nL, apl);
-    store_node(nodes, i, mn, mx, 0, rope(i, delr, codes, nL, apl));
+    {
+        float4 *q = reinterpret_cast<float4 *>(nodes + (i - 1));
+        q[0] = make_float4(mn[0], mn[1], mn[2], mx[0]);
+        q[1] = make_float4(mx[1], mx[2], __int_as_float(0), __int_as_float(rope(i, delr, codes, nL, apl)));
+    }
 
     // Every climb step strictly widens [rangel, ranger], so a walker takes at
     // most nL steps; the bound and the index checks only matter for codes
@@ -93,40 +86,143 @@ k_lbvh(const float *__restrict__ xyz, int32_t stride, const int32_t *__restrict_
             leftChild = i;
             const int32_t split = ranger;
             if (split < 1 || split > nL - 1) break;
-            const int32_t other = cas_acq_rel(&store[split - 1], rangel);
+            const int32_t other = cas_relaxed(&store[split - 1], rangel);
             if (other == 0) return;
             if (other < 1 || other > nL) break;
             ranger = other;
             delr = delta(ranger, codes, nL, apl);
-            int32_t rightChild = split + 1;
-            if (rightChild != ranger) rightChild += nL;
-            merge_node(nodes, rightChild, mn, mx);
         } else {  // right child; sibling to the left
             const int32_t split = rangel - 1;
             if (split < 1 || split > nL - 1) break;
-            const int32_t other = cas_acq_rel(&store[split - 1], ranger);
+            const int32_t other = cas_relaxed(&store[split - 1], ranger);
             if (other == 0) return;
             if (other < 1 || other > nL) break;
             rangel = other;
             dell = delta(rangel - 1, codes, nL, apl);
             leftChild = split;
             if (leftChild != rangel) leftChild += nL;
-            merge_node(nodes, leftChild, mn, mx);
         }
         const int32_t q = (delr < dell) ? ranger : rangel;
         if (q < 1 || q > nL - 1) break;
         const int32_t parent = q + nL;
-        const int32_t skip = rope(ranger, delr, codes, nL, apl);
-        if (parent == nL + 1 && override_root) {
-            const float z[3] = {0.0f, 0.0f, 0.0f}, o[3] = {1.0f, 1.0f, 1.0f};
-            store_node(nodes, parent, z, o, leftChild, skip);
-        } else {
-            store_node(nodes, parent, mn, mx, leftChild, skip);
-        }
+        *reinterpret_cast<int2 *>(&nodes[parent - 1].left) =
+            make_int2(leftChild, rope(ranger, delr, codes, nL, apl));
+        ranges[q - 1] = make_int2(rangel, ranger);
         i = parent;
         if (i == nL + 1) return;
     }
     if (status) atomicOr(status, nd::kStatusTree);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = jl_min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = jl_max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+struct Box6 {
+    float mn0, mn1, mn2, mx0, mx1, mx2;
+    __device__ void init() {
+        const float inf = __int_as_float(0x7F800000);
+        mn0 = mn1 = mn2 = inf;
+        mx0 = mx1 = mx2 = -inf;
+    }
+    // node/chunk box stored as (min xyz, max x) + (max y, max z, ...)
+    __device__ void merge(const float4 a, const float4 b) {
+        mn0 = jl_min(mn0, a.x); mn1 = jl_min(mn1, a.y); mn2 = jl_min(mn2, a.z);
+        mx0 = jl_max(mx0, a.w); mx1 = jl_max(mx1, b.x); mx2 = jl_max(mx2, b.y);
+    }
+    __device__ void wave_reduce() {
+        mn0 = wave_min(mn0); mn1 = wave_min(mn1); mn2 = wave_min(mn2);
+        mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
+    }
+};
+
+constexpr int kChunk = 64;  // leaves per refit chunk
+
+// Box of every aligned chunk of 64 leaves (one wave per chunk), stored in the
+// same 32-byte layout as a node.
+__global__ void __launch_bounds__(nd::kBlock)
+k_lbvh_chunks(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__ chunks) {
+    const int32_t c = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if ((int64_t)c * kChunk >= nL) return;
+    Box6 b;
+    b.init();
+    const int32_t l = c * kChunk + lane;  // 0-based leaf
+    if (l < nL) {
+        const float4 *p = reinterpret_cast<const float4 *>(nodes + l);
+        b.merge(p[0], p[1]);
+    }
+    b.wave_reduce();
+    if (lane == 0) {
+        chunks[2 * c] = make_float4(b.mn0, b.mn1, b.mn2, b.mx0);
+        chunks[2 * c + 1] = make_float4(b.mx1, b.mx2, 0.0f, 0.0f);
+    }
+}
+
+constexpr int kGroup = 8;  // lanes per internal node in the refit
+
+__device__ __forceinline__ float group_min(float v) {
+#pragma unroll
+    for (int off = kGroup / 2; off > 0; off >>= 1) v = jl_min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float group_max(float v) {
+#pragma unroll
+    for (int off = kGroup / 2; off > 0; off >>= 1) v = jl_max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Eight lanes per internal node: leaves before the first whole chunk, whole
+// chunks, leaves after the last whole chunk (at most 63 + nL/64 + 63 reads).
+__global__ void __launch_bounds__(nd::kBlock)
+k_lbvh_refit(nd_node *__restrict__ nodes, const int2 *__restrict__ ranges,
+             const float4 *__restrict__ chunks, int32_t nL, int32_t override_root) {
+    const int32_t q = 1 + (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kGroup);
+    const int lane = threadIdx.x & (kGroup - 1);
+    if (q > nL - 1) return;  // the whole 8-lane group exits together
+    const int2 rg = ranges[q - 1];
+    // only reachable with an incomplete topology (status already flagged)
+    if (rg.x < 1 || rg.y > nL || rg.x > rg.y) return;
+    const int32_t l0 = rg.x - 1, r0 = rg.y - 1;           // 0-based inclusive leaf range
+    const int32_t c0 = (l0 + kChunk - 1) / kChunk;         // first whole chunk
+    const int32_t c1 = (r0 + 1) / kChunk;                  // one past the last whole chunk
+    Box6 b;
+    b.init();
+    const nd_node *leaves = nodes;
+    if (c0 < c1) {
+        for (int32_t l = l0 + lane; l < c0 * kChunk; l += kGroup) {
+            const float4 *p = reinterpret_cast<const float4 *>(leaves + l);
+            b.merge(p[0], p[1]);
+        }
+        for (int32_t c = c0 + lane; c < c1; c += kGroup) b.merge(chunks[2 * c], chunks[2 * c + 1]);
+        for (int32_t l = c1 * kChunk + lane; l <= r0; l += kGroup) {
+            const float4 *p = reinterpret_cast<const float4 *>(leaves + l);
+            b.merge(p[0], p[1]);
+        }
+    } else {
+        for (int32_t l = l0 + lane; l <= r0; l += kGroup) {
+            const float4 *p = reinterpret_cast<const float4 *>(leaves + l);
+            b.merge(p[0], p[1]);
+        }
+    }
+    float mn0 = group_min(b.mn0), mn1 = group_min(b.mn1), mn2 = group_min(b.mn2);
+    float mx0 = group_max(b.mx0), mx1 = group_max(b.mx1), mx2 = group_max(b.mx2);
+    if (lane == 0) {
+        if (q == 1 && override_root) {  // BVHTraverse.jl:856-858
+            mn0 = mn1 = mn2 = 0.0f;
+            mx0 = mx1 = mx2 = 1.0f;
+        }
+        float *p = reinterpret_cast<float *>(nodes + (q + nL - 1));
+        *reinterpret_cast<float4 *>(p) = make_float4(mn0, mn1, mn2, mx0);
+        *reinterpret_cast<float2 *>(p + 4) = make_float2(mx1, mx2);
+    }
 }
 
 }  // namespace
@@ -136,10 +232,20 @@ namespace nd {
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
                  nd_node *nodes, int32_t *store, int32_t *status) {
+    // store (lbvh_scratch_ints(nL) ints): nL-1 CAS slots | nL-1 int2 leaf
+    // ranges | one 32-byte box per 64-leaf chunk
     launch_fill_i32(s, store, nL - 1, 0);
+    const size_t off_r = ((size_t)nL + 3) & ~(size_t)3;
+    const size_t off_c = (off_r + 2 * (size_t)nL + 3) & ~(size_t)3;
+    int2 *ranges = reinterpret_cast<int2 *>(store + off_r);
+    float4 *chunks = reinterpret_cast<float4 *>(store + off_c);
     const float h = r / 2.0f;  // neighbor_distance/2 in Float32
-    k_lbvh<<<grid_for(nL), kBlock, 0, s>>>(xyz_sorted, xyz_stride, codes_sorted, nL, apl, h,
-                                          override_root, nodes, store, status);
+    k_lbvh_topology<<<grid_for(nL), kBlock, 0, s>>>(xyz_sorted, xyz_stride, codes_sorted, nL, apl, h,
+                                                   nodes, store, ranges, status);
+    const int64_t nchunks = (nL + kChunk - 1) / kChunk;
+    k_lbvh_chunks<<<grid_for(nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);
+    k_lbvh_refit<<<grid_for((int64_t)(nL - 1) * kGroup), kBlock, 0, s>>>(nodes, ranges, chunks, nL,
+                                                                          override_root);
     ND_HIP(hipGetLastError());
 }
 
@@ -158,7 +264,7 @@ extern "C" int nd_bounding_volume_hierarchy(nd_ctx *ctx, const float *d_xyz_sort
                "nd_bounding_volume_hierarchy: invalid SpheresBVHSpecs");
     ND_HIP(hipSetDevice(ctx->device));
     const int32_t nL = chk.leaves_count;
-    ctx->store.reserve(sizeof(int32_t) * (size_t)nL);
+    ctx->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(nL));
     nd::launch_lbvh(ctx->stream, d_xyz_sorted, 3, d_codes_sorted, nL, chk.atomsperleaf,
                     chk.neighbor_distance, override_root, d_nodes, ctx->store.as<int32_t>(),
                     ctx->status.as<int32_t>());

@@ -9,8 +9,10 @@
 //   vel[2]  float4 / double4 (vx, vy, vz, 1/m), double-buffered for re-sorting
 //   frc     float4 / double4 force of the last completed step
 //   orig[2] int32 original atom index of each slot
-//   nbr     int32 [cap][n] neighbour slots, transposed so that the k-th
-//           neighbours of 64 consecutive atoms are one contiguous 256-byte row
+//   nbr     int32 neighbour slots, one block of [cap][64] per 64 consecutive
+//           atoms: row k of a block holds the k-th neighbours of the block's
+//           atoms (one contiguous 256-byte read per k for a wave), and rows of
+//           different k are adjacent rather than a power of two apart
 //   ncount  int32 [n]
 // Rebuild (every rebuild_every steps): key -> radix sort -> permute state ->
 // LBVH (leaf boxes +-r_list/2, nd_lbvh.hip) -> one lane per atom walks the
@@ -103,6 +105,54 @@ __device__ __forceinline__ void block_sum4(double v[4], double (*sh)[4]) {
         }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap
+// the workgroup index so that XCD x processes one contiguous slab of atoms.
+// Atoms are in Morton order, so each XCD's neighbour gathers then stay in a
+// compact region whose positions fit its own L2.
+__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nblocks) {
+    constexpr int32_t kXcd = 8;
+    const int32_t per = nblocks / kXcd, rem = nblocks % kXcd;
+    const int32_t x = b % kXcd, k = b / kXcd;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+// One neighbour's contribution: LJ (r < rc) + Coulomb (r < rc_coul), both
+// evaluated without branches and masked, so the loads of the next neighbours
+// can be issued before this one finishes.
+template <class PosS, class CalcS, class AccS, bool ENERGY>
+struct PairAcc {
+    using P4 = typename V4<PosS>::T;
+    CalcS rc2, rcc2, sigma2, eps24, eps4, kcq, eljs, ecs;
+    P4 pi;
+    AccS fx = 0, fy = 0, fz = 0, e = 0, w = 0;
+    int32_t np = 0;
+    __device__ __forceinline__ void add(const P4 &pj) {
+        const CalcS dx = (CalcS)(pi.x - pj.x);
+        const CalcS dy = (CalcS)(pi.y - pj.y);
+        const CalcS dz = (CalcS)(pi.z - pj.z);
+        const CalcS r2 = dx * dx + dy * dy + dz * dz;
+        const CalcS ir2 = rcp_(r2);
+        const CalcS sr2 = sigma2 * ir2;
+        const CalcS sr6 = sr2 * sr2 * sr2;
+        const bool in_lj = r2 < rc2, in_c = r2 < rcc2;
+        const CalcS flj = in_lj ? eps24 * ir2 * sr6 * (sr6 + sr6 - (CalcS)1) : (CalcS)0;
+        const CalcS qq = kcq * (CalcS)pj.w;
+        const CalcS ir = rsq_(r2);
+        const CalcS fc = in_c ? qq * ir * ir2 : (CalcS)0;
+        const CalcS fs = flj + fc;
+        fx += (AccS)(fs * dx);
+        fy += (AccS)(fs * dy);
+        fz += (AccS)(fs * dz);
+        if (ENERGY) {
+            const CalcS elj = in_lj ? eps4 * sr6 * (sr6 - (CalcS)1) - eljs : (CalcS)0;
+            const CalcS ec = in_c ? qq * (ir - ecs) : (CalcS)0;
+            e += (AccS)(elj + ec);
+            w += (AccS)(fs * r2);
+            np += (in_lj | in_c) ? 1 : 0;
+        }
+    }
+};
+
 // PosS: storage scalar of positions/velocities; CalcS: pair arithmetic;
 // AccS: per-atom accumulation.  FP32 = <float,float,float>,
 // MIXED = <double,float,double>, FP64 = <double,double,double>.
@@ -115,52 +165,48 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     using P4 = typename V4<PosS>::T;
     using Q = PP<CalcS>;
     __shared__ double sh[nd::kBlock / 64][4];
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t i = blk * blockDim.x + threadIdx.x;
     const int32_t n = P.n;
     AccS fx = 0, fy = 0, fz = 0;
     double pe = 0.0, vir = 0.0, ke = 0.0, np = 0.0;
     if (i < n) {
-        const P4 pi = pos_in[i];
-        const CalcS qi = (CalcS)pi.w;
-        const CalcS rcmax2 = Q::rcmax2(P), rc2 = Q::rc2(P), rcc2 = Q::rcc2(P);
-        const CalcS sigma2 = Q::sigma2(P), eps24 = Q::eps24(P), eps4 = Q::eps4(P), kc = Q::kc(P);
+        PairAcc<PosS, CalcS, AccS, ENERGY> acc;
+        acc.pi = pos_in[i];
+        acc.rc2 = Q::rc2(P);
+        acc.rcc2 = Q::rcc2(P);
+        acc.sigma2 = Q::sigma2(P);
+        acc.eps24 = Q::eps24(P);
+        acc.eps4 = Q::eps4(P);
+        acc.kcq = Q::kc(P) * (CalcS)acc.pi.w;
+        acc.eljs = Q::eljs(P);
+        acc.ecs = Q::ecs(P);
         int32_t cnt = ncount[i];
         cnt = cnt < P.cap ? cnt : P.cap;
-        AccS epair = 0, vpair = 0;  // FP64 sums in the MIXED and FP64 forms
-        int32_t npair = 0;
-        const int32_t *col = nbr + i;
-#pragma unroll 4
-        for (int32_t k = 0; k < cnt; ++k) {
-            const int32_t j = col[(int64_t)k * n];
-            const P4 pj = pos_in[j];
-            const CalcS dx = (CalcS)(pi.x - pj.x);
-            const CalcS dy = (CalcS)(pi.y - pj.y);
-            const CalcS dz = (CalcS)(pi.z - pj.z);
-            const CalcS r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < rcmax2) {
-                const CalcS ir2 = rcp_(r2);
-                CalcS fs = 0;
-                if (r2 < rc2) {
-                    const CalcS sr2 = sigma2 * ir2;
-                    const CalcS sr6 = sr2 * sr2 * sr2;
-                    fs = eps24 * ir2 * sr6 * (sr6 + sr6 - (CalcS)1);
-                    if (ENERGY) epair += (AccS)(eps4 * sr6 * (sr6 - (CalcS)1) - Q::eljs(P));
-                }
-                if (r2 < rcc2) {
-                    const CalcS qq = kc * qi * (CalcS)pj.w;
-                    const CalcS ir = rsq_(r2);
-                    fs += qq * ir * ir2;
-                    if (ENERGY) epair += (AccS)(qq * (ir - Q::ecs(P)));
-                }
-                fx += (AccS)(fs * dx);
-                fy += (AccS)(fs * dy);
-                fz += (AccS)(fs * dz);
-                if (ENERGY) {
-                    vpair += (AccS)(fs * r2);
-                    npair += 1;
-                }
-            }
+        // this wave's block of the list: row k = 64 consecutive int32
+        const int32_t *col = nbr + ((int64_t)(i >> 6) * P.cap) * 64 + (i & 63);
+        int32_t k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            const int32_t j0 = col[(k + 0) * 64];
+            const int32_t j1 = col[(k + 1) * 64];
+            const int32_t j2 = col[(k + 2) * 64];
+            const int32_t j3 = col[(k + 3) * 64];
+            const P4 p0 = pos_in[j0];
+            const P4 p1 = pos_in[j1];
+            const P4 p2 = pos_in[j2];
+            const P4 p3 = pos_in[j3];
+            acc.add(p0);
+            acc.add(p1);
+            acc.add(p2);
+            acc.add(p3);
         }
+        for (; k < cnt; ++k) acc.add(pos_in[col[k * 64]]);
+        fx = acc.fx;
+        fy = acc.fy;
+        fz = acc.fz;
+        const AccS epair = acc.e, vpair = acc.w;
+        const int32_t npair = acc.np;
+        const P4 pi = acc.pi;
         // each unordered pair is visited from both atoms: halve the pair sums
         pe = 0.5 * (double)epair;
         vir = 0.5 * (double)vpair;
@@ -267,36 +313,43 @@ k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>:
 }
 
 // Neumaier-compensated sum of the per-workgroup partials; appends one energy
-// row (step, KE, PE, virial, pairs) and keeps KE for the thermostat.
+// row (step, KE, PE, virial, pairs) and keeps KE for the thermostat.  Each
+// lane sums a strided share with compensation, then the (sum, error) pairs
+// are merged pairwise with an exact TwoSum in a fixed tree order.
+__device__ __forceinline__ void two_sum_merge(double &s, double &c, double s2, double c2) {
+    const double t = s + s2;
+    const double bp = t - s;
+    const double err = (s - (t - bp)) + (s2 - bp);
+    s = t;
+    c = c + c2 + err;
+}
+
 __global__ void __launch_bounds__(nd::kBlock)
 k_energy_finalize(const double *__restrict__ partials, int32_t nblocks, double *__restrict__ rows,
                   int32_t max_rows, SimMisc *__restrict__ misc) {
-    __shared__ double ss[nd::kBlock][4], cc[nd::kBlock][4];
+    __shared__ double ss[4][nd::kBlock], cc[4][nd::kBlock];
     double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
     for (int32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double x = partials[4 * (int64_t)b + q];
-            const double t = s[q] + x;
-            c[q] += fabs(s[q]) >= fabs(x) ? (s[q] - t) + x : (x - t) + s[q];
-            s[q] = t;
-        }
+        for (int q = 0; q < 4; ++q) two_sum_merge(s[q], c[q], partials[4 * (int64_t)b + q], 0.0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { ss[threadIdx.x][q] = s[q]; cc[threadIdx.x][q] = c[q]; }
+    for (int q = 0; q < 4; ++q) { ss[q][threadIdx.x] = s[q]; cc[q][threadIdx.x] = c[q]; }
     __syncthreads();
+    for (int half = nd::kBlock / 2; half > 0; half >>= 1) {
+        if ((int)threadIdx.x < half) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double a = ss[q][threadIdx.x], e = cc[q][threadIdx.x];
+                two_sum_merge(a, e, ss[q][threadIdx.x + half], cc[q][threadIdx.x + half]);
+                ss[q][threadIdx.x] = a;
+                cc[q][threadIdx.x] = e;
+            }
+        }
+        __syncthreads();
+    }
     if (threadIdx.x != 0) return;
     double tot[4];
-    for (int q = 0; q < 4; ++q) {
-        double S = 0.0, C = 0.0;
-        for (int k = 0; k < (int)blockDim.x; ++k) {
-            const double x = ss[k][q];
-            const double t = S + x;
-            C += fabs(S) >= fabs(x) ? (S - t) + x : (x - t) + S;
-            S = t;
-            C += cc[k][q];
-        }
-        tot[q] = S + C;
-    }
+    for (int q = 0; q < 4; ++q) tot[q] = ss[q][0] + cc[q][0];
     const int32_t r = misc->rowctr;
     if (r < max_rows) {
         double *row = rows + 5 * (int64_t)r;
@@ -358,55 +411,89 @@ k_permute(const int32_t *__restrict__ perm, int32_t n, const typename V4<PosS>::
     if (pos32) pos32[s] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
 }
 
-// One lane per atom: stackless walk from the root over the +-r/2 boxes,
-// appending every slot j != i with |x_i - x_j|^2 < r^2 to column i.
+// One wave per 64 consecutive atoms (Morton-adjacent, so spatially compact):
+// the wave walks the tree from the root as a unit.  The current node is
+// wave-uniform (one scalar load per visited node instead of 64 divergent
+// pointer chases); every lane tests its own box (atom +- r/2) against it and
+// the wave descends where any lane overlaps.  At a leaf each overlapping lane
+// appends every slot j != i with |x_i - x_j|^2 < r^2 to its column.  A lane
+// gets exactly the list its own walk would give: leaves it does not overlap
+// are skipped by its own box test.
+// A = leaf size known at compile time (0: runtime `apl`).  With A > 0 the
+// first min(A, 8) atoms of a leaf are fetched in the same batch of scalar
+// loads as its node (leaves are numbered 1..nL, so a leaf is recognised
+// before its node arrives), so a leaf visit costs one memory latency.
+template <int A>
 __global__ void __launch_bounds__(nd::kBlock)
 k_nlist(const float4 *__restrict__ pos, const nd_node *__restrict__ nodes, int32_t n, int32_t nL,
         int32_t apl, float h, float rl2, int32_t cap, int32_t *__restrict__ nbr,
         int32_t *__restrict__ ncount, SimMisc *__restrict__ misc) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int PF = (A > 0 && A < 8) ? A : 8;
+    const int32_t la = A > 0 ? A : apl;
+    // contiguous Morton slab per XCD: its subtree and positions stay in its L2
+    const int32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    // the wave's block of the list: cap rows of 64 int32 (row k = k-th
+    // neighbours of the wave's 64 atoms), so rows never sit a power of two apart
+    int32_t *lst = nbr + ((int64_t)(i >> 6) * cap) * 64 + (i & 63);
+    const float4 p = active ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float qmn0 = p.x - h, qmn1 = p.y - h, qmn2 = p.z - h;
+    const float qmx0 = p.x + h, qmx1 = p.y + h, qmx2 = p.z + h;
     int32_t cnt = 0;
-    if (i < n) {
-        const float4 p = pos[i];
-        const float qmn0 = p.x - h, qmn1 = p.y - h, qmn2 = p.z - h;
-        const float qmx0 = p.x + h, qmx1 = p.y + h, qmx2 = p.z + h;
-        int32_t t = nL + 1;  // root
-        int32_t visits = 0;
-        while (t != 0) {
-            // each node is visited at most once per query (DFS order)
-            if (++visits > 2 * nL || t < 1 || t > 2 * nL - 1) {
-                atomicOr(&misc->status, nd::kStatusTree);
-                break;
-            }
-            const float4 *np4 = reinterpret_cast<const float4 *>(nodes + (t - 1));
-            const float4 a = np4[0];
-            const float4 b = np4[1];
-            const bool ov = (qmn0 < a.w) & (qmx0 > a.x) & (qmn1 < b.x) & (qmx1 > a.y) &
-                            (qmn2 < b.y) & (qmx2 > a.z);
-            const int32_t left = __float_as_int(b.z), skip = __float_as_int(b.w);
-            if (ov) {
-                if (left == 0) {
-                    const int32_t base = (t - 1) * apl;
-                    for (int32_t m = 0; m < apl; ++m) {
-                        const int32_t j = base + m;
-                        const float4 q = pos[j];
-                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                        const float s = dx * dx + dy * dy + dz * dz;
-                        if (j != i && s < rl2) {
-                            if (cnt < cap) nbr[(int64_t)cnt * n + i] = j;
-                            ++cnt;
-                        }
-                    }
-                    t = skip;
-                } else {
-                    t = left;
-                }
-            } else {
-                t = skip;
-            }
+    auto test = [&](const float4 &q, int32_t j) {
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float s = dx * dx + dy * dy + dz * dz;
+        if (j != i && s < rl2) {
+            if (cnt < cap) lst[cnt * 64] = j;
+            ++cnt;
         }
-        ncount[i] = cnt;
+    };
+    int32_t t = nL + 1;  // root
+    int32_t visits = 0;
+    while (t != 0) {
+        t = __builtin_amdgcn_readfirstlane(t);
+        // each node is visited at most once per walk (DFS order)
+        if (++visits > 2 * nL || t < 1 || t > 2 * nL - 1) {
+            if ((threadIdx.x & 63) == 0) atomicOr(&misc->status, nd::kStatusTree);
+            break;
+        }
+        const float4 *np4 = reinterpret_cast<const float4 *>(nodes + (t - 1));
+        const float4 a = np4[0];
+        const float4 b = np4[1];
+        const bool leaf = t <= nL;
+        const int32_t base = (t - 1) * la;
+        float4 q[PF];
+        if (A > 0 && leaf) {
+#pragma unroll
+            for (int m = 0; m < PF; ++m) q[m] = pos[base + m];
+        }
+        const bool ov = active & (qmn0 < a.w) & (qmx0 > a.x) & (qmn1 < b.x) & (qmx1 > a.y) &
+                        (qmn2 < b.y) & (qmx2 > a.z);
+        const int32_t left = __float_as_int(b.z), skip = __float_as_int(b.w);
+        if (__ballot(ov) == 0ull) {
+            t = skip;
+        } else if (left != 0) {
+            t = left;
+        } else {
+            if (ov) {
+                if (A > 0) {
+#pragma unroll
+                    for (int m = 0; m < PF; ++m) test(q[m], base + m);
+                    for (int m0 = PF; m0 < A; m0 += 8) {
+                        float4 r8[8];
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) r8[m] = pos[base + m0 + m];
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) test(r8[m], base + m0 + m);
+                    }
+                } else {
+                    for (int32_t m = 0; m < la; ++m) test(pos[base + m], base + m);
+                }
+            }
+            t = skip;
+        }
     }
+    if (active) ncount[i] = cnt;
     // wave-level max, one atomic per wave
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
@@ -463,7 +550,8 @@ void drop_graphs(nd_sim *s) {
 void alloc_lists(nd_sim *s, int32_t cap) {
     s->cap = cap;
     s->sp.cap = cap;
-    s->nbr.reserve(sizeof(int32_t) * (size_t)cap * (size_t)s->n);
+    // one block of cap rows x 64 slots per 64 atoms (n rounded up to 64)
+    s->nbr.reserve(sizeof(int32_t) * (size_t)cap * (((size_t)s->n + 63) & ~(size_t)63));
     drop_graphs(s);
 }
 
@@ -511,9 +599,17 @@ void launch_rebuild(nd_sim *s) {
                                : reinterpret_cast<const float *>(s->pos[s->cur].p);
     nd::launch_lbvh(st, p32, 4, codes, s->nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
                     s->store.as<int32_t>(), &misc->status);
-    k_nlist<<<g, nd::kBlock, 0, st>>>(reinterpret_cast<const float4 *>(p32), s->nodes.as<nd_node>(), n,
-                                      s->nL, s->apl, s->rlist / 2.0f, s->rlist * s->rlist, s->cap,
-                                      s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), misc);
+    const float4 *p4 = reinterpret_cast<const float4 *>(p32);
+    const float hl = s->rlist / 2.0f, rl2 = s->rlist * s->rlist;
+    int32_t *nb = s->nbr.as<int32_t>(), *nc = s->ncount.as<int32_t>();
+    nd_node *nodes = s->nodes.as<nd_node>();
+    switch (s->apl) {
+        case 4: k_nlist<4><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 4, hl, rl2, s->cap, nb, nc, misc); break;
+        case 8: k_nlist<8><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 8, hl, rl2, s->cap, nb, nc, misc); break;
+        case 16: k_nlist<16><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 16, hl, rl2, s->cap, nb, nc, misc); break;
+        case 32: k_nlist<32><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 32, hl, rl2, s->cap, nb, nc, misc); break;
+        default: k_nlist<0><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, s->apl, hl, rl2, s->cap, nb, nc, misc);
+    }
     ND_HIP(hipGetLastError());
 }
 
@@ -822,7 +918,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->codes.reserve(sizeof(int32_t) * n);
         s->perm.reserve(sizeof(int32_t) * n);
         s->nodes.reserve(sizeof(nd_node) * (size_t)(2 * s->nL - 1));
-        s->store.reserve(sizeof(int32_t) * (size_t)s->nL);
+        s->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(s->nL));
         s->ncount.reserve(sizeof(int32_t) * n);
         s->misc.reserve(sizeof(SimMisc));
         s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
@@ -830,7 +926,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         const int32_t ntiles = (int32_t)nd::radix_scratch_blocks(s->n);
         s->sort.keys_alt.reserve(sizeof(int32_t) * n);
         s->sort.vals_alt.reserve(sizeof(int32_t) * n);
-        s->sort.hist.reserve(sizeof(int32_t) * 256 * (size_t)ntiles);
+        s->sort.hist.reserve(sizeof(int32_t) * (256 * (size_t)ntiles + 256));
         ND_HIP(hipMemsetAsync(s->misc.p, 0, sizeof(SimMisc), ctx->stream));
         int32_t cap = p->max_neighbors;
         if (cap <= 0) {
@@ -1047,6 +1143,41 @@ int nd_sim_stats_get(nd_sim *s, nd_sim_stats *out) {
     out->max_neighbors = s->last_maxcnt;
     out->capacity = s->cap;
     out->pairs_in_cutoff = s->pairs_last;
+    ND_API_END(ctx)
+}
+
+int nd_sim_time_fused_window(nd_sim *s, int32_t nsteps, double *avg_fused_ms) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s && avg_fused_ms, "nd_sim_time_fused_window: null argument");
+    ND_REQUIRE(s->have_state, "nd_sim_time_fused_window: call nd_sim_set_state first");
+    ND_REQUIRE(nsteps >= 2 && nsteps <= s->p.rebuild_every,
+               "nd_sim_time_fused_window: need 2 <= nsteps <= rebuild_every");
+    ND_HIP(hipSetDevice(ctx->device));
+    const bool timing = s->timing;
+    s->timing = false;
+    rebuild_checked(s);  // fresh list for this window (checks capacity)
+    s->last_build_step = s->step;
+    launch_force_step(s, kModeForce, s->step);
+    launch_kick_drift(s, false);
+    ND_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int32_t o = 1; o < nsteps; ++o) launch_step_prec<kModeFused, false>(s), s->cur ^= 1;
+    ND_HIP(hipGetLastError());
+    ND_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    launch_force_step(s, kModeFinal, s->step + nsteps);
+    ND_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    ND_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *avg_fused_ms = ms / (double)(nsteps - 1);
+    s->step += nsteps;
+    s->have_force = true;
+    s->timing = timing;
+    SimMisc m;
+    read_misc(s, m);  // kernels are stream-ordered; this also syncs the final step
+    // keep the device step counter in line with the host's
+    m.step = s->step;
+    ND_HIP(hipMemcpyAsync(s->misc.p, &m, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
+    ND_HIP(hipStreamSynchronize(ctx->stream));
     ND_API_END(ctx)
 }
 

@@ -36,38 +36,54 @@ k_radix_hist(const int32_t *__restrict__ keys, int32_t n, int shift, int32_t nti
     hist[(int64_t)t * ntiles + blockIdx.x] = cnt[t];
 }
 
-// Exclusive scan of hist[0..total) in place with one workgroup of 1024 lanes.
-__global__ void __launch_bounds__(1024) k_radix_scan(int32_t *__restrict__ hist, int64_t total) {
-    __shared__ int32_t part[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (total + 1023) / 1024;
-    const int64_t lo = t * per, hi = lo + per < total ? lo + per : total;
-    int32_t s = 0;
-    for (int64_t k = lo; k < hi; ++k) s += hist[k];
-    part[t] = s;
+// Exclusive scan of v over the 256 lanes of a workgroup; returns the prefix
+// and leaves the workgroup total in *total (shared).
+__device__ __forceinline__ int32_t block_exclusive_scan256(int32_t v, int32_t *wsum, int32_t *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t u = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += u;
+    }
+    if (lane == 63) wsum[w] = inc;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int32_t v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
+    int32_t base = 0;
+    for (int k = 0; k < w; ++k) base += wsum[k];
+    if (threadIdx.x == nd::kBlock - 1) *total = base + inc;
+    __syncthreads();
+    return base + inc - v;
+}
+
+// One workgroup per digit: exclusive scan of that digit's row of tile counts
+// (hist[d][0..ntiles)) in place; the row total goes to totals[d].
+__global__ void __launch_bounds__(nd::kBlock)
+k_radix_scan_rows(int32_t *__restrict__ hist, int32_t ntiles, int32_t *__restrict__ totals) {
+    __shared__ int32_t wsum[kWaves];
+    __shared__ int32_t tot;
+    int32_t *row = hist + (int64_t)blockIdx.x * ntiles;
+    int32_t carry = 0;
+    for (int32_t c0 = 0; c0 < ntiles; c0 += nd::kBlock) {
+        const int32_t k = c0 + threadIdx.x;
+        const int32_t v = k < ntiles ? row[k] : 0;
+        const int32_t ex = block_exclusive_scan256(v, wsum, &tot);
+        if (k < ntiles) row[k] = carry + ex;
+        carry += tot;
         __syncthreads();
     }
-    int32_t run = part[t] - s;  // exclusive prefix of this lane's chunk
-    for (int64_t k = lo; k < hi; ++k) {
-        int32_t c = hist[k];
-        hist[k] = run;
-        run += c;
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 __global__ void __launch_bounds__(nd::kBlock)
 k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
                 int32_t n, int shift, int32_t ntiles, const int32_t *__restrict__ hist,
-                int32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out) {
+                const int32_t *__restrict__ totals, int32_t *__restrict__ keys_out,
+                int32_t *__restrict__ vals_out) {
     __shared__ int32_t wcnt[kWaves][256];
     __shared__ int32_t bpre[256];
     __shared__ int32_t gofs[256];
     __shared__ int32_t wsum[kWaves];
+    __shared__ int32_t tot0;
     __shared__ int32_t sk[kTile];
     __shared__ int32_t sv[kTile];
 
@@ -77,7 +93,10 @@ k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__
     const int64_t base = (int64_t)blockIdx.x * kTile;
     const int32_t nvalid = (int32_t)((n - base) < kTile ? (n - base) : kTile);
 
-    gofs[t] = hist[(int64_t)t * ntiles + blockIdx.x];
+    // global start of digit t in this tile = (keys with smaller digits) +
+    // (keys with digit t in earlier tiles)
+    const int32_t dbase = block_exclusive_scan256(totals[t], wsum, &tot0);
+    gofs[t] = dbase + hist[(int64_t)t * ntiles + blockIdx.x];
 #pragma unroll
     for (int k = 0; k < kWaves; ++k) wcnt[k][t] = 0;
     __syncthreads();
@@ -189,14 +208,14 @@ void radix_sort_pairs(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, 
     const int32_t ntiles = (int32_t)radix_scratch_blocks(n);
     b.keys_alt.reserve(sizeof(int32_t) * (size_t)n);
     b.vals_alt.reserve(sizeof(int32_t) * (size_t)n);
-    b.hist.reserve(sizeof(int32_t) * 256 * (size_t)ntiles);
+    b.hist.reserve(sizeof(int32_t) * (256 * (size_t)ntiles + 256));
+    int32_t *hist = b.hist.as<int32_t>(), *totals = hist + 256 * (size_t)ntiles;
     int32_t *ki = keys, *vi = vals, *ko = b.keys_alt.as<int32_t>(), *vo = b.vals_alt.as<int32_t>();
     int passes = 0;
     for (int shift = first_bit; shift < end_bit; shift += 8) {
-        k_radix_hist<<<ntiles, kBlock, 0, s>>>(ki, n, shift, ntiles, b.hist.as<int32_t>());
-        k_radix_scan<<<1, 1024, 0, s>>>(b.hist.as<int32_t>(), (int64_t)256 * ntiles);
-        k_radix_scatter<<<ntiles, kBlock, 0, s>>>(ki, vi, n, shift, ntiles,
-                                                  b.hist.as<int32_t>(), ko, vo);
+        k_radix_hist<<<ntiles, kBlock, 0, s>>>(ki, n, shift, ntiles, hist);
+        k_radix_scan_rows<<<256, kBlock, 0, s>>>(hist, ntiles, totals);
+        k_radix_scatter<<<ntiles, kBlock, 0, s>>>(ki, vi, n, shift, ntiles, hist, totals, ko, vo);
         std::swap(ki, ko);
         std::swap(vi, vo);
         ++passes;

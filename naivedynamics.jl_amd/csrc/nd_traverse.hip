@@ -348,7 +348,7 @@ int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec
     ctx->perm.reserve(sizeof(int32_t) * (size_t)n);
     ctx->xyz_sorted.reserve(sizeof(float) * 3 * (size_t)n);
     ctx->nodes.reserve(sizeof(nd_node) * (size_t)(2 * nL - 1));
-    ctx->store.reserve(sizeof(int32_t) * (size_t)nL);
+    ctx->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(nL));
     // mortoncodes! -> codes (and perm = 1:N)
     nd::launch_morton(ctx, d_xyz, n, morton_mode, nullptr, nullptr, ctx->codes_sorted.as<int32_t>(),
                       ctx->perm.as<int32_t>(), ctx->status.as<int32_t>());

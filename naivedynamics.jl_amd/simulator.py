@@ -129,6 +129,14 @@ class Simulation:
         check(load().nd_sim_stats_get(self.handle, ctypes.byref(s)), self.ctx, "nd_sim_stats")
         return {k: getattr(s, k) for k, _ in SimStats._fields_}
 
+    def time_fused_window(self, nsteps):
+        """Advance one rebuild window of nsteps; returns the fused force+integrate
+        kernel's average duration in ms (back-to-back launches between HIP events)."""
+        ms = ctypes.c_double(0)
+        check(load().nd_sim_time_fused_window(self.handle, int(nsteps), ctypes.byref(ms)),
+              self.ctx, "nd_sim_time_fused_window")
+        return ms.value
+
     def kernel_timing(self, enable=-1):
         ms = ctypes.c_double(0)
         n = ctypes.c_int64(0)
