@@ -185,6 +185,11 @@ typedef struct {
     int32_t thermostat_every; /* velocity rescale period (rescale_velocity!); 0 = off */
     int32_t use_graph;    /* 1: replay each rebuild window as a captured hipGraph */
     int32_t max_neighbors;/* initial per-atom list capacity (0 = estimate) */
+    int32_t resort_every; /* full re-sort + tree build period in steps (a multiple of
+                             rebuild_every; 0 = every rebuild).  Rebuilds in between
+                             keep the atom order and tree topology and refit the
+                             boxes to the current positions. */
+    int32_t list_walk;    /* list walk: 0 = child-box stack walk, 1 = rope walk */
     double eps, sigma, rc; /* LJ */
     double k_coul, rc_coul;/* Coulomb (rc_coul = 0 disables) */
     double skin;           /* list radius = max(rc, rc_coul) + skin */

@@ -103,6 +103,11 @@ size_t radix_scratch_blocks(int32_t n);
 // index invariants (only possible for caller-supplied codes with bit 31 set).
 // store must hold lbvh_scratch_ints(nL) int32.
 inline size_t lbvh_scratch_ints(int32_t nL) { return 4 * ((size_t)nL + 16); }
+// Box-only update of a tree built by launch_lbvh on the same store.
+void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride, int32_t nL,
+                       int32_t apl, float r, nd_node *nodes, int32_t *store);
+// Child-box form of the internal nodes (4 float4 per internal node).
+void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *wide);
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
                  nd_node *nodes, int32_t *store, int32_t *status);

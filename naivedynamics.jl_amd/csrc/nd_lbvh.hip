@@ -225,9 +225,76 @@ k_lbvh_refit(nd_node *__restrict__ nodes, const int2 *__restrict__ ranges,
     }
 }
 
+// Leaf boxes only (the tree topology is kept): box of each leaf's atoms
+// +- r/2, written into the leaf node without touching left/skip.
+__global__ void __launch_bounds__(nd::kBlock)
+k_leaf_boxes(const float *__restrict__ xyz, int32_t stride, int32_t nL, int32_t apl, float h,
+             nd_node *__restrict__ nodes) {
+    const int32_t l = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (l >= nL) return;
+    const float *p0 = xyz + (int64_t)stride * (int64_t)l * apl;
+    float mn[3], mx[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { mn[d] = p0[d] - h; mx[d] = p0[d] + h; }
+    for (int32_t a = 0; a < apl; ++a) {
+        const float *p = p0 + (int64_t)stride * a;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = jl_min(mn[d], p[d] - h);
+            mx[d] = jl_max(mx[d], p[d] + h);
+        }
+    }
+    float *q = reinterpret_cast<float *>(nodes + l);
+    *reinterpret_cast<float4 *>(q) = make_float4(mn[0], mn[1], mn[2], mx[0]);
+    *reinterpret_cast<float2 *>(q + 4) = make_float2(mx[1], mx[2]);
+}
+
+// Child-box form of the internal nodes for stack-based traversal: wide[k] for
+// internal node nL+1+k holds both children's boxes and numbers
+// (float4 x3: min_L xyz max_L x | max_L yz min_R xy | min_R z max_R xyz;
+// int4: left, right, 0, 0).  The right child is the left child's rope.
+__global__ void __launch_bounds__(nd::kBlock)
+k_lbvh_wide(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__ wide) {
+    const int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= nL - 1) return;
+    const int32_t L = nodes[nL + k].left;
+    if (L < 1 || L > 2 * nL - 1) return;
+    const float4 *pl = reinterpret_cast<const float4 *>(nodes + (L - 1));
+    const float4 la = pl[0], lb = pl[1];
+    const int32_t R = __float_as_int(lb.w);  // rope of the left child = right sibling
+    if (R < 1 || R > 2 * nL - 1) return;
+    const float4 *pr = reinterpret_cast<const float4 *>(nodes + (R - 1));
+    const float4 ra = pr[0], rb = pr[1];
+    float4 *w = wide + 4 * (int64_t)k;
+    w[0] = make_float4(la.x, la.y, la.z, la.w);
+    w[1] = make_float4(lb.x, lb.y, ra.x, ra.y);
+    w[2] = make_float4(ra.z, ra.w, rb.x, rb.y);
+    w[3] = make_float4(__int_as_float(L), __int_as_float(R), 0.0f, 0.0f);
+}
+
 }  // namespace
 
 namespace nd {
+
+// Box-only update of an existing tree (same leaf order, same topology):
+// leaf boxes from the current positions, chunk boxes, internal boxes.
+void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride, int32_t nL,
+                       int32_t apl, float r, nd_node *nodes, int32_t *store) {
+    const size_t off_r = ((size_t)nL + 3) & ~(size_t)3;
+    const size_t off_c = (off_r + 2 * (size_t)nL + 3) & ~(size_t)3;
+    const int2 *ranges = reinterpret_cast<const int2 *>(store + off_r);
+    float4 *chunks = reinterpret_cast<float4 *>(store + off_c);
+    k_leaf_boxes<<<grid_for(nL), kBlock, 0, s>>>(xyz_sorted, xyz_stride, nL, apl, r / 2.0f, nodes);
+    const int64_t nchunks = (nL + kChunk - 1) / kChunk;
+    k_lbvh_chunks<<<grid_for(nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);
+    k_lbvh_refit<<<grid_for((int64_t)(nL - 1) * kGroup), kBlock, 0, s>>>(nodes, ranges, chunks, nL, 0);
+    ND_HIP(hipGetLastError());
+}
+
+void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *wide) {
+    k_lbvh_wide<<<grid_for(nL - 1), kBlock, 0, s>>>(nodes, nL, wide);
+    ND_HIP(hipGetLastError());
+}
 
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,

@@ -503,6 +503,112 @@ k_nlist(const float4 *__restrict__ pos, const nd_node *__restrict__ nodes, int32
     }
 }
 
+// Child-box walk: the same wave-cooperative walk over the child-box form of
+// the tree (each internal node carries both children's boxes), depth first
+// with a wave-uniform stack in LDS.  One fetch tests both children, so only
+// internal nodes that some lane overlaps are ever fetched: about half the
+// dependent fetches of the rope walk.  The next node's fetch is issued
+// before the leaf tests of the current one.
+constexpr int kStackDepth = 96;
+
+template <int A>
+__global__ void __launch_bounds__(nd::kBlock)
+k_nlist2(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
+         int32_t apl, float h, float rl2, int32_t cap, int32_t *__restrict__ nbr,
+         int32_t *__restrict__ ncount, SimMisc *__restrict__ misc) {
+    __shared__ int32_t stk[nd::kBlock / 64][kStackDepth];
+    int32_t *st = stk[threadIdx.x >> 6];
+    const int32_t la = A > 0 ? A : apl;
+    const int32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    int32_t *lst = nbr + ((int64_t)(i >> 6) * cap) * 64 + (i & 63);
+    const float4 p = active ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float qmn0 = p.x - h, qmn1 = p.y - h, qmn2 = p.z - h;
+    const float qmx0 = p.x + h, qmx1 = p.y + h, qmx2 = p.z + h;
+    int32_t cnt = 0;
+    auto test = [&](const float4 &q, int32_t j) {
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float s = dx * dx + dy * dy + dz * dz;
+        if (j != i && s < rl2) {
+            if (cnt < cap) lst[cnt * 64] = j;
+            ++cnt;
+        }
+    };
+    auto leaf_work = [&](int32_t leaf, bool ov) {
+        const int32_t base = (leaf - 1) * la;
+        if (A > 0) {
+            for (int m0 = 0; m0 < A; m0 += 8) {
+                constexpr int kB = A <= 0 ? 1 : (A < 8 ? A : 8);
+                float4 q[kB];
+#pragma unroll
+                for (int m = 0; m < kB; ++m) q[m] = pos[base + m0 + m];
+                if (ov) {
+#pragma unroll
+                    for (int m = 0; m < kB; ++m) test(q[m], base + m0 + m);
+                }
+            }
+        } else if (ov) {
+            for (int32_t m = 0; m < la; ++m) test(pos[base + m], base + m);
+        }
+    };
+    int32_t sp = 0;
+    // the root nL + 1 is internal (nL >= 2): its record is wide[0]
+    float4 a = wide[0], b = wide[1], c = wide[2], d = wide[3];
+    for (int32_t iters = 0;; ++iters) {
+        const int32_t Lc = __builtin_amdgcn_readfirstlane(__float_as_int(d.x));
+        const int32_t Rc = __builtin_amdgcn_readfirstlane(__float_as_int(d.y));
+        // every internal node is fetched at most once; children are node numbers
+        if (iters >= nL || Lc < 1 || Lc > 2 * nL - 1 || Rc < 1 || Rc > 2 * nL - 1) {
+            if ((threadIdx.x & 63) == 0) atomicOr(&misc->status, nd::kStatusTree);
+            break;
+        }
+        const bool ovL = active & (qmn0 < a.w) & (qmx0 > a.x) & (qmn1 < b.x) & (qmx1 > a.y) &
+                         (qmn2 < b.y) & (qmx2 > a.z);
+        const bool ovR = active & (qmn0 < c.y) & (qmx0 > b.z) & (qmn1 < c.z) & (qmx1 > b.w) &
+                         (qmn2 < c.w) & (qmx2 > c.x);
+        const bool anyL = __ballot(ovL) != 0ull, anyR = __ballot(ovR) != 0ull;
+        const bool leafL = Lc <= nL, leafR = Rc <= nL;
+        int32_t next = 0;
+        if (anyL && !leafL) next = Lc;
+        if (anyR && !leafR) {
+            if (next == 0) {
+                next = Rc;
+            } else {
+                if (sp >= kStackDepth) {
+                    if ((threadIdx.x & 63) == 0) atomicOr(&misc->status, nd::kStatusTree);
+                    break;
+                }
+                st[sp++] = Rc;
+            }
+        }
+        if (next == 0 && sp > 0) next = __builtin_amdgcn_readfirstlane(st[--sp]);
+        if (next != 0) {
+            // issue the next record's fetch before this node's leaf tests
+            const float4 *nw = wide + 4 * (int64_t)(next - nL - 1);
+            a = nw[0]; b = nw[1]; c = nw[2]; d = nw[3];
+        }
+        if (anyL && leafL) leaf_work(Lc, ovL);
+        if (anyR && leafR) leaf_work(Rc, ovR);
+        if (next == 0) break;
+    }
+    if (active) ncount[i] = cnt;
+    int32_t m = cnt;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0) {
+        atomicMax(&misc->maxcnt, m);
+        if (m > cap) atomicOr(&misc->overflow, 1);
+    }
+}
+
+// FP32 copy of FP64 positions for a box refit without a re-sort.
+__global__ void __launch_bounds__(nd::kBlock)
+k_pos32(const double4 *__restrict__ pos, int32_t n, float4 *__restrict__ out) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double4 p = pos[s];
+    out[s] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+}
+
 __global__ void k_misc_reset_counters(SimMisc *misc) {
     if (threadIdx.x == 0) { misc->overflow = 0; misc->status = 0; misc->maxcnt = 0; }
 }
@@ -514,8 +620,8 @@ struct nd_sim {
     nd_sim_params p{};
     int32_t n = 0, nL = 0, apl = 0;
     bool wide = false;  // FP64 storage (MIXED, FP64)
-    nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, codes, perm, nodes, store, nbr, ncount, misc,
-        partials, rows, snap_pos, snap_vel, snap_frc, snap_orig;
+    nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, codes, perm, nodes, store, kids, nbr, ncount,
+        misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig;
     nd::SortBufs sort;
     int cur = 0;   // position buffer (flips every drift and every re-sort)
     int vcur = 0;  // velocity / original-index buffer (flips only on a re-sort)
@@ -523,10 +629,13 @@ struct nd_sim {
     int32_t max_rows = 1 << 16;
     int64_t step = 0, rebuilds = 0, last_build_step = -1;
     bool have_force = false, list_valid = false, have_state = false;
+    // the tree in nodes/store matches the current atom order (a refit may reuse it)
+    bool tree_valid = false;
     float rlist = 0.0f;
     StepParams sp{};
-    // graphs of one rebuild window, per buffer parity
-    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+    // graphs of one rebuild window, per buffer parity (bits 0-1) and kind
+    // (bit 2: window that re-sorts)
+    hipGraphExec_t gexec[8] = {};
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -566,7 +675,62 @@ hipEvent_t next_event(nd_sim *s) {
 
 // ---- launch sequences (asynchronous) ----
 
-void launch_rebuild(nd_sim *s) {
+int32_t resort_period(const nd_sim *s) {
+    return s->p.resort_every > 0 ? s->p.resort_every : s->p.rebuild_every;
+}
+
+// A rebuild at step k re-sorts (new keys, order and topology) when the
+// period says so or when no tree matches the current order.
+bool resort_due(const nd_sim *s, int64_t k) { return !s->tree_valid || k % resort_period(s) == 0; }
+
+void launch_sort_and_build(nd_sim *s);
+
+// Neighbour-list rebuild: a full re-sort + tree build, or a box refit of the
+// current tree, followed by the list walk.
+void launch_rebuild(nd_sim *s, bool full) {
+    hipStream_t st = s->ctx->stream;
+    const int32_t n = s->n;
+    const unsigned g = nd::grid_for(n);
+    SimMisc *misc = s->misc.as<SimMisc>();
+    if (full) {
+        launch_sort_and_build(s);
+        s->tree_valid = true;
+    } else {
+        if (s->wide)
+            k_pos32<<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), n, s->pos32.as<float4>());
+        const float *p32 = s->wide ? reinterpret_cast<const float *>(s->pos32.p)
+                                   : reinterpret_cast<const float *>(s->pos[s->cur].p);
+        nd::launch_lbvh_refit(st, p32, 4, s->nL, s->apl, s->rlist, s->nodes.as<nd_node>(),
+                              s->store.as<int32_t>());
+    }
+    const float4 *p4 = s->wide ? s->pos32.as<float4>() : s->pos[s->cur].as<float4>();
+    const float hl = s->rlist / 2.0f, rl2 = s->rlist * s->rlist;
+    int32_t *nb = s->nbr.as<int32_t>(), *nc = s->ncount.as<int32_t>();
+    if (s->p.list_walk == 1) {
+        const nd_node *nodes = s->nodes.as<nd_node>();
+        switch (s->apl) {
+            case 4: k_nlist<4><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 4, hl, rl2, s->cap, nb, nc, misc); break;
+            case 8: k_nlist<8><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 8, hl, rl2, s->cap, nb, nc, misc); break;
+            case 16: k_nlist<16><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 16, hl, rl2, s->cap, nb, nc, misc); break;
+            case 32: k_nlist<32><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 32, hl, rl2, s->cap, nb, nc, misc); break;
+            default: k_nlist<0><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, s->apl, hl, rl2, s->cap, nb, nc, misc);
+        }
+    } else {
+        float4 *kids = s->kids.as<float4>();
+        nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
+        switch (s->apl) {
+            case 4: k_nlist2<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 4, hl, rl2, s->cap, nb, nc, misc); break;
+            case 8: k_nlist2<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 8, hl, rl2, s->cap, nb, nc, misc); break;
+            case 16: k_nlist2<16><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 16, hl, rl2, s->cap, nb, nc, misc); break;
+            case 32: k_nlist2<32><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 32, hl, rl2, s->cap, nb, nc, misc); break;
+            default: k_nlist2<0><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, s->apl, hl, rl2, s->cap, nb, nc, misc);
+        }
+    }
+    ND_HIP(hipGetLastError());
+}
+
+// New keys, stable sort, state permutation, tree build.
+void launch_sort_and_build(nd_sim *s) {
     hipStream_t st = s->ctx->stream;
     const int32_t n = s->n;
     const unsigned g = nd::grid_for(n);
@@ -599,17 +763,6 @@ void launch_rebuild(nd_sim *s) {
                                : reinterpret_cast<const float *>(s->pos[s->cur].p);
     nd::launch_lbvh(st, p32, 4, codes, s->nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
                     s->store.as<int32_t>(), &misc->status);
-    const float4 *p4 = reinterpret_cast<const float4 *>(p32);
-    const float hl = s->rlist / 2.0f, rl2 = s->rlist * s->rlist;
-    int32_t *nb = s->nbr.as<int32_t>(), *nc = s->ncount.as<int32_t>();
-    nd_node *nodes = s->nodes.as<nd_node>();
-    switch (s->apl) {
-        case 4: k_nlist<4><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 4, hl, rl2, s->cap, nb, nc, misc); break;
-        case 8: k_nlist<8><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 8, hl, rl2, s->cap, nb, nc, misc); break;
-        case 16: k_nlist<16><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 16, hl, rl2, s->cap, nb, nc, misc); break;
-        case 32: k_nlist<32><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, 32, hl, rl2, s->cap, nb, nc, misc); break;
-        default: k_nlist<0><<<g, nd::kBlock, 0, st>>>(p4, nodes, n, s->nL, s->apl, hl, rl2, s->cap, nb, nc, misc);
-    }
     ND_HIP(hipGetLastError());
 }
 
@@ -702,7 +855,7 @@ void read_misc(nd_sim *s, SimMisc &m) {
 void rebuild_checked(nd_sim *s) {
     for (int attempt = 0; attempt < 8; ++attempt) {
         const int cur0 = s->cur;
-        launch_rebuild(s);
+        launch_rebuild(s, true);
         SimMisc m;
         read_misc(s, m);
         if (m.status & nd::kStatusDomain)
@@ -756,42 +909,50 @@ bool graph_eligible(const nd_sim *s) {
 }
 
 // Window at k0 (k0 % W == 0): rebuild(x_k0) + middle steps k0 .. k0+W-1.
-void launch_window(nd_sim *s, int64_t k0) {
-    launch_rebuild(s);
+void launch_window(nd_sim *s, int64_t k0, bool full) {
+    launch_rebuild(s, full);
     s->rebuilds++;
     s->last_build_step = k0;
     for (int32_t o = 0; o < s->p.rebuild_every; ++o) launch_middle_step(s, k0 + o);
 }
 
 void run_window_graph(nd_sim *s, int64_t k0) {
-    const int parity = s->cur | (s->vcur << 1);
-    if (!s->gexec[parity]) {
+    const bool full = resort_due(s, k0);
+    const int key = s->cur | (s->vcur << 1) | (full ? 4 : 0);
+    if (!s->gexec[key]) {
         hipStream_t st = s->ctx->stream;
         hipGraph_t graph;
         const int cur0 = s->cur, vcur0 = s->vcur;
         const int64_t rb0 = s->rebuilds;
+        const bool tv0 = s->tree_valid;
         ND_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
-            launch_window(s, k0);
+            launch_window(s, k0, full);
         } catch (...) {
             hipStreamEndCapture(st, &graph);
             throw;
         }
         ND_HIP(hipStreamEndCapture(st, &graph));
-        ND_HIP(hipGraphInstantiate(&s->gexec[parity], graph, nullptr, nullptr, 0));
+        ND_HIP(hipGraphInstantiate(&s->gexec[key], graph, nullptr, nullptr, 0));
         ND_HIP(hipGraphDestroy(graph));
         s->cur = cur0;  // capture did not execute anything
         s->vcur = vcur0;
         s->rebuilds = rb0;
+        s->tree_valid = tv0;
     }
-    ND_HIP(hipGraphLaunch(s->gexec[parity], s->ctx->stream));
-    // Host-side bookkeeping of launch_window: the permutation swaps the state
-    // buffers once and every middle step swaps them once more (a fused step,
-    // or a final step followed by a kick-drift).
+    ND_HIP(hipGraphLaunch(s->gexec[key], s->ctx->stream));
+    // Host-side bookkeeping of launch_window: a re-sort swaps the state
+    // buffers once and every middle step swaps the positions once more (a
+    // fused step, or a final step followed by a kick-drift).
     s->rebuilds++;
     s->last_build_step = k0;
-    s->cur ^= (1 + s->p.rebuild_every) & 1;
-    s->vcur ^= 1;
+    if (full) {
+        s->cur ^= (1 + s->p.rebuild_every) & 1;
+        s->vcur ^= 1;
+        s->tree_valid = true;
+    } else {
+        s->cur ^= s->p.rebuild_every & 1;
+    }
 }
 
 void run_once(nd_sim *s, int32_t nsteps) {
@@ -822,7 +983,7 @@ void run_once(nd_sim *s, int32_t nsteps) {
                 j += W;
                 continue;
             }
-            launch_rebuild(s);
+            launch_rebuild(s, resort_due(s, j));
             s->rebuilds++;
             s->last_build_step = j;
         }
@@ -831,7 +992,7 @@ void run_once(nd_sim *s, int32_t nsteps) {
     }
     // final integer step: rebuild if due, force + finishing half kick
     if (kend % W == 0 && s->last_build_step != kend) {
-        launch_rebuild(s);
+        launch_rebuild(s, resort_due(s, kend));
         s->rebuilds++;
         s->last_build_step = kend;
     }
@@ -887,6 +1048,9 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     ND_REQUIRE(p->morton_mode == ND_MORTON_MASKED || p->morton_mode == ND_MORTON_INTERLEAVE,
                "nd_sim_create: key mode");
     ND_REQUIRE(p->rebuild_every >= 1, "nd_sim_create: rebuild_every >= 1");
+    ND_REQUIRE(p->resort_every >= 0 && p->resort_every % p->rebuild_every == 0,
+               "nd_sim_create: resort_every must be a multiple of rebuild_every (or 0)");
+    ND_REQUIRE(p->list_walk == 0 || p->list_walk == 1, "nd_sim_create: list_walk must be 0 or 1");
     ND_REQUIRE(p->dt > 0.0f && p->rc >= 0.0f && p->rc_coul >= 0.0f && p->skin >= 0.0f,
                "nd_sim_create: dt > 0, cutoffs and skin >= 0");
     ND_REQUIRE(p->rc > 0.0f || p->rc_coul > 0.0f, "nd_sim_create: no interaction enabled");
@@ -919,6 +1083,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->perm.reserve(sizeof(int32_t) * n);
         s->nodes.reserve(sizeof(nd_node) * (size_t)(2 * s->nL - 1));
         s->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(s->nL));
+        s->kids.reserve(4 * sizeof(float4) * (size_t)(s->nL - 1));
         s->ncount.reserve(sizeof(int32_t) * n);
         s->misc.reserve(sizeof(SimMisc));
         s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
@@ -957,7 +1122,7 @@ int nd_sim_destroy(nd_sim *s) {
     drop_graphs(s);
     for (auto e : s->ev_pool) hipEventDestroy(e);
     nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
-                          &s->orig[1], &s->pos32, &s->codes, &s->perm, &s->nodes, &s->store,
+                          &s->orig[1], &s->pos32, &s->codes, &s->perm, &s->nodes, &s->store, &s->kids,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
                           &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
                           &s->sort.vals_alt, &s->sort.hist, &s->sort.flags};
@@ -1008,6 +1173,7 @@ int nd_sim_set_state(nd_sim *s, const double *h_x, const double *h_v, const doub
     s->step = 0;
     s->have_force = false;
     s->list_valid = false;
+    s->tree_valid = false;
     s->have_state = true;
     s->last_build_step = -1;
     drop_graphs(s);
@@ -1101,6 +1267,7 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
         s->last_build_step = lb0;
         s->have_force = hf0;
         s->list_valid = false;
+        s->tree_valid = false;
         (void)lv0;
         s->ev_used = 0;
     }

@@ -45,6 +45,8 @@ class SimSpec:
     vDamp: float = 1.0
     use_graph: bool = True
     record_every: int = 0
+    resort_every: int = 0
+    list_walk: int = 0
 
 
 @dataclass
@@ -73,6 +75,8 @@ class Simulation:
         p.thermostat_every = int(spec.thermostat_every)
         p.use_graph = int(bool(spec.use_graph))
         p.max_neighbors = int(max_neighbors)
+        p.resort_every = int(spec.resort_every)
+        p.list_walk = int(spec.list_walk)
         p.eps, p.sigma, p.rc = spec.eps, spec.sigma, spec.rc
         p.k_coul, p.rc_coul = spec.k_coul, spec.rc_coul
         p.skin, p.dt = spec.skin, spec.stepwidth

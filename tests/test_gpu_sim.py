@@ -69,11 +69,50 @@ def test_trajectory_vs_oracle(pkg, O, precision, xtol):
     sim.close()
 
 
-def test_graph_and_eager_identical(pkg):
+@pytest.mark.parametrize("precision,xtol", [("fp32", 2e-4), ("fp64", 1e-10)])
+@pytest.mark.parametrize("resort_every,list_walk", [(0, 1), (50, 0), (50, 1)])
+def test_refit_windows_and_walks_vs_oracle(pkg, O, precision, xtol, resort_every, list_walk):
+    # resort_every = 50 with rebuild_every = 10: rebuilds at steps 10..40 and
+    # 60..90 refit the boxes of the tree built at 0 and 50 (atom order and
+    # topology kept); the rope walk and the child-box walk must both give
+    # every pair inside the list radius
+    x, v, q, L = _cfg(pkg, 4096, 7)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision, steps=100, resort_every=resort_every,
+                                       list_walk=list_walk))
+    sim.set_state(x, v, q)
+    sim.run(100)
+    xg, vg, fg = sim.get_state()
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 100,
+                               walls=True, lo=0.0, hi=L)
+    assert np.abs(xg - xr).max() < xtol
+    assert _rel(fg, fr) < (5e-4 if precision == "fp32" else 1e-10)
+    e = sim.energies()
+    ktol = {"fp32": 1e-4, "fp64": 1e-10}[precision]
+    assert np.allclose(e[:, 1], E[:, 0], rtol=ktol)
+    assert sim.stats()["rebuilds"] == 11
+    sim.close()
+
+
+def test_walks_give_the_same_list(pkg):
+    x, v, q, L = _cfg(pkg, 16384, 8)
+    counts = []
+    for walk in (0, 1):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", list_walk=walk, atomsperleaf=16))
+        sim.set_state(x, v, q)
+        sim.run(0)
+        st = sim.stats()
+        counts.append((st["list_entries"], st["max_neighbors"]))
+        sim.close()
+    assert counts[0] == counts[1] and counts[0][0] > 0
+
+
+@pytest.mark.parametrize("resort_every", [0, 20])
+def test_graph_and_eager_identical(pkg, resort_every):
     x, v, q, L = _cfg(pkg, 8192, 3)
     out = []
     for g in (True, False):
-        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", use_graph=g, energy_every=5))
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", use_graph=g, energy_every=5,
+                                           resort_every=resort_every))
         sim.set_state(x, v, q)
         for k in (7, 33, 40):  # windows start and end at different offsets
             sim.run(k)
