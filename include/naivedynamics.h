@@ -189,7 +189,8 @@ typedef struct {
                              rebuild_every; 0 = every rebuild).  Rebuilds in between
                              keep the atom order and tree topology and refit the
                              boxes to the current positions. */
-    int32_t list_walk;    /* list walk: 0 = child-box stack walk, 1 = rope walk */
+    int32_t list_walk;    /* list walk: 0 = two-phase child-box walk (node frontier, then
+                             queued leaves), 1 = rope walk, 2 = one-node child-box walk */
     double eps, sigma, rc; /* LJ */
     double k_coul, rc_coul;/* Coulomb (rc_coul = 0 disables) */
     double skin;           /* list radius = max(rc, rc_coul) + skin */

@@ -600,6 +600,186 @@ k_nlist2(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     }
 }
 
+// Order this wave's LDS writes before its later LDS reads (other lanes' data).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Two-phase child-box walk for leaves of A atoms (A = 4, 8, 16, 32).
+// Phase 1 pops up to kFront internal nodes from the wave's LDS stack and
+// fetches their child-box records together (lane l loads float4 l & 3 of
+// node l >> 2), so one memory latency covers up to 16 nodes; every lane
+// tests its box against both children of each node, overlapped internal
+// children are pushed, and an overlapped leaf is queued for each group of
+// 8 lanes (8 consecutive atoms) with an overlapping lane, together with the
+// group's 8 overlap bits.  Phase 2 drains the 8 group queues side by side:
+// per step each group stages the atoms of its next leaf in its own LDS slot
+// (the following leaf's load already in flight) and each lane whose bit is
+// set tests them, collecting hits in a bit mask that it then appends.  A
+// group of 8 Morton-consecutive atoms overlaps about as many leaves as its
+// busiest lane, so the wave does far fewer leaf steps than a walk that
+// tests the whole wave against every leaf any lane overlaps.  A lane gets
+// exactly the neighbour set of its own single-atom walk.
+// The stack never overflows for trees of depth <= 63 (any LBVH over 30-bit
+// keys): the frontier shrinks to one node (plain depth-first order) when
+// fewer than 64 free entries remain.  Queue entries pack (leaf << 8) | bits,
+// so leaves must number below 2^23.
+constexpr int kFront = 16, kStack2 = 192, kGroupQ = 64;
+constexpr int32_t kMaxLeaves3 = 1 << 23;
+
+template <int A>
+__global__ void __launch_bounds__(nd::kBlock)
+k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
+         float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
+         SimMisc *__restrict__ misc) {
+    static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
+    constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
+    constexpr int kSlot = 4 * A + 4;     // floats per group slot (16 B pad: no bank conflicts)
+    constexpr int kBufF = 8 * kSlot > 256 ? 8 * kSlot : 256;  // node records use 64 float4
+    constexpr int kW = nd::kBlock / 64;
+    __shared__ __align__(16) float s_buf[kW][kBufF];
+    __shared__ int32_t s_stk[kW][kStack2];
+    __shared__ int32_t s_q[kW][8 * kGroupQ + 64];  // 8 group queues + dummy slots
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int grp = lane >> 3, gl = lane & 7;
+    float *buf = s_buf[w];
+    float4 *rec = reinterpret_cast<float4 *>(buf);
+    int32_t *stk = s_stk[w], *gq = s_q[w];
+
+    const int32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    int32_t *lst = nbr + ((int64_t)(i >> 6) * cap) * 64 + (i & 63);
+    const float4 p = active ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // lanes without an atom get an empty query box (it overlaps nothing)
+    const float qmn0 = active ? p.x - h : INFINITY, qmn1 = active ? p.y - h : INFINITY,
+                qmn2 = active ? p.z - h : INFINITY;
+    const float qmx0 = active ? p.x + h : -INFINITY, qmx1 = active ? p.y + h : -INFINITY,
+                qmx2 = active ? p.z + h : -INFINITY;
+    int32_t cnt = 0;
+    bool bad = false;
+
+    int32_t glen = 0;  // lanes 0..7: length of queue `lane`
+    int32_t nqu = 0;   // leaves queued since the last drain (bounds every glen)
+    // branch-free: lanes with nothing to queue write a private dummy slot
+    auto enqueue = [&](int32_t leaf, uint64_t mk) {
+        const uint32_t bits = (uint32_t)(mk >> (8 * lane)) & 0xFFu;
+        const bool put = (lane < 8) & (bits != 0u);
+        gq[put ? lane * kGroupQ + glen : 8 * kGroupQ + lane] = (leaf << 8) | (int32_t)bits;
+        glen += put ? 1 : 0;
+        ++nqu;
+    };
+
+    int32_t sp = 1, fetched = 0;
+    stk[0] = nL + 1;  // the root is internal (nL >= 2)
+    wave_lds_sync();
+    while (sp > 0 || nqu > 0) {
+        if (nqu > 0 && (sp == 0 || nqu > kGroupQ - 2 * kFront)) {
+            // ---- phase 2: drain the group queues ----
+            const int32_t len = __shfl(glen, grp, 64);
+            int32_t steps = len;
+            for (int off = 32; off > 0; off >>= 1) steps = max(steps, __shfl_xor(steps, off, 64));
+            steps = __builtin_amdgcn_readfirstlane(steps);
+            float *slot = buf + grp * kSlot;
+            // atom gl + 8 r of the group's leaf k; lanes past the leaf size load atom 0
+            // (named registers, not an array: the loads stay in flight across the step)
+            const float4 *src = pos + (gl < A ? gl : 0);
+            const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float4 v0 = zero, v1 = zero, v2 = zero, v3 = zero;
+            auto load4 = [&](int32_t e) {
+                const float4 *lp = src + ((e >> 8) - 1) * A;
+                v0 = lp[0];
+                if constexpr (NV > 1) v1 = lp[8];
+                if constexpr (NV > 2) { v2 = lp[16]; v3 = lp[24]; }
+            };
+            if (len > 0) load4(gq[grp * kGroupQ]);
+            for (int32_t k = 0; k < steps; ++k) {
+                wave_lds_sync();  // the previous step's reads are done
+                if (k < len && gl < A) {
+                    float4 *dst = reinterpret_cast<float4 *>(slot) + gl;
+                    dst[0] = v0;
+                    if constexpr (NV > 1) dst[8] = v1;
+                    if constexpr (NV > 2) { dst[16] = v2; dst[24] = v3; }
+                }
+                wave_lds_sync();
+                const int32_t e = k < len ? gq[grp * kGroupQ + k] : 0;
+                if (k + 1 < len) load4(gq[grp * kGroupQ + k + 1]);
+                uint32_t hits = 0u;
+                const int32_t base = ((e >> 8) - 1) * A;
+                if ((e >> gl) & 1) {
+#pragma unroll
+                    for (int m = 0; m < A; ++m) {
+                        const float4 q = *reinterpret_cast<const float4 *>(slot + 4 * m);
+                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                        const float s = dx * dx + dy * dy + dz * dz;
+                        hits |= (s < rl2 ? 1u : 0u) << m;
+                    }
+                    const uint32_t self = (uint32_t)(i - base);
+                    if (self < (uint32_t)A) hits &= ~(1u << self);
+                    while (hits) {
+                        const int32_t m = __builtin_ctz(hits);
+                        hits &= hits - 1u;
+                        if (cnt < cap) lst[cnt * 64] = base + m;
+                        ++cnt;
+                    }
+                }
+            }
+            glen = 0;
+            nqu = 0;
+            wave_lds_sync();
+            continue;
+        }
+        // ---- phase 1: expand up to kFront nodes of the stack top ----
+        const int32_t room = kStack2 - 64 - sp;
+        int32_t k = sp < kFront ? sp : kFront;
+        if (k > room) k = room > 1 ? room : 1;
+        fetched += k;
+        // more internal nodes than the tree has, or a stack that could overflow
+        // (the k pops leave room for the at most 2k pushes below)
+        if (fetched > nL || sp + k > kStack2) { bad = true; break; }
+        sp -= k;
+        {
+            const int32_t slot = lane >> 2;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (slot < k) v = wide[4 * (int64_t)(stk[sp + slot] - nL - 1) + (lane & 3)];
+            wave_lds_sync();
+            rec[lane] = v;
+            wave_lds_sync();
+        }
+        for (int32_t s = 0; s < k; ++s) {
+            const float4 a = rec[4 * s], b = rec[4 * s + 1], c = rec[4 * s + 2], d = rec[4 * s + 3];
+            const bool ovL = (qmn0 < a.w) & (qmx0 > a.x) & (qmn1 < b.x) & (qmx1 > a.y) &
+                             (qmn2 < b.y) & (qmx2 > a.z);
+            const bool ovR = (qmn0 < c.y) & (qmx0 > b.z) & (qmn1 < c.z) & (qmx1 > b.w) &
+                             (qmn2 < c.w) & (qmx2 > c.x);
+            const uint64_t mL = __builtin_amdgcn_ballot_w64(ovL), mR = __builtin_amdgcn_ballot_w64(ovR);
+            const int32_t Lc = __builtin_amdgcn_readfirstlane(__float_as_int(d.x));
+            const int32_t Rc = __builtin_amdgcn_readfirstlane(__float_as_int(d.y));
+            const uint32_t nn = (uint32_t)(2 * nL - 1);  // valid node numbers are 1 .. 2 nL - 1
+            if (((uint32_t)(Lc - 1) >= nn) | ((uint32_t)(Rc - 1) >= nn)) { bad = true; break; }
+            if (mL) {
+                if (Lc <= nL) enqueue(Lc, mL);
+                else stk[sp++] = Lc;
+            }
+            if (mR) {
+                if (Rc <= nL) enqueue(Rc, mR);
+                else stk[sp++] = Rc;
+            }
+        }
+        if (bad) break;
+        wave_lds_sync();
+    }
+    if (bad && lane == 0) atomicOr(&misc->status, nd::kStatusTree);
+    if (active) ncount[i] = cnt;
+    int32_t m = cnt;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
+    if (lane == 0 && m > 0) {
+        atomicMax(&misc->maxcnt, m);
+        if (m > cap) atomicOr(&misc->overflow, 1);
+    }
+}
+
 // FP32 copy of FP64 positions for a box refit without a re-sort.
 __global__ void __launch_bounds__(nd::kBlock)
 k_pos32(const double4 *__restrict__ pos, int32_t n, float4 *__restrict__ out) {
@@ -718,7 +898,16 @@ void launch_rebuild(nd_sim *s, bool full) {
     } else {
         float4 *kids = s->kids.as<float4>();
         nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
-        switch (s->apl) {
+        const bool two_phase = s->p.list_walk == 0 && s->nL < kMaxLeaves3;
+        if (two_phase && s->apl == 4)
+            k_nlist3<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+        else if (two_phase && s->apl == 8)
+            k_nlist3<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+        else if (two_phase && s->apl == 16)
+            k_nlist3<16><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+        else if (two_phase && s->apl == 32)
+            k_nlist3<32><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+        else switch (s->apl) {
             case 4: k_nlist2<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 4, hl, rl2, s->cap, nb, nc, misc); break;
             case 8: k_nlist2<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 8, hl, rl2, s->cap, nb, nc, misc); break;
             case 16: k_nlist2<16><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 16, hl, rl2, s->cap, nb, nc, misc); break;
@@ -1050,7 +1239,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     ND_REQUIRE(p->rebuild_every >= 1, "nd_sim_create: rebuild_every >= 1");
     ND_REQUIRE(p->resort_every >= 0 && p->resort_every % p->rebuild_every == 0,
                "nd_sim_create: resort_every must be a multiple of rebuild_every (or 0)");
-    ND_REQUIRE(p->list_walk == 0 || p->list_walk == 1, "nd_sim_create: list_walk must be 0 or 1");
+    ND_REQUIRE(p->list_walk >= 0 && p->list_walk <= 2, "nd_sim_create: list_walk must be 0, 1 or 2");
     ND_REQUIRE(p->dt > 0.0f && p->rc >= 0.0f && p->rc_coul >= 0.0f && p->skin >= 0.0f,
                "nd_sim_create: dt > 0, cutoffs and skin >= 0");
     ND_REQUIRE(p->rc > 0.0f || p->rc_coul > 0.0f, "nd_sim_create: no interaction enabled");

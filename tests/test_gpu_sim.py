@@ -93,17 +93,20 @@ def test_refit_windows_and_walks_vs_oracle(pkg, O, precision, xtol, resort_every
     sim.close()
 
 
-def test_walks_give_the_same_list(pkg):
+@pytest.mark.parametrize("apl", [2, 4, 8, 16, 32])
+def test_walks_give_the_same_list(pkg, apl):
+    # 0: two-phase child-box walk (leaf sizes 4..32; others use walk 2),
+    # 1: rope walk, 2: one-node child-box walk
     x, v, q, L = _cfg(pkg, 16384, 8)
     counts = []
-    for walk in (0, 1):
-        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", list_walk=walk, atomsperleaf=16))
+    for walk in (0, 1, 2):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", list_walk=walk, atomsperleaf=apl))
         sim.set_state(x, v, q)
         sim.run(0)
         st = sim.stats()
         counts.append((st["list_entries"], st["max_neighbors"]))
         sim.close()
-    assert counts[0] == counts[1] and counts[0][0] > 0
+    assert counts[0] == counts[1] == counts[2] and counts[0][0] > 0
 
 
 @pytest.mark.parametrize("resort_every", [0, 20])
