@@ -3,8 +3,10 @@
 The product path has no fallback: if the shared library is missing or no
 device is present, every call raises.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -163,17 +165,33 @@ def check(rc, ctx=None, what="", required=None):
 
 
 class Context:
-    """One HIP stream + grow-only device scratch (nd_ctx)."""
+    """One HIP stream + grow-only device scratch (nd_ctx).
+
+    Objects that hold device state of this context (simulations, device
+    arrays) register with it; closing the context closes them first, and every
+    open context is closed at interpreter exit before the HIP runtime goes
+    away, so no handle outlives the context it points into."""
 
     def __init__(self, device=0):
         self.handle = None
+        self._dependents = weakref.WeakSet()
         h = _P()
         rc = load().nd_ctx_create(device, ctypes.byref(h))
         check(rc, None, "nd_ctx_create")
         self.handle = h
+        _contexts.add(self)
+
+    def register(self, obj):
+        """obj.close() must release obj's device state; called before this context closes."""
+        self._dependents.add(obj)
 
     def close(self):
         if self.handle:
+            for obj in list(self._dependents):
+                try:
+                    obj.close()
+                except Exception:
+                    pass
             load().nd_ctx_destroy(self.handle)
             self.handle = None
 
@@ -206,6 +224,7 @@ class DeviceArray:
         p = _P()
         check(load().nd_malloc(ctx.handle, max(self.nbytes, 1), ctypes.byref(p)), ctx, "nd_malloc")
         self.ptr = p
+        ctx.register(self)
 
     @classmethod
     def from_host(cls, ctx, a):
@@ -229,8 +248,11 @@ class DeviceArray:
 
     def free(self):
         if self.ptr:
-            load().nd_free(self.ctx.handle, self.ptr)
+            if self.ctx.handle:
+                load().nd_free(self.ctx.handle, self.ptr)
             self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -241,6 +263,16 @@ class DeviceArray:
 
 
 _default_ctx = None
+_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_contexts():
+    for c in list(_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
 
 
 def default_context():

@@ -21,9 +21,10 @@
 // the force kernel needs no atomics and is deterministic).
 //
 // Precision forms: FP32 (float state and pair maths), MIXED (pair maths in
-// FP32 on FP64 coordinate differences, per-atom force / position / velocity
-// accumulation in FP64), FP64 (everything in double).  Energies and the virial
+// FP32 on exact fixed-point coordinate differences, per-atom force / position
+// / velocity accumulation in FP64), FP64 (everything in double).  Energies and the virial
 // are reduced per workgroup in FP64 and summed with Neumaier compensation.
+#include <cmath>
 #include <vector>
 
 #include "nd_internal.hpp"
@@ -49,9 +50,20 @@ struct StepParams {
     int32_t n, cap, walls, pad;
     float eps24, eps4, sigma2, rc2, rcc2, rcmax2, kc, elj_shift, ecoul_inv_rc;
     float dtf, lof, hif;
+    float fx_inv;  // fixed-point unit in coordinate units (MIXED)
     double deps24, deps4, dsigma2, drc2, drcc2, drcmax2, dkc, delj_shift, decoul_inv_rc;
     double dt, lo, hi;
+    double fx_c, fx_s;  // fixed-point frame: X = round((x - fx_c) * fx_s) (MIXED)
 };
+
+// MIXED keeps a fixed-point copy of the positions for its pair kernel:
+// int32 (X, Y, Z) with X = round((x - c) s) and the charge's bits in w.
+// A neighbour gather then moves 16 bytes (as in FP32), and X_i - X_j is exact
+// in integers, so the coordinate difference reaches the FP32 pair maths with
+// two roundings (int -> float, times 1/s) on top of the frame's resolution
+// 1/s, instead of the error of FP32 absolute coordinates.  A coordinate
+// outside the frame sets the domain flag.
+__device__ __forceinline__ int32_t to_fix(double x, const StepParams &P, SimMisc *misc);
 
 template <class S> struct V4;
 template <> struct V4<float> { using T = float4; };
@@ -124,19 +136,28 @@ struct PairAcc {
     using P4 = typename V4<PosS>::T;
     CalcS rc2, rcc2, sigma2, eps24, eps4, kcq, eljs, ecs;
     P4 pi;
+    int4 pfx;     // MIXED: fixed-point self position
+    float inv;    // MIXED: fixed-point unit
     AccS fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     int32_t np = 0;
     __device__ __forceinline__ void add(const P4 &pj) {
-        const CalcS dx = (CalcS)(pi.x - pj.x);
-        const CalcS dy = (CalcS)(pi.y - pj.y);
-        const CalcS dz = (CalcS)(pi.z - pj.z);
+        add_d((CalcS)(pi.x - pj.x), (CalcS)(pi.y - pj.y), (CalcS)(pi.z - pj.z), (CalcS)pj.w);
+    }
+    // fixed-point neighbour (MIXED): exact integer difference, then FP32
+    __device__ __forceinline__ void add(const int4 &pj) {
+        const CalcS dx = (CalcS)(float)(int32_t)((uint32_t)pfx.x - (uint32_t)pj.x) * (CalcS)inv;
+        const CalcS dy = (CalcS)(float)(int32_t)((uint32_t)pfx.y - (uint32_t)pj.y) * (CalcS)inv;
+        const CalcS dz = (CalcS)(float)(int32_t)((uint32_t)pfx.z - (uint32_t)pj.z) * (CalcS)inv;
+        add_d(dx, dy, dz, (CalcS)__int_as_float(pj.w));
+    }
+    __device__ __forceinline__ void add_d(const CalcS dx, const CalcS dy, const CalcS dz, const CalcS qj) {
         const CalcS r2 = dx * dx + dy * dy + dz * dz;
         const CalcS ir2 = rcp_(r2);
         const CalcS sr2 = sigma2 * ir2;
         const CalcS sr6 = sr2 * sr2 * sr2;
         const bool in_lj = r2 < rc2, in_c = r2 < rcc2;
         const CalcS flj = in_lj ? eps24 * ir2 * sr6 * (sr6 + sr6 - (CalcS)1) : (CalcS)0;
-        const CalcS qq = kcq * (CalcS)pj.w;
+        const CalcS qq = kcq * qj;
         const CalcS ir = rsq_(r2);
         const CalcS fc = in_c ? qq * ir * ir2 : (CalcS)0;
         const CalcS fs = flj + fc;
@@ -161,9 +182,12 @@ __global__ void __launch_bounds__(nd::kBlock)
 k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
        typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
        const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, StepParams P,
-       double *__restrict__ partials, SimMisc *__restrict__ misc) {
+       double *__restrict__ partials, SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in,
+       int4 *__restrict__ fix_out) {
     using P4 = typename V4<PosS>::T;
     using Q = PP<CalcS>;
+    // MIXED gathers the fixed-point copy (16 B per neighbour)
+    constexpr bool FX = sizeof(PosS) == 8 && sizeof(CalcS) == 4;
     __shared__ double sh[nd::kBlock / 64][4];
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t i = blk * blockDim.x + threadIdx.x;
@@ -173,6 +197,10 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     if (i < n) {
         PairAcc<PosS, CalcS, AccS, ENERGY> acc;
         acc.pi = pos_in[i];
+        if (FX) {
+            acc.pfx = fix_in[i];
+            acc.inv = P.fx_inv;
+        }
         acc.rc2 = Q::rc2(P);
         acc.rcc2 = Q::rcc2(P);
         acc.sigma2 = Q::sigma2(P);
@@ -191,16 +219,27 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
             const int32_t j1 = col[(k + 1) * 64];
             const int32_t j2 = col[(k + 2) * 64];
             const int32_t j3 = col[(k + 3) * 64];
-            const P4 p0 = pos_in[j0];
-            const P4 p1 = pos_in[j1];
-            const P4 p2 = pos_in[j2];
-            const P4 p3 = pos_in[j3];
-            acc.add(p0);
-            acc.add(p1);
-            acc.add(p2);
-            acc.add(p3);
+            if constexpr (FX) {
+                const int4 p0 = fix_in[j0], p1 = fix_in[j1], p2 = fix_in[j2], p3 = fix_in[j3];
+                acc.add(p0);
+                acc.add(p1);
+                acc.add(p2);
+                acc.add(p3);
+            } else {
+                const P4 p0 = pos_in[j0];
+                const P4 p1 = pos_in[j1];
+                const P4 p2 = pos_in[j2];
+                const P4 p3 = pos_in[j3];
+                acc.add(p0);
+                acc.add(p1);
+                acc.add(p2);
+                acc.add(p3);
+            }
         }
-        for (; k < cnt; ++k) acc.add(pos_in[col[k * 64]]);
+        for (; k < cnt; ++k) {
+            if constexpr (FX) acc.add(fix_in[col[k * 64]]);
+            else acc.add(pos_in[col[k * 64]]);
+        }
         fx = acc.fx;
         fy = acc.fy;
         fz = acc.fz;
@@ -245,6 +284,9 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                     if (x.z > hi) { x.z = hi; v.z = -v.z; }
                 }
                 pos_out[i] = x;
+                if constexpr (FX)
+                    fix_out[i] = make_int4(to_fix(x.x, P, misc), to_fix(x.y, P, misc),
+                                           to_fix(x.z, P, misc), __float_as_int((float)x.w));
             }
             vel[i] = v;
         } else if (ENERGY) {
@@ -273,7 +315,8 @@ template <class PosS>
 __global__ void __launch_bounds__(nd::kBlock)
 k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
              typename V4<PosS>::T *__restrict__ vel, const typename V4<PosS>::T *__restrict__ frc,
-             StepParams P, SimMisc *__restrict__ misc, int32_t rescale, double t0, double gamma) {
+             StepParams P, SimMisc *__restrict__ misc, int32_t rescale, double t0, double gamma,
+             int4 *__restrict__ fix_out) {
     using P4 = typename V4<PosS>::T;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P.n) {
@@ -307,9 +350,21 @@ k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>:
             if (x.z > hi) { x.z = hi; v.z = -v.z; }
         }
         pos_out[i] = x;
+        if (fix_out)
+            fix_out[i] = make_int4(to_fix((double)x.x, P, misc), to_fix((double)x.y, P, misc),
+                                   to_fix((double)x.z, P, misc), __float_as_int((float)x.w));
         vel[i] = v;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) misc->step += 1;
+}
+
+__device__ __forceinline__ int32_t to_fix(double x, const StepParams &P, SimMisc *misc) {
+    const double t = rint((x - P.fx_c) * P.fx_s);
+    if (!(t > -2147483520.0 && t < 2147483520.0)) {
+        atomicOr(&misc->status, nd::kStatusDomain);
+        return 0;
+    }
+    return (int32_t)t;
 }
 
 // Neumaier-compensated sum of the per-workgroup partials; appends one energy
@@ -394,13 +449,15 @@ k_sim_keys(const typename V4<PosS>::T *__restrict__ pos, int32_t n, int32_t mode
 }
 
 // Gather the state into the new order; also writes the FP32 copy of the
-// positions that the tree build and traversal read.
+// positions that the tree build and traversal read, and (MIXED) the
+// fixed-point copy that the pair kernel gathers.
 template <class PosS>
 __global__ void __launch_bounds__(nd::kBlock)
 k_permute(const int32_t *__restrict__ perm, int32_t n, const typename V4<PosS>::T *__restrict__ pos_in,
           typename V4<PosS>::T *__restrict__ pos_out, const typename V4<PosS>::T *__restrict__ vel_in,
           typename V4<PosS>::T *__restrict__ vel_out, const int32_t *__restrict__ orig_in,
-          int32_t *__restrict__ orig_out, float4 *__restrict__ pos32) {
+          int32_t *__restrict__ orig_out, float4 *__restrict__ pos32, int4 *__restrict__ fix_out,
+          StepParams P, SimMisc *__restrict__ misc) {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int32_t o = perm[s];
@@ -409,6 +466,9 @@ k_permute(const int32_t *__restrict__ perm, int32_t n, const typename V4<PosS>::
     vel_out[s] = vel_in[o];
     orig_out[s] = orig_in[o];
     if (pos32) pos32[s] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+    if (fix_out)
+        fix_out[s] = make_int4(to_fix((double)p.x, P, misc), to_fix((double)p.y, P, misc),
+                               to_fix((double)p.z, P, misc), __float_as_int((float)p.w));
 }
 
 // One wave per 64 consecutive atoms (Morton-adjacent, so spatially compact):
@@ -800,8 +860,8 @@ struct nd_sim {
     nd_sim_params p{};
     int32_t n = 0, nL = 0, apl = 0;
     bool wide = false;  // FP64 storage (MIXED, FP64)
-    nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, codes, perm, nodes, store, kids, nbr, ncount,
-        misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig;
+    nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, fix[2], codes, perm, nodes, store, kids, nbr,
+        ncount, misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig;
     nd::SortBufs sort;
     int cur = 0;   // position buffer (flips every drift and every re-sort)
     int vcur = 0;  // velocity / original-index buffer (flips only on a re-sort)
@@ -830,6 +890,9 @@ struct nd_sim {
 namespace {
 
 size_t pos_bytes(const nd_sim *s) { return (s->wide ? sizeof(double4) : sizeof(float4)) * (size_t)s->n; }
+
+// fixed-point position buffer b (MIXED only)
+int4 *fix_ptr(nd_sim *s, int b) { return s->p.precision == ND_PREC_MIXED ? s->fix[b].as<int4>() : nullptr; }
 
 void drop_graphs(nd_sim *s) {
     for (auto &g : s->gexec)
@@ -940,12 +1003,12 @@ void launch_sort_and_build(nd_sim *s) {
         k_permute<double><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
                                                     s->vel[s->vcur].as<double4>(), s->vel[va].as<double4>(),
                                                     s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),
-                                                    s->pos32.as<float4>());
+                                                    s->pos32.as<float4>(), fix_ptr(s, a), s->sp, misc);
     else
         k_permute<float><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<float4>(), s->pos[a].as<float4>(),
                                                    s->vel[s->vcur].as<float4>(), s->vel[va].as<float4>(),
                                                    s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),
-                                                   nullptr);
+                                                   nullptr, nullptr, s->sp, misc);
     s->cur = a;
     s->vcur = va;
     const float *p32 = s->wide ? reinterpret_cast<const float *>(s->pos32.p)
@@ -964,7 +1027,7 @@ void launch_step_t(nd_sim *s) {
     k_step<PosS, CalcS, AccS, MODE, E><<<g, nd::kBlock, 0, st>>>(
         s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
-        s->misc.as<SimMisc>());
+        s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a));
 }
 
 template <int MODE, bool E>
@@ -1007,11 +1070,13 @@ void launch_kick_drift(nd_sim *s, bool rescale) {
     if (s->wide)
         k_kick_drift<double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
                                                        s->vel[s->vcur].as<double4>(), s->frc.as<double4>(), s->sp,
-                                                       s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp);
+                                                       s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp,
+                                                       fix_ptr(s, a));
     else
         k_kick_drift<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), s->pos[a].as<float4>(),
                                                       s->vel[s->vcur].as<float4>(), s->frc.as<float4>(), s->sp,
-                                                      s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp);
+                                                      s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp,
+                                                      nullptr);
     ND_HIP(hipGetLastError());
     s->cur ^= 1;
 }
@@ -1220,6 +1285,16 @@ void set_step_params(nd_sim *s) {
     q.elj_shift = 0.0f; q.ecoul_inv_rc = 0.0f;
     q.dt = p.dt; q.lo = p.lo; q.hi = p.hi;
     q.dtf = p.dt; q.lof = p.lo; q.hif = p.hi;
+    // fixed-point frame of the MIXED pair kernel: centred on the box; with
+    // walls it spans the box, without them the box and 1.5 box lengths on
+    // each side.  The scale is a power of two, so the unit 1/s is exact in
+    // FP32 (an inexact unit would scale every difference by the same
+    // relative error and bias the pressure coherently).
+    const double span = p.hi > p.lo ? p.hi - p.lo : 1.0;
+    const double half = p.walls ? 0.5 * span * 1.001 + 1e-6 : 2.0 * span;
+    q.fx_c = 0.5 * (p.lo + p.hi);
+    q.fx_s = std::ldexp(1.0, (int)std::floor(std::log2(0.999 * 2147483648.0 / half)));
+    q.fx_inv = (float)(1.0 / q.fx_s);
 }
 
 }  // namespace
@@ -1245,6 +1320,8 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     ND_REQUIRE(p->rc > 0.0f || p->rc_coul > 0.0f, "nd_sim_create: no interaction enabled");
     ND_REQUIRE(p->hi > p->lo || (p->morton_mode == ND_MORTON_MASKED && !p->walls),
                "nd_sim_create: box hi must exceed lo");
+    ND_REQUIRE(p->hi > p->lo || p->precision != ND_PREC_MIXED,
+               "nd_sim_create: the mixed form needs a box (hi > lo) for its fixed-point frame");
     ND_HIP(hipSetDevice(ctx->device));
     nd_sim *s = new nd_sim();
     try {
@@ -1268,6 +1345,8 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         }
         s->frc.reserve(pb);
         if (s->wide) s->pos32.reserve(sizeof(float4) * n);
+        if (p->precision == ND_PREC_MIXED)
+            for (int b = 0; b < 2; ++b) s->fix[b].reserve(sizeof(int4) * n);
         s->codes.reserve(sizeof(int32_t) * n);
         s->perm.reserve(sizeof(int32_t) * n);
         s->nodes.reserve(sizeof(nd_node) * (size_t)(2 * s->nL - 1));
@@ -1311,7 +1390,7 @@ int nd_sim_destroy(nd_sim *s) {
     drop_graphs(s);
     for (auto e : s->ev_pool) hipEventDestroy(e);
     nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
-                          &s->orig[1], &s->pos32, &s->codes, &s->perm, &s->nodes, &s->store, &s->kids,
+                          &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
                           &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
                           &s->sort.vals_alt, &s->sort.hist, &s->sort.flags};

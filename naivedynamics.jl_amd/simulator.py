@@ -89,10 +89,12 @@ class Simulation:
         check(load().nd_sim_create(self.ctx.handle, ctypes.byref(p), ctypes.byref(h)), self.ctx,
               "nd_sim_create")
         self.handle = h
+        self.ctx.register(self)
 
     def close(self):
         if self.handle:
-            load().nd_sim_destroy(self.handle)
+            if self.ctx.handle:  # the context closes its simulations before itself
+                load().nd_sim_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -70,5 +70,49 @@ def plain_scale(seed=2, n=262144):
                 rc_coul=2.5, dt=0.001, steps=1000, rebuild_every=10, skin=0.3)
 
 
+def blob_positions(n, L, seed, nblobs=64, std=2.0, spacing=0.95, jitter=0.05):
+    """Clustered positions for the precision workload: nblobs centres uniform in
+    [0, L)^3; the atoms take n sites of a simple-cubic lattice of the given
+    spacing (jittered by +-jitter), chosen without replacement by Gumbel-top-k
+    on the log density of the Gaussian mixture (one isotropic Gaussian of std
+    `std` per centre).  Each blob therefore fills outwards from its centre at
+    lattice density 1/spacing^3 with a soft edge, leaving voids between blobs,
+    and no pair starts closer than spacing - 2 jitter."""
+    rng = np.random.default_rng(seed)
+    centres = rng.random((nblobs, 3)) * L
+    side = int(np.floor(L / spacing))
+    if side ** 3 < n:
+        raise ValueError("blob_positions: lattice too coarse for n sites")
+    a = L / side
+    g = (np.arange(side) + 0.5) * a
+    logw = np.empty(side ** 3)
+    inv = -0.5 / (std * std)
+    chunk = 1 << 18
+    for s0 in range(0, side ** 3, chunk):
+        idx = np.arange(s0, min(s0 + chunk, side ** 3))
+        iz, rem = np.divmod(idx, side * side)
+        iy, ix = np.divmod(rem, side)
+        p = np.stack([g[ix], g[iy], g[iz]], axis=1)
+        d2 = ((p[:, None, :] - centres[None, :, :]) ** 2).sum(axis=2) * inv
+        m = d2.max(axis=1)
+        logw[s0:s0 + len(idx)] = m + np.log(np.exp(d2 - m[:, None]).sum(axis=1))
+    key = logw - np.log(-np.log(rng.random(side ** 3)))   # Gumbel-top-k
+    sites = np.sort(np.argpartition(-key, n - 1)[:n])
+    iz, rem = np.divmod(sites, side * side)
+    iy, ix = np.divmod(rem, side)
+    x = np.stack([g[ix], g[iy], g[iz]], axis=1) + rng.uniform(-jitter, jitter, size=(n, 3))
+    return rng.permutation(x)
+
+
+def precision_blobs(seed=3, n=1048576):
+    """1,048,576 atoms in 64 blobs (box side for mean density 0.8, L = 109.4),
+    LJ rc = 2.5 plus cutoff Coulomb (q = +-1, rc = 2.5), dt = 0.001, 500 steps,
+    list rebuilt every 10 steps with skin 0.3."""
+    L = box_side(n)
+    return dict(workload="ljc-blobs-1048576", n=n, L=L, x=blob_positions(n, L, seed),
+                v=gaussian_velocities(n, 1.0, seed), q=balanced_charges(n, seed), rc=2.5,
+                rc_coul=2.5, dt=0.001, steps=500, rebuild_every=10, skin=0.3)
+
+
 __all__ = ["box_side", "uniform_positions", "lattice_positions", "gaussian_velocities",
-           "balanced_charges", "plain_small", "plain_scale"]
+           "balanced_charges", "plain_small", "plain_scale", "blob_positions", "precision_blobs"]
