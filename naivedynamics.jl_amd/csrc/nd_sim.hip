@@ -156,17 +156,24 @@ struct PairAcc {
         const CalcS sr2 = sigma2 * ir2;
         const CalcS sr6 = sr2 * sr2 * sr2;
         const bool in_lj = r2 < rc2, in_c = r2 < rcc2;
-        const CalcS flj = in_lj ? eps24 * ir2 * sr6 * (sr6 + sr6 - (CalcS)1) : (CalcS)0;
+        // both terms are computed for every neighbour and then masked with a
+        // select (a conditional expression over unevaluated operands would
+        // become a branch per pair)
+        const CalcS flj_all = eps24 * ir2 * sr6 * (sr6 + sr6 - (CalcS)1);
+        const CalcS flj = in_lj ? flj_all : (CalcS)0;
         const CalcS qq = kcq * qj;
         const CalcS ir = rsq_(r2);
-        const CalcS fc = in_c ? qq * ir * ir2 : (CalcS)0;
+        const CalcS fc_all = qq * ir * ir2;
+        const CalcS fc = in_c ? fc_all : (CalcS)0;
         const CalcS fs = flj + fc;
         fx += (AccS)(fs * dx);
         fy += (AccS)(fs * dy);
         fz += (AccS)(fs * dz);
         if (ENERGY) {
-            const CalcS elj = in_lj ? eps4 * sr6 * (sr6 - (CalcS)1) - eljs : (CalcS)0;
-            const CalcS ec = in_c ? qq * (ir - ecs) : (CalcS)0;
+            const CalcS elj_all = eps4 * sr6 * (sr6 - (CalcS)1) - eljs;
+            const CalcS ec_all = qq * (ir - ecs);
+            const CalcS elj = in_lj ? elj_all : (CalcS)0;
+            const CalcS ec = in_c ? ec_all : (CalcS)0;
             e += (AccS)(elj + ec);
             w += (AccS)(fs * r2);
             np += (in_lj | in_c) ? 1 : 0;
