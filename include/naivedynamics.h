@@ -234,6 +234,17 @@ int nd_sim_kernel_timing(nd_sim *sim, int32_t enable, double *avg_force_ms,
  * the event time divided by nsteps-1 (the fused kernel's average duration). */
 int nd_sim_time_fused_window(nd_sim *sim, int32_t nsteps, double *avg_fused_ms);
 
+/* ---- batched replicas (no reference counterpart: the reference simulates
+ * one system per simulate! call, Simulator.jl:154-256) ----
+ * Declares the atoms of the simulation to be nrep independent replicas packed
+ * in space: h_rep[i] is the replica of atom i (original order; -1 = an inert
+ * padding atom without walls) and h_boxes[6 r .. 6 r + 5] = (lo_x, lo_y, lo_z,
+ * hi_x, hi_y, hi_z) is replica r's wall box.  Replica boxes must be separated
+ * by more than the list radius so that no pair crosses replicas; with walls
+ * on, each atom reflects off its own replica's box instead of [lo, hi]^3.
+ * Call after nd_sim_create and before nd_sim_run; nrep = 0 clears it. */
+int nd_sim_set_replicas(nd_sim *sim, int32_t nrep, const int32_t *h_rep, const double *h_boxes);
+
 #ifdef __cplusplus
 }
 #endif

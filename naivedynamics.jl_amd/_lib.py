@@ -119,6 +119,7 @@ SIGNATURES = {
     "nd_sim_stats_get": [_P, ctypes.POINTER(SimStats)],
     "nd_sim_kernel_timing": [_P, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)],
     "nd_sim_time_fused_window": [_P, _i32, ctypes.POINTER(ctypes.c_double)],
+    "nd_sim_set_replicas": [_P, _i32, _P, _P],
 }
 _RESTYPE = {"nd_last_error": ctypes.c_char_p}
 

@@ -65,6 +65,40 @@ struct StepParams {
 // outside the frame sets the domain flag.
 __device__ __forceinline__ int32_t to_fix(double x, const StepParams &P, SimMisc *misc);
 
+// Wall boxes of packed replicas: atom slot i belongs to replica
+// rep[orig[i]] (-1: no walls) with box boxes[6 r .. 6 r + 5]; rep == nullptr
+// means one system with the box [lo, hi]^3.
+struct Walls {
+    const int32_t *orig;
+    const int32_t *rep;
+    const double *boxes;
+};
+
+// boundary_reflect! (Simulator.jl:81-111): a coordinate past a wall is put
+// on the wall and its velocity component reversed.
+template <class P4, class PosS>
+__device__ __forceinline__ void reflect(P4 &x, P4 &v, const StepParams &P, const Walls &W, int32_t i) {
+    PosS lo[3], hi[3];
+    if (W.rep) {
+        const int32_t r = W.rep[W.orig[i]];
+        if (r < 0) return;
+        const double *b = W.boxes + 6 * (int64_t)r;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { lo[d] = (PosS)b[d]; hi[d] = (PosS)b[3 + d]; }
+    } else {
+        const PosS l = sizeof(PosS) == 4 ? (PosS)P.lof : (PosS)P.lo;
+        const PosS h = sizeof(PosS) == 4 ? (PosS)P.hif : (PosS)P.hi;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { lo[d] = l; hi[d] = h; }
+    }
+    if (x.x < lo[0]) { x.x = lo[0]; v.x = -v.x; }
+    if (x.x > hi[0]) { x.x = hi[0]; v.x = -v.x; }
+    if (x.y < lo[1]) { x.y = lo[1]; v.y = -v.y; }
+    if (x.y > hi[1]) { x.y = hi[1]; v.y = -v.y; }
+    if (x.z < lo[2]) { x.z = lo[2]; v.z = -v.z; }
+    if (x.z > hi[2]) { x.z = hi[2]; v.z = -v.z; }
+}
+
 template <class S> struct V4;
 template <> struct V4<float> { using T = float4; };
 template <> struct V4<double> { using T = double4; };
@@ -190,7 +224,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
        typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
        const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, StepParams P,
        double *__restrict__ partials, SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in,
-       int4 *__restrict__ fix_out) {
+       int4 *__restrict__ fix_out, Walls W) {
     using P4 = typename V4<PosS>::T;
     using Q = PP<CalcS>;
     // MIXED gathers the fixed-point copy (16 B per neighbour)
@@ -279,17 +313,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 x.x += dt * v.x;
                 x.y += dt * v.y;
                 x.z += dt * v.z;
-                if (P.walls) {
-                    const PosS lo = sizeof(PosS) == 4 ? (PosS)P.lof : (PosS)P.lo;
-                    const PosS hi = sizeof(PosS) == 4 ? (PosS)P.hif : (PosS)P.hi;
-                    // boundary_reflect! (Simulator.jl:81-111)
-                    if (x.x < lo) { x.x = lo; v.x = -v.x; }
-                    if (x.x > hi) { x.x = hi; v.x = -v.x; }
-                    if (x.y < lo) { x.y = lo; v.y = -v.y; }
-                    if (x.y > hi) { x.y = hi; v.y = -v.y; }
-                    if (x.z < lo) { x.z = lo; v.z = -v.z; }
-                    if (x.z > hi) { x.z = hi; v.z = -v.z; }
-                }
+                if (P.walls) reflect<P4, PosS>(x, v, P, W, i);
                 pos_out[i] = x;
                 if constexpr (FX)
                     fix_out[i] = make_int4(to_fix(x.x, P, misc), to_fix(x.y, P, misc),
@@ -323,7 +347,7 @@ __global__ void __launch_bounds__(nd::kBlock)
 k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
              typename V4<PosS>::T *__restrict__ vel, const typename V4<PosS>::T *__restrict__ frc,
              StepParams P, SimMisc *__restrict__ misc, int32_t rescale, double t0, double gamma,
-             int4 *__restrict__ fix_out) {
+             int4 *__restrict__ fix_out, Walls W) {
     using P4 = typename V4<PosS>::T;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P.n) {
@@ -346,16 +370,7 @@ k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>:
         x.x += dt * v.x;
         x.y += dt * v.y;
         x.z += dt * v.z;
-        if (P.walls) {
-            const PosS lo = sizeof(PosS) == 4 ? (PosS)P.lof : (PosS)P.lo;
-            const PosS hi = sizeof(PosS) == 4 ? (PosS)P.hif : (PosS)P.hi;
-            if (x.x < lo) { x.x = lo; v.x = -v.x; }
-            if (x.x > hi) { x.x = hi; v.x = -v.x; }
-            if (x.y < lo) { x.y = lo; v.y = -v.y; }
-            if (x.y > hi) { x.y = hi; v.y = -v.y; }
-            if (x.z < lo) { x.z = lo; v.z = -v.z; }
-            if (x.z > hi) { x.z = hi; v.z = -v.z; }
-        }
+        if (P.walls) reflect<P4, PosS>(x, v, P, W, i);
         pos_out[i] = x;
         if (fix_out)
             fix_out[i] = make_int4(to_fix((double)x.x, P, misc), to_fix((double)x.y, P, misc),
@@ -868,7 +883,8 @@ struct nd_sim {
     int32_t n = 0, nL = 0, apl = 0;
     bool wide = false;  // FP64 storage (MIXED, FP64)
     nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, fix[2], codes, perm, nodes, store, kids, nbr,
-        ncount, misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig;
+        ncount, misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig, rep, boxes;
+    int32_t nrep = 0;  // packed replicas (0: one system)
     nd::SortBufs sort;
     int cur = 0;   // position buffer (flips every drift and every re-sort)
     int vcur = 0;  // velocity / original-index buffer (flips only on a re-sort)
@@ -897,6 +913,11 @@ struct nd_sim {
 namespace {
 
 size_t pos_bytes(const nd_sim *s) { return (s->wide ? sizeof(double4) : sizeof(float4)) * (size_t)s->n; }
+
+Walls walls_of(nd_sim *s) {
+    return Walls{s->orig[s->vcur].as<int32_t>(), s->nrep > 0 ? s->rep.as<int32_t>() : nullptr,
+                 s->nrep > 0 ? s->boxes.as<double>() : nullptr};
+}
 
 // fixed-point position buffer b (MIXED only)
 int4 *fix_ptr(nd_sim *s, int b) { return s->p.precision == ND_PREC_MIXED ? s->fix[b].as<int4>() : nullptr; }
@@ -1034,7 +1055,7 @@ void launch_step_t(nd_sim *s) {
     k_step<PosS, CalcS, AccS, MODE, E><<<g, nd::kBlock, 0, st>>>(
         s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
-        s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a));
+        s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s));
 }
 
 template <int MODE, bool E>
@@ -1078,12 +1099,12 @@ void launch_kick_drift(nd_sim *s, bool rescale) {
         k_kick_drift<double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
                                                        s->vel[s->vcur].as<double4>(), s->frc.as<double4>(), s->sp,
                                                        s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp,
-                                                       fix_ptr(s, a));
+                                                       fix_ptr(s, a), walls_of(s));
     else
         k_kick_drift<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), s->pos[a].as<float4>(),
                                                       s->vel[s->vcur].as<float4>(), s->frc.as<float4>(), s->sp,
                                                       s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp,
-                                                      nullptr);
+                                                      nullptr, walls_of(s));
     ND_HIP(hipGetLastError());
     s->cur ^= 1;
 }
@@ -1397,7 +1418,7 @@ int nd_sim_destroy(nd_sim *s) {
     drop_graphs(s);
     for (auto e : s->ev_pool) hipEventDestroy(e);
     nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
-                          &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids,
+                          &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids, &s->rep, &s->boxes,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
                           &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
                           &s->sort.vals_alt, &s->sort.hist, &s->sort.flags};
@@ -1620,6 +1641,33 @@ int nd_sim_time_fused_window(nd_sim *s, int32_t nsteps, double *avg_fused_ms) {
     m.step = s->step;
     ND_HIP(hipMemcpyAsync(s->misc.p, &m, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
     ND_HIP(hipStreamSynchronize(ctx->stream));
+    ND_API_END(ctx)
+}
+
+int nd_sim_set_replicas(nd_sim *s, int32_t nrep, const int32_t *h_rep, const double *h_boxes) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s && nrep >= 0, "nd_sim_set_replicas: bad argument");
+    ND_HIP(hipSetDevice(ctx->device));
+    if (nrep == 0) {
+        s->nrep = 0;
+        drop_graphs(s);
+        return ND_OK;
+    }
+    ND_REQUIRE(h_rep && h_boxes, "nd_sim_set_replicas: null argument");
+    for (int32_t i = 0; i < s->n; ++i)
+        ND_REQUIRE(h_rep[i] >= -1 && h_rep[i] < nrep, "nd_sim_set_replicas: replica index out of range");
+    for (int32_t r = 0; r < nrep; ++r)
+        for (int d = 0; d < 3; ++d)
+            ND_REQUIRE(h_boxes[6 * r + 3 + d] > h_boxes[6 * r + d], "nd_sim_set_replicas: empty replica box");
+    s->rep.reserve(sizeof(int32_t) * (size_t)s->n);
+    s->boxes.reserve(sizeof(double) * 6 * (size_t)nrep);
+    ND_HIP(hipMemcpyAsync(s->rep.p, h_rep, sizeof(int32_t) * (size_t)s->n, hipMemcpyHostToDevice, ctx->stream));
+    ND_HIP(hipMemcpyAsync(s->boxes.p, h_boxes, sizeof(double) * 6 * (size_t)nrep, hipMemcpyHostToDevice,
+                          ctx->stream));
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    s->nrep = nrep;
+    drop_graphs(s);  // captured launches carry the wall pointers
     ND_API_END(ctx)
 }
 
