@@ -245,6 +245,48 @@ int nd_sim_time_fused_window(nd_sim *sim, int32_t nsteps, double *avg_fused_ms);
  * Call after nd_sim_create and before nd_sim_run; nrep = 0 clears it. */
 int nd_sim_set_replicas(nd_sim *sim, int32_t nrep, const int32_t *h_rep, const double *h_boxes);
 
+/* ---- differentiable simulation: adjoint velocity Verlet (no reference
+ * counterpart: NaiveDynamics.jl has no gradients; this differentiates the
+ * simulate! step of Simulator.jl:198-225 with textbook forces) ----
+ * Forward: unit masses, no walls, shifted-force LJ (eps, sigma, rc) and
+ * shifted-force Coulomb (k_coul, rc_coul; 0 disables), velocity Verlet with
+ * step dt for nsteps steps in FP32 or FP64 state, pair lists of radius
+ * max(rc, rc_coul, rdf_rmax + rdf_h) + skin rebuilt every rebuild_every steps,
+ * (x, v) checkpoints every checkpoint_every steps (a multiple of
+ * rebuild_every).  Loss: L = sum_b (g_b - t_b)^2 over nbins bins of width
+ * rdf_rmax / nbins, g_b = V / (n(n-1)/2 4 pi r_b^2 dr) sum_{i<j} K(r_ij - r_b),
+ * K(d) = (1 - (d/h)^2)^3 for |d| < h = rdf_h, V = volume. */
+typedef struct {
+    int32_t n;               /* atoms (even, >= 4) */
+    int32_t precision;       /* ND_PREC_FP32 or ND_PREC_FP64 (state and forward pair maths) */
+    int32_t nsteps;
+    int32_t checkpoint_every;
+    int32_t rebuild_every;
+    int32_t nbins;
+    double eps, sigma, rc;   /* shifted-force LJ */
+    double k_coul, rc_coul;  /* shifted-force Coulomb */
+    double skin, dt;
+    double rdf_rmax, rdf_h, volume;
+} nd_adj_params;
+
+typedef struct nd_adj nd_adj;
+
+int nd_adj_create(nd_ctx *ctx, const nd_adj_params *p, nd_adj **out);
+int nd_adj_destroy(nd_adj *adj);
+/* Replace eps and sigma (the other parameters are fixed at creation). */
+int nd_adj_set_potential(nd_adj *adj, double eps, double sigma);
+/* Run the forward model from (x0, v0) (n x 3, original order) with charges q
+ * (n, NULL = 0); writes the final radial distribution (nbins) and the loss
+ * against h_target (nbins; NULL: no loss, *h_loss = 0). */
+int nd_adj_forward(nd_adj *adj, const double *h_x0, const double *h_v0, const double *h_q,
+                   const double *h_target, double *h_rdf, double *h_loss);
+/* Final (x, v) of the last forward run. */
+int nd_adj_final_state(nd_adj *adj, double *h_x, double *h_v);
+/* Gradients of the last forward run's loss: dL/dx0, dL/dv0 (n x 3), dL/dq
+ * (n) and dL/d(eps, sigma) (2); any output may be NULL.  Recomputes each
+ * checkpoint segment; a new forward run is needed before the next call. */
+int nd_adj_backward(nd_adj *adj, double *h_dx0, double *h_dv0, double *h_dq, double *h_dparams);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,15 @@ class SimParams(ctypes.Structure):
                 ("vdamp", ctypes.c_double)]
 
 
+class AdjParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("precision", ctypes.c_int32), ("nsteps", ctypes.c_int32),
+                ("checkpoint_every", ctypes.c_int32), ("rebuild_every", ctypes.c_int32),
+                ("nbins", ctypes.c_int32), ("eps", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("rc", ctypes.c_double), ("k_coul", ctypes.c_double), ("rc_coul", ctypes.c_double),
+                ("skin", ctypes.c_double), ("dt", ctypes.c_double), ("rdf_rmax", ctypes.c_double),
+                ("rdf_h", ctypes.c_double), ("volume", ctypes.c_double)]
+
+
 class SimStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("rebuilds", ctypes.c_int64),
                 ("list_entries", ctypes.c_int64), ("max_neighbors", ctypes.c_int32),
@@ -120,6 +129,12 @@ SIGNATURES = {
     "nd_sim_kernel_timing": [_P, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)],
     "nd_sim_time_fused_window": [_P, _i32, ctypes.POINTER(ctypes.c_double)],
     "nd_sim_set_replicas": [_P, _i32, _P, _P],
+    "nd_adj_create": [_P, _P, ctypes.POINTER(_P)],
+    "nd_adj_destroy": [_P],
+    "nd_adj_set_potential": [_P, ctypes.c_double, ctypes.c_double],
+    "nd_adj_forward": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double)],
+    "nd_adj_final_state": [_P, _P, _P],
+    "nd_adj_backward": [_P, _P, _P, _P, _P],
 }
 _RESTYPE = {"nd_last_error": ctypes.c_char_p}
 

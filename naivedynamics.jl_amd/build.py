@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libnaivehip.so")
 ARCH = os.environ.get("ND_OFFLOAD_ARCH", "gfx950")
 
 EXACT = ["nd_ctx.hip", "nd_morton.hip", "nd_sort.hip", "nd_lbvh.hip", "nd_traverse.hip"]
-FAST = ["nd_force.hip", "nd_sim.hip"]
+FAST = ["nd_force.hip", "nd_sim.hip", "nd_adjoint.hip"]
 
 COMMON = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

@@ -21,6 +21,13 @@ PRED_SCALAR, PRED_SIMD = 0, 1
 MORTON_MASKED, MORTON_INTERLEAVE = 0, 1
 
 
+class Adj64Params(ctypes.Structure):
+    _fields_ = [("eps", ctypes.c_double), ("sigma", ctypes.c_double), ("rc", ctypes.c_double),
+                ("k_coul", ctypes.c_double), ("rc_coul", ctypes.c_double), ("dt", ctypes.c_double),
+                ("rdf_rmax", ctypes.c_double), ("rdf_h", ctypes.c_double), ("volume", ctypes.c_double),
+                ("nbins", ctypes.c_int), ("nsteps", ctypes.c_int)]
+
+
 class O64Params(ctypes.Structure):
     _fields_ = [("eps", ctypes.c_double), ("sigma", ctypes.c_double),
                 ("rc_lj", ctypes.c_double), ("k_coul", ctypes.c_double),
@@ -83,6 +90,7 @@ def lib():
         L.o64_vv_run.argtypes = [P, P, P, P, i32, P, ctypes.c_double, i32, i32,
                                  ctypes.c_double, ctypes.c_double, P, P]
         L.base_run.argtypes = [P, P, P, i32, P, i32, P, P]
+        L.adj64_run.argtypes = [P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -269,3 +277,26 @@ def base_run(x, v, q, params, nsteps):
     if rc != 0:
         raise ValueError("oracle baseline error %d" % rc)
     return x, v, times, pe.value
+
+
+def adj64_run(x0, v0, q, params, target=None, grads=False):
+    """FP64 CPU oracle of the differentiable simulation (oracle/adjoint64.c):
+    returns (loss, rdf) or, with grads=True, (loss, rdf, dx0, dv0, dq, d_eps_sigma)."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    v0 = np.ascontiguousarray(v0, np.float64).reshape(-1, 3)
+    n = len(x0)
+    qq = None if q is None else np.ascontiguousarray(q, np.float64)
+    tg = None if target is None else np.ascontiguousarray(target, np.float64)
+    g = np.zeros(params.nbins)
+    L = ctypes.c_double(0)
+    dx = np.zeros((n, 3)) if grads else None
+    dv = np.zeros((n, 3)) if grads else None
+    dq = np.zeros(n) if grads else None
+    dp = np.zeros(2) if grads else None
+    rc = lib().adj64_run(_p(x0), _p(v0), _p(qq), n, ctypes.byref(params), _p(tg), _p(g),
+                         ctypes.byref(L), _p(dx), _p(dv), _p(dq), _p(dp))
+    if rc != 0:
+        raise MemoryError("adj64_run")
+    if grads:
+        return L.value, g, dx, dv, dq, dp
+    return L.value, g

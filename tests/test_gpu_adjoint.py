@@ -1,0 +1,114 @@
+"""GPU: the adjoint velocity Verlet against the FP64 CPU oracle
+(oracle/adjoint64.c, itself pinned by finite differences) and against
+central finite differences of the GPU forward model."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(n, seed, spacing=1.12, jitter=0.08):
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(n ** (1 / 3) - 1e-9))
+    g = np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    x = g * spacing + rng.uniform(-jitter, jitter, size=(n, 3))
+    v = rng.normal(0, 1.0, size=(n, 3))
+    v -= v.mean(0)
+    q = np.where(rng.permutation(n) % 2 == 0, 1.0, -1.0)
+    return x, v, q, (side * spacing) ** 3
+
+
+def _pair(pkg, O, n, steps, precision="fp64", dt=0.002, nbins=20, h=0.15):
+    x, v, q, V = _system(n, 3)
+    kw = dict(dt=dt, eps=1.0, sigma=1.0, rc=2.5, k_coul=1.0, rc_coul=2.5, skin=0.3,
+              checkpoint_every=20, rebuild_every=10, nbins=nbins, rdf_rmax=2.5, rdf_h=h, volume=V)
+    adj = pkg.Adjoint(n, steps, precision=precision, **kw)
+    op = O.Adj64Params(eps=1.0, sigma=1.0, rc=2.5, k_coul=1.0, rc_coul=2.5, dt=dt, rdf_rmax=2.5,
+                       rdf_h=h, volume=V, nbins=nbins, nsteps=steps)
+    return adj, op, x, v, q
+
+
+def test_adjoint_matches_the_fp64_oracle(pkg, O):
+    adj, op, x, v, q = _pair(pkg, O, 48, 60)
+    ot = O.Adj64Params(*[getattr(op, f) for f, _ in op._fields_])
+    ot.eps, ot.sigma = 1.1, 0.95
+    _, target = O.adj64_run(x, v, q, ot)
+    L_o, g_o, dx_o, dv_o, dq_o, dp_o = O.adj64_run(x, v, q, op, target=target, grads=True)
+    L, g = adj.forward(x, v, q, target)
+    xg, _ = adj.final_state()
+    assert np.allclose(g, g_o, rtol=1e-9, atol=1e-12)
+    assert abs(L - L_o) < 1e-9 * abs(L_o)
+    gr = adj.backward()
+
+    def close(a, b, tol=1e-8):
+        return np.abs(a - b).max() <= tol * max(1.0, np.abs(b).max())
+    assert close(gr["dx0"], dx_o), np.abs(gr["dx0"] - dx_o).max()
+    assert close(gr["dv0"], dv_o)
+    assert close(gr["dq"], dq_o)
+    assert abs(gr["deps"] - dp_o[0]) < 1e-8 * max(1.0, abs(dp_o[0]))
+    assert abs(gr["dsigma"] - dp_o[1]) < 1e-8 * max(1.0, abs(dp_o[1]))
+    adj.close()
+
+
+def test_target_from_perturbed_potential_and_rerun_is_reproducible(pkg, O):
+    adj, op, x, v, q = _pair(pkg, O, 512, 100)
+    adj.set_potential(1.1, 0.95)
+    _, target = adj.forward(x, v, q)
+    adj.set_potential(1.0, 1.0)
+    L1, g1 = adj.forward(x, v, q, target)
+    x1, v1 = adj.final_state()
+    L2, g2 = adj.forward(x, v, q, target)
+    x2, v2 = adj.final_state()
+    # fixed-point force sums: the forward model is bitwise reproducible
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+    assert L1 > 0 and np.allclose(g1, g2, rtol=1e-12)
+    adj.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-4)])
+def test_adjoint_matches_central_finite_differences(pkg, O, precision, tol):
+    n, steps = 1000, 200
+    adj, op, x, v, q = _pair(pkg, O, n, steps, precision=precision, dt=0.001, nbins=25, h=0.2)
+    adj.set_potential(1.1, 0.95)
+    _, target = adj.forward(x, v, q)
+    adj.set_potential(1.0, 1.0)
+    L0, _ = adj.forward(x, v, q, target)
+    gr = adj.backward()
+    h = 1e-3
+
+    def loss(x_=x, q_=q, eps=1.0, sigma=1.0):
+        adj.set_potential(eps, sigma)
+        return adj.forward(x_, v, q_, target)[0]
+
+    checks = [("eps", gr["deps"], (loss(eps=1 + h) - loss(eps=1 - h)) / (2 * h)),
+              ("sigma", gr["dsigma"], (loss(sigma=1 + h) - loss(sigma=1 - h)) / (2 * h))]
+    for i in (3, 500):
+        qp, qm = q.copy(), q.copy()
+        qp[i] += h
+        qm[i] -= h
+        checks.append(("q%d" % i, gr["dq"][i], (loss(q_=qp) - loss(q_=qm)) / (2 * h)))
+    for i, c in ((0, 0), (321, 2)):
+        xp, xm = x.copy(), x.copy()
+        xp[i, c] += h
+        xm[i, c] -= h
+        checks.append(("x%d.%d" % (i, c), gr["dx0"][i, c], (loss(x_=xp) - loss(x_=xm)) / (2 * h)))
+    scale = max(abs(c[2]) for c in checks)
+    for name, ad, fd in checks:
+        assert abs(ad - fd) <= tol * max(abs(fd), 1e-2 * scale), (name, ad, fd)
+    adj.close()
+
+
+def test_fp32_gradient_close_to_fp64(pkg, O):
+    out = {}
+    for prec in ("fp32", "fp64"):
+        adj, op, x, v, q = _pair(pkg, O, 512, 40, precision=prec)
+        adj.set_potential(1.1, 0.95)
+        _, target = adj.forward(x, v, q)
+        adj.set_potential(1.0, 1.0)
+        adj.forward(x, v, q, target)
+        out[prec] = adj.backward()
+        adj.close()
+    a, b = out["fp32"], out["fp64"]
+    assert abs(a["deps"] - b["deps"]) < 1e-3 * abs(b["deps"])
+    assert abs(a["dsigma"] - b["dsigma"]) < 1e-3 * abs(b["dsigma"])
+    assert np.linalg.norm(a["dx0"] - b["dx0"]) < 1e-3 * np.linalg.norm(b["dx0"])

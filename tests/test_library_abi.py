@@ -35,6 +35,7 @@ def test_bindings_cover_every_declared_symbol(pkg):
 def test_struct_layouts(pkg):
     assert pkg.NODE_DTYPE.itemsize == 32 and pkg.PAIR_DTYPE.itemsize == 12
     assert ctypes.sizeof(pkg._lib.SimParams) == 12 * 4 + 11 * 8
+    assert ctypes.sizeof(pkg._lib.AdjParams) == 6 * 4 + 10 * 8
     assert ctypes.sizeof(pkg._lib.BvhSpec) == 20
 
 
