@@ -23,10 +23,12 @@
 // skin) come from the library's LBVH pair-list path and are rebuilt at the
 // same steps (multiples of rebuild_every) in the forward and the recompute.
 //
-// Forces of the forward model are accumulated in 2^-40 fixed point with
-// integer atomics: the sum does not depend on the order of the atomic
-// updates, so the recompute reproduces the forward trajectory bit for bit in
-// both precisions.  Adjoint accumulations use FP64 atomics.
+// Kernels run one lane per atom over the atom's row of the pair list (every
+// j paired with i; rows are rebuilt with the list), so nothing is added with
+// atomics.  Forces are summed in 2^-40 fixed point: the sum does not depend
+// on the order within a row, which the list build leaves unspecified, so the
+// recompute reproduces the forward trajectory bit for bit in both
+// precisions.  The reverse sweep sums in FP64.
 //
 // No reference counterpart: NaiveDynamics.jl has no gradients.  This is the
 // differentiation form of the simulate! hot path (Simulator.jl:154-256).
@@ -108,30 +110,88 @@ __device__ __forceinline__ PairD pair_d(double r2, double qi, double qj, const A
     return o;
 }
 
-__device__ __forceinline__ void add_fix(unsigned long long *p, double v) {
-    atomicAdd(p, (unsigned long long)(long long)llrint(v * kFix));
+// Neighbour rows from the LBVH: one wave per 64 consecutive sorted atoms walks
+// the tree from the root with ropes as a unit (the node is wave-uniform; each
+// lane tests its own box, the wave descends where any lane overlaps), and
+// each lane appends every sorted slot t != s within the list radius to the
+// row of its original atom, as original indices: row i = nbr[i cap ..
+// i cap + cnt[i]).  flags[0] receives the longest row, flags[1] a tree error.
+__global__ void __launch_bounds__(nd::kBlock)
+k_adj_nlist(const float *__restrict__ xs, const int32_t *__restrict__ perm,
+            const nd_node *__restrict__ nodes, int32_t n, int32_t nL, int32_t apl, float h, float rl2,
+            int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ cnt,
+            int32_t *__restrict__ flags) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = s < n;
+    const float px = active ? xs[3 * (int64_t)s] : 0.0f, py = active ? xs[3 * (int64_t)s + 1] : 0.0f,
+                pz = active ? xs[3 * (int64_t)s + 2] : 0.0f;
+    const int32_t row = active ? perm[s] : 0;
+    int32_t *out = nbr + (int64_t)row * cap;
+    int32_t c = 0;
+    int32_t t = nL + 1;  // root
+    int32_t visits = 0;
+    while (t != 0) {
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (++visits > 2 * nL || t < 1 || t > 2 * nL - 1) {
+            if ((threadIdx.x & 63) == 0) atomicOr(&flags[1], 1);
+            break;
+        }
+        const float4 *np4 = reinterpret_cast<const float4 *>(nodes + (t - 1));
+        const float4 a = np4[0], b = np4[1];
+        const bool ov = active & (px - h < a.w) & (px + h > a.x) & (py - h < b.x) & (py + h > a.y) &
+                        (pz - h < b.y) & (pz + h > a.z);
+        const int32_t left = __float_as_int(b.z), skip = __float_as_int(b.w);
+        if (__ballot(ov) == 0ull) {
+            t = skip;
+        } else if (left != 0) {
+            t = left;
+        } else {
+            if (ov) {
+                const int32_t base = (t - 1) * apl;
+                for (int32_t m = 0; m < apl; ++m) {
+                    const int32_t u = base + m;
+                    const float dx = px - xs[3 * (int64_t)u], dy = py - xs[3 * (int64_t)u + 1],
+                                dz = pz - xs[3 * (int64_t)u + 2];
+                    if (u != s && dx * dx + dy * dy + dz * dz < rl2) {
+                        if (c < cap) out[c] = perm[u];
+                        ++c;
+                    }
+                }
+            }
+            t = skip;
+        }
+    }
+    if (active) cnt[row] = c < cap ? c : cap;
+    int32_t mx = c;
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_down(mx, o, 64));
+    if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&flags[0], mx);
 }
 
+// F_i = sum_j s(r_ij) r_ij over row i, summed in 2^-40 fixed point: the sum
+// is the same whatever the order of the row.
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_adj_force(const nd_pair *__restrict__ pairs, int64_t np, const T *__restrict__ x,
-            const double *__restrict__ q, AdjP P, unsigned long long *__restrict__ F) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= np) return;
-    const int32_t i = pairs[k].i, j = pairs[k].j;
-    const T dx = x[3 * (int64_t)i] - x[3 * (int64_t)j];
-    const T dy = x[3 * (int64_t)i + 1] - x[3 * (int64_t)j + 1];
-    const T dz = x[3 * (int64_t)i + 2] - x[3 * (int64_t)j + 2];
-    const T r2 = dx * dx + dy * dy + dz * dz;
-    if (!(r2 < (T)P.rc2 || r2 < (T)P.rcc2)) return;
-    const T s = pair_s<T>(r2, (T)(q[i] * q[j]), P);
-    const double fx = (double)(s * dx), fy = (double)(s * dy), fz = (double)(s * dz);
-    add_fix(&F[3 * (int64_t)i], fx);
-    add_fix(&F[3 * (int64_t)i + 1], fy);
-    add_fix(&F[3 * (int64_t)i + 2], fz);
-    add_fix(&F[3 * (int64_t)j], -fx);
-    add_fix(&F[3 * (int64_t)j + 1], -fy);
-    add_fix(&F[3 * (int64_t)j + 2], -fz);
+k_adj_force(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
+            const T *__restrict__ x, const double *__restrict__ q, AdjP P, unsigned long long *__restrict__ F) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
+    const T xi = x[3 * (int64_t)i], yi = x[3 * (int64_t)i + 1], zi = x[3 * (int64_t)i + 2];
+    const double qi = q[i];
+    long long fx = 0, fy = 0, fz = 0;
+    for (int64_t e = e0; e < e1; ++e) {
+        const int32_t j = nbr[e];
+        const T dx = xi - x[3 * (int64_t)j], dy = yi - x[3 * (int64_t)j + 1], dz = zi - x[3 * (int64_t)j + 2];
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < (T)P.rc2 || r2 < (T)P.rcc2)) continue;
+        const T s = pair_s<T>(r2, (T)(qi * q[j]), P);
+        fx += llrint((double)(s * dx) * kFix);
+        fy += llrint((double)(s * dy) * kFix);
+        fz += llrint((double)(s * dz) * kFix);
+    }
+    F[3 * (int64_t)i] = (unsigned long long)fx;
+    F[3 * (int64_t)i + 1] = (unsigned long long)fy;
+    F[3 * (int64_t)i + 2] = (unsigned long long)fz;
 }
 
 __device__ __forceinline__ double fix_val(unsigned long long v) { return (double)(long long)v * kInvFix; }
@@ -174,52 +234,71 @@ __device__ __forceinline__ double rdf_dK(double d, double h) {
     return fabs(u) < 1.0 ? -6.0 * u * t * t / h : 0.0;
 }
 
-// raw histogram sum_{i<j} K(r_ij - r_b) (normalisation on the host)
+// raw histogram sum_{i<j} K(r_ij - r_b) (normalisation on the host); bins
+// are summed per workgroup in LDS, then added once per workgroup
+constexpr int kMaxBins = 1024;
+
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_rdf_hist(const nd_pair *__restrict__ pairs, int64_t np, const T *__restrict__ x, AdjP P,
-           double *__restrict__ hist) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= np) return;
-    const int32_t i = pairs[k].i, j = pairs[k].j;
-    const double dx = (double)x[3 * (int64_t)i] - (double)x[3 * (int64_t)j];
-    const double dy = (double)x[3 * (int64_t)i + 1] - (double)x[3 * (int64_t)j + 1];
-    const double dz = (double)x[3 * (int64_t)i + 2] - (double)x[3 * (int64_t)j + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (!(r < P.rmax + P.h)) return;
-    const int32_t b0 = max(0, (int32_t)floor((r - P.h) / P.dr - 0.5));
-    const int32_t b1 = min(P.nbins - 1, (int32_t)ceil((r + P.h) / P.dr - 0.5));
-    for (int32_t b = b0; b <= b1; ++b) {
-        const double w = rdf_K(r - (b + 0.5) * P.dr, P.h);
-        if (w != 0.0) unsafeAtomicAdd(&hist[b], w);
+k_rdf_hist(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
+           const T *__restrict__ x, AdjP P, double *__restrict__ hist) {
+    __shared__ double hb[kMaxBins];
+    for (int b = threadIdx.x; b < P.nbins; b += blockDim.x) hb[b] = 0.0;
+    __syncthreads();
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) {
+        const double xi = (double)x[3 * (int64_t)i], yi = (double)x[3 * (int64_t)i + 1],
+                     zi = (double)x[3 * (int64_t)i + 2];
+        const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
+        for (int64_t e = e0; e < e1; ++e) {
+            const int32_t j = nbr[e];
+            if (j < i) continue;  // each unordered pair once
+            const double dx = xi - (double)x[3 * (int64_t)j], dy = yi - (double)x[3 * (int64_t)j + 1],
+                         dz = zi - (double)x[3 * (int64_t)j + 2];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(r < P.rmax + P.h)) continue;
+            const int32_t b0 = max(0, (int32_t)floor((r - P.h) / P.dr - 0.5));
+            const int32_t b1 = min(P.nbins - 1, (int32_t)ceil((r + P.h) / P.dr - 0.5));
+            for (int32_t b = b0; b <= b1; ++b) {
+                const double w = rdf_K(r - (b + 0.5) * P.dr, P.h);
+                if (w != 0.0) atomicAdd(&hb[b], w);
+            }
+        }
     }
+    __syncthreads();
+    for (int b = threadIdx.x; b < P.nbins; b += blockDim.x)
+        if (hb[b] != 0.0) unsafeAtomicAdd(&hist[b], hb[b]);
 }
 
 // lambda_x += dL/dx with L = sum_b (g_b - t_b)^2: wb[b] = 2 (g_b - t_b) norm_b
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_rdf_grad(const nd_pair *__restrict__ pairs, int64_t np, const T *__restrict__ x, AdjP P,
-           const double *__restrict__ wb, double *__restrict__ lam) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= np) return;
-    const int32_t i = pairs[k].i, j = pairs[k].j;
-    const double dx = (double)x[3 * (int64_t)i] - (double)x[3 * (int64_t)j];
-    const double dy = (double)x[3 * (int64_t)i + 1] - (double)x[3 * (int64_t)j + 1];
-    const double dz = (double)x[3 * (int64_t)i + 2] - (double)x[3 * (int64_t)j + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (!(r < P.rmax + P.h)) return;
-    const int32_t b0 = max(0, (int32_t)floor((r - P.h) / P.dr - 0.5));
-    const int32_t b1 = min(P.nbins - 1, (int32_t)ceil((r + P.h) / P.dr - 0.5));
-    double g = 0.0;
-    for (int32_t b = b0; b <= b1; ++b) g += wb[b] * rdf_dK(r - (b + 0.5) * P.dr, P.h);
-    if (g == 0.0) return;
-    const double s = g / r;
-    unsafeAtomicAdd(&lam[3 * (int64_t)i], s * dx);
-    unsafeAtomicAdd(&lam[3 * (int64_t)i + 1], s * dy);
-    unsafeAtomicAdd(&lam[3 * (int64_t)i + 2], s * dz);
-    unsafeAtomicAdd(&lam[3 * (int64_t)j], -s * dx);
-    unsafeAtomicAdd(&lam[3 * (int64_t)j + 1], -s * dy);
-    unsafeAtomicAdd(&lam[3 * (int64_t)j + 2], -s * dz);
+k_rdf_grad(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
+           const T *__restrict__ x, AdjP P, const double *__restrict__ wb, double *__restrict__ lam) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
+    const double xi = (double)x[3 * (int64_t)i], yi = (double)x[3 * (int64_t)i + 1],
+                 zi = (double)x[3 * (int64_t)i + 2];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+        const int32_t j = nbr[e];
+        const double dx = xi - (double)x[3 * (int64_t)j], dy = yi - (double)x[3 * (int64_t)j + 1],
+                     dz = zi - (double)x[3 * (int64_t)j + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (!(r < P.rmax + P.h)) continue;
+        const int32_t b0 = max(0, (int32_t)floor((r - P.h) / P.dr - 0.5));
+        const int32_t b1 = min(P.nbins - 1, (int32_t)ceil((r + P.h) / P.dr - 0.5));
+        double g = 0.0;
+        for (int32_t b = b0; b <= b1; ++b) g += wb[b] * rdf_dK(r - (b + 0.5) * P.dr, P.h);
+        const double s = g / r;
+        gx += s * dx;
+        gy += s * dy;
+        gz += s * dz;
+    }
+    lam[3 * (int64_t)i] += gx;
+    lam[3 * (int64_t)i + 1] += gy;
+    lam[3 * (int64_t)i + 2] += gz;
 }
 
 // lambda_h = lambda_v + dt lambda_x; lambda_v = lambda_h; a = w lambda_h
@@ -233,48 +312,49 @@ k_adj_vec(const double *__restrict__ lx, double *__restrict__ lv, double *__rest
     a[k] = w * h;
 }
 
-// lambda_x += J(x) a; dq, (eps, sigma) += (dF/dtheta)^T a
+// lambda_x += J(x) a; dq, (eps, sigma) += (dF/dtheta)^T a.  Row i gives atom
+// i's own terms; each unordered pair sits in two rows, so its eps / sigma
+// term is taken with weight 1/2 from each.
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_adj_hvp(const nd_pair *__restrict__ pairs, int64_t np, const T *__restrict__ x,
-          const double *__restrict__ q, AdjP P, const double *__restrict__ a,
+k_adj_hvp(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
+          const T *__restrict__ x, const double *__restrict__ q, AdjP P, const double *__restrict__ a,
           double *__restrict__ lam, double *__restrict__ dq, double *__restrict__ dpar) {
     __shared__ double sh[2][nd::kBlock / 64];
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double ge = 0.0, gs = 0.0;
-    if (k < np) {
-        const int32_t i = pairs[k].i, j = pairs[k].j;
-        const double dx = (double)x[3 * (int64_t)i] - (double)x[3 * (int64_t)j];
-        const double dy = (double)x[3 * (int64_t)i + 1] - (double)x[3 * (int64_t)j + 1];
-        const double dz = (double)x[3 * (int64_t)i + 2] - (double)x[3 * (int64_t)j + 2];
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < P.rc2 || r2 < P.rcc2) {
-            const double qi = q[i], qj = q[j];
-            const PairD d = pair_d(r2, qi, qj, P);
-            const double ax = a[3 * (int64_t)i] - a[3 * (int64_t)j];
-            const double ay = a[3 * (int64_t)i + 1] - a[3 * (int64_t)j + 1];
-            const double az = a[3 * (int64_t)i + 2] - a[3 * (int64_t)j + 2];
+    if (i < P.n) {
+        const double xi = (double)x[3 * (int64_t)i], yi = (double)x[3 * (int64_t)i + 1],
+                     zi = (double)x[3 * (int64_t)i + 2];
+        const double ai = a[3 * (int64_t)i], bi = a[3 * (int64_t)i + 1], ci = a[3 * (int64_t)i + 2];
+        const double qi = q[i];
+        double mx = 0.0, my = 0.0, mz = 0.0, gq = 0.0;
+        const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
+        for (int64_t e = e0; e < e1; ++e) {
+            const int32_t j = nbr[e];
+            const double dx = xi - (double)x[3 * (int64_t)j], dy = yi - (double)x[3 * (int64_t)j + 1],
+                         dz = zi - (double)x[3 * (int64_t)j + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 < P.rc2 || r2 < P.rcc2)) continue;
+            const PairD d = pair_d(r2, qi, q[j], P);
+            const double ax = ai - a[3 * (int64_t)j], ay = bi - a[3 * (int64_t)j + 1],
+                         az = ci - a[3 * (int64_t)j + 2];
             const double rda = dx * ax + dy * ay + dz * az;
-            const double mx = d.s * ax + d.ds * rda * dx;
-            const double my = d.s * ay + d.ds * rda * dy;
-            const double mz = d.s * az + d.ds * rda * dz;
-            unsafeAtomicAdd(&lam[3 * (int64_t)i], mx);
-            unsafeAtomicAdd(&lam[3 * (int64_t)i + 1], my);
-            unsafeAtomicAdd(&lam[3 * (int64_t)i + 2], mz);
-            unsafeAtomicAdd(&lam[3 * (int64_t)j], -mx);
-            unsafeAtomicAdd(&lam[3 * (int64_t)j + 1], -my);
-            unsafeAtomicAdd(&lam[3 * (int64_t)j + 2], -mz);
-            if (d.H != 0.0) {
-                unsafeAtomicAdd(&dq[i], P.kc * qj * d.H * rda);
-                unsafeAtomicAdd(&dq[j], P.kc * qi * d.H * rda);
-            }
-            ge = d.s_eps * rda;
-            gs = d.s_sig * rda;
+            mx += d.s * ax + d.ds * rda * dx;
+            my += d.s * ay + d.ds * rda * dy;
+            mz += d.s * az + d.ds * rda * dz;
+            gq += P.kc * q[j] * d.H * rda;
+            ge += 0.5 * d.s_eps * rda;
+            gs += 0.5 * d.s_sig * rda;
         }
+        lam[3 * (int64_t)i] += mx;
+        lam[3 * (int64_t)i + 1] += my;
+        lam[3 * (int64_t)i + 2] += mz;
+        dq[i] += gq;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        ge += __shfl_down(ge, off, 64);
-        gs += __shfl_down(gs, off, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+        ge += __shfl_down(ge, o, 64);
+        gs += __shfl_down(gs, o, 64);
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) { sh[0][w] = ge; sh[1][w] = gs; }
@@ -287,10 +367,6 @@ k_adj_hvp(const nd_pair *__restrict__ pairs, int64_t np, const T *__restrict__ x
     }
 }
 
-__global__ void k_zero_u64(unsigned long long *p, int64_t m) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k < m) p[k] = 0ull;
-}
 __global__ void k_zero_f64(double *p, int64_t m) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k < m) p[k] = 0.0;
@@ -307,8 +383,10 @@ struct nd_adj {
     int32_t apl = 1;
     float rlist = 0.0f;
     // state and scratch
-    nd::DevBuf x, v, F, xf, pairs, q, ckx, ckv, seg, lx, lv, a, dq, dpar, hist, wb;
-    int64_t cap = 0, np = 0;
+    nd::DevBuf x, v, F, xf, xs, codes, perm, nodes, store, flags, q, ckx, ckv, seg, lx, lv, a, dq,
+        dpar, hist, wb, cnt, nbr;
+    nd::SortBufs sort;
+    int32_t cap = 0;  // row capacity
     int32_t nck = 0;
     bool have_forward = false;
     std::vector<double> target, rdf, norm;
@@ -321,39 +399,48 @@ size_t state_bytes(const nd_adj *s) { return s->esz * 3 * (size_t)s->p.n; }
 
 template <class T> T *at(nd::DevBuf &b, size_t off_elems = 0) { return b.as<T>() + off_elems; }
 
-// pair list (radius rlist, interleaved key) at positions xs (device, T)
+// neighbour rows (radius rlist, interleaved key LBVH) at positions xp (device, T)
 template <class T>
-void build_list(nd_adj *s, const T *xs) {
-    hipStream_t st = s->ctx->stream;
-    const int64_t m = 3 * (int64_t)s->p.n;
-    k_adj_to_f32<T><<<nd::grid_for(m), nd::kBlock, 0, st>>>(xs, m, s->xf.as<float>());
-    ND_HIP(hipGetLastError());
-    nd_bvh_spec spec;
-    ND_REQUIRE(nd_bvh_spec_init(&spec, s->rlist, s->p.n, s->apl) == ND_OK, "adjoint: bad list spec");
+void build_list(nd_adj *s, const T *xp) {
+    nd_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const int32_t n = s->p.n, nL = n / s->apl;
+    const int64_t m = 3 * (int64_t)n;
+    int32_t *codes = s->codes.as<int32_t>(), *perm = s->perm.as<int32_t>(), *flags = s->flags.as<int32_t>();
+    k_adj_to_f32<T><<<nd::grid_for(m), nd::kBlock, 0, st>>>(xp, m, s->xf.as<float>());
+    nd::launch_fill_i32(st, flags, 4, 0);
+    nd::launch_morton(ctx, s->xf.as<float>(), n, ND_MORTON_INTERLEAVE, nullptr, nullptr, codes, perm, flags + 2);
+    nd::radix_sort_pairs(st, s->sort, codes, perm, n, 0, 30);  // interleaved keys are < 2^30
+    nd::launch_gather_xyz(st, s->xf.as<float>(), perm, n, s->xs.as<float>());
+    nd::launch_lbvh(st, s->xs.as<float>(), 3, codes, nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
+                    s->store.as<int32_t>(), flags + 1);
+    const float hl = s->rlist / 2.0f, rl2 = s->rlist * s->rlist;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        int64_t count = 0;
-        const int rc = nd_leafbuild_traverse_bvh(s->ctx, s->xf.as<float>(), &spec, ND_PRED_SCALAR,
-                                                 ND_MORTON_INTERLEAVE, s->pairs.as<nd_pair>(), s->cap, &count);
-        if (rc == ND_OK) {
-            s->np = count;
-            return;
-        }
-        if (rc != ND_ERR_CAPACITY) throw nd::Error(rc, std::string("adjoint pair list: ") + nd_last_error(s->ctx));
-        s->cap = count + count / 4 + 1024;
-        s->pairs.reserve(sizeof(nd_pair) * (size_t)s->cap);
+        k_adj_nlist<<<nd::grid_for(n), nd::kBlock, 0, st>>>(s->xs.as<float>(), perm, s->nodes.as<nd_node>(), n, nL,
+                                                            s->apl, hl, rl2, s->cap, s->nbr.as<int32_t>(),
+                                                            s->cnt.as<int32_t>(), flags);
+        ND_HIP(hipGetLastError());
+        int32_t h[4];
+        ND_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, st));
+        ND_HIP(hipStreamSynchronize(st));
+        if (h[2]) throw nd::Error(ND_ERR_DOMAIN, "adjoint: non-finite coordinate");
+        if (h[1]) throw nd::Error(ND_ERR_STATE, "adjoint: malformed neighbour tree");
+        if (h[0] <= s->cap) return;
+        s->cap = h[0] + h[0] / 4 + 8;
+        s->nbr.reserve(sizeof(int32_t) * (size_t)s->cap * (size_t)n);
+        nd::launch_fill_i32(st, flags, 1, 0);
     }
-    throw nd::Error(ND_ERR_CAPACITY, "adjoint: pair list kept overflowing");
+    throw nd::Error(ND_ERR_CAPACITY, "adjoint: neighbour rows kept overflowing");
 }
 
 template <class T>
 void compute_force(nd_adj *s, const T *xs) {
     hipStream_t st = s->ctx->stream;
     const int64_t m = 3 * (int64_t)s->p.n;
-    k_zero_u64<<<nd::grid_for(m), nd::kBlock, 0, st>>>(s->F.as<unsigned long long>(), m);
-    if (s->np > 0)
-        k_adj_force<T><<<nd::grid_for(s->np), nd::kBlock, 0, st>>>(s->pairs.as<nd_pair>(), s->np, xs,
-                                                                    s->q.as<double>(), s->P,
-                                                                    s->F.as<unsigned long long>());
+    (void)m;
+    k_adj_force<T><<<nd::grid_for(s->p.n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+                                                                 xs, s->q.as<double>(), s->P,
+                                                                 s->F.as<unsigned long long>());
     ND_HIP(hipGetLastError());
 }
 
@@ -392,9 +479,8 @@ void forward_impl(nd_adj *s) {
     if (T_ % s->p.rebuild_every != 0) build_list<T>(s, x);
     const int32_t nb = s->p.nbins;
     k_zero_f64<<<nd::grid_for(nb), nd::kBlock, 0, st>>>(s->hist.as<double>(), nb);
-    if (s->np > 0)
-        k_rdf_hist<T><<<nd::grid_for(s->np), nd::kBlock, 0, st>>>(s->pairs.as<nd_pair>(), s->np, x, s->P,
-                                                                   s->hist.as<double>());
+    k_rdf_hist<T><<<nd::grid_for(s->p.n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+                                                                x, s->P, s->hist.as<double>());
     ND_HIP(hipGetLastError());
     std::vector<double> h(nb);
     ND_HIP(hipMemcpyAsync(h.data(), s->hist.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
@@ -422,9 +508,8 @@ void backward_impl(nd_adj *s) {
     std::vector<double> wb(s->p.nbins);
     for (int32_t b = 0; b < s->p.nbins; ++b) wb[b] = 2.0 * (s->rdf[b] - s->target[b]) * s->norm[b];
     ND_HIP(hipMemcpyAsync(s->wb.p, wb.data(), sizeof(double) * wb.size(), hipMemcpyHostToDevice, st));
-    if (s->np > 0)
-        k_rdf_grad<T><<<nd::grid_for(s->np), nd::kBlock, 0, st>>>(s->pairs.as<nd_pair>(), s->np, s->x.as<T>(),
-                                                                   s->P, s->wb.as<double>(), lx);
+    k_rdf_grad<T><<<nd::grid_for(n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+                                                           s->x.as<T>(), s->P, s->wb.as<double>(), lx);
     ND_HIP(hipGetLastError());
     T *seg = s->seg.as<T>();
     for (int32_t c = (T_ - 1) / K; c >= 0; --c) {
@@ -445,10 +530,9 @@ void backward_impl(nd_adj *s) {
             }
             k_adj_vec<<<nd::grid_for(m), nd::kBlock, 0, st>>>(lx, lv, a, m, s->P.dt,
                                                                 k == 0 ? s->P.c : 2.0 * s->P.c);
-            if (s->np > 0)
-                k_adj_hvp<T><<<nd::grid_for(s->np), nd::kBlock, 0, st>>>(s->pairs.as<nd_pair>(), s->np, xk,
-                                                                          s->q.as<double>(), s->P, a, lx,
-                                                                          s->dq.as<double>(), s->dpar.as<double>());
+            k_adj_hvp<T><<<nd::grid_for(n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+                                                                  xk, s->q.as<double>(), s->P, a, lx,
+                                                                  s->dq.as<double>(), s->dpar.as<double>());
             ND_HIP(hipGetLastError());
         }
     }
@@ -470,8 +554,9 @@ int nd_adj_create(nd_ctx *ctx, const nd_adj_params *p, nd_adj **out) {
                "nd_adj_create: checkpoint_every must be a multiple of rebuild_every");
     ND_REQUIRE(p->precision == ND_PREC_FP32 || p->precision == ND_PREC_FP64,
                "nd_adj_create: precision must be FP32 or FP64");
-    ND_REQUIRE(p->nbins >= 1 && p->rdf_rmax > 0.0 && p->rdf_h > 0.0 && p->volume > 0.0,
-               "nd_adj_create: RDF bins, range, kernel width and volume must be positive");
+    ND_REQUIRE(p->nbins >= 1 && p->nbins <= kMaxBins && p->rdf_rmax > 0.0 && p->rdf_h > 0.0 &&
+                   p->volume > 0.0,
+               "nd_adj_create: 1..1024 RDF bins; range, kernel width and volume must be positive");
     ND_REQUIRE(p->dt > 0.0 && p->eps > 0.0 && p->sigma > 0.0 && p->rc > 0.0 && p->skin >= 0.0,
                "nd_adj_create: dt, eps, sigma, rc > 0 and skin >= 0");
     ND_HIP(hipSetDevice(ctx->device));
@@ -504,9 +589,16 @@ int nd_adj_create(nd_ctx *ctx, const nd_adj_params *p, nd_adj **out) {
         s->dq.reserve(8 * n);
         s->dpar.reserve(16);
         s->hist.reserve(8 * (size_t)p->nbins);
+        s->cnt.reserve(4 * n);
+        s->xs.reserve(4 * m);
+        s->codes.reserve(4 * n);
+        s->perm.reserve(4 * n);
+        s->nodes.reserve(sizeof(nd_node) * (size_t)(2 * (p->n / s->apl) - 1));
+        s->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(p->n / s->apl));
+        s->flags.reserve(64);
         s->wb.reserve(8 * (size_t)p->nbins);
-        s->cap = 64 * (int64_t)n + 1024;
-        s->pairs.reserve(sizeof(nd_pair) * (size_t)s->cap);
+        s->cap = 96;
+        s->nbr.reserve(sizeof(int32_t) * (size_t)s->cap * n);
         // RDF normalisation: ideal-gas pair count in each bin's shell
         const double dr = p->rdf_rmax / p->nbins, npair = 0.5 * (double)p->n * (p->n - 1);
         s->norm.resize(p->nbins);
@@ -529,8 +621,10 @@ int nd_adj_destroy(nd_adj *s) {
         (void)hipSetDevice(s->ctx->device);
         (void)hipStreamSynchronize(s->ctx->stream);
     }
-    nd::DevBuf *bufs[] = {&s->x, &s->v, &s->F, &s->xf, &s->pairs, &s->q, &s->ckx, &s->ckv, &s->seg,
-                          &s->lx, &s->lv, &s->a, &s->dq, &s->dpar, &s->hist, &s->wb};
+    nd::DevBuf *bufs[] = {&s->x, &s->v, &s->F, &s->xf, &s->xs, &s->codes, &s->perm, &s->nodes,
+                          &s->store, &s->flags, &s->q, &s->ckx, &s->ckv, &s->seg, &s->lx, &s->lv,
+                          &s->a, &s->dq, &s->dpar, &s->hist, &s->wb, &s->cnt, &s->nbr,
+                          &s->sort.keys_alt, &s->sort.vals_alt, &s->sort.hist, &s->sort.flags};
     for (auto *b : bufs) b->release();
     delete s;
     return ND_OK;
