@@ -9,9 +9,12 @@ synthetic (seeded jittered lattice, Gaussian velocities at kT = 1) and resident
 in HBM before the timed region starts.
 
 Prints one JSON line.  Extra fields: the mixed-precision and FP64 forms on the
-same workload, the reference-key (masked Morton) form, the fused force
-kernel's roofline, and the threaded CPU baseline (the reference's CPU path,
-restated in C under oracle/, timed on a bounded sample of the same workload).
+same workload, the reference-key (masked Morton) form, a variant that re-sorts
+every 50 steps and refits the tree in between, the fused force kernel's
+roofline, the three added forms (precision on 1,048,576 clustered atoms,
+512 batched ragged replicas, the adjoint on 32,768 atoms), and the threaded
+CPU baseline (the reference's CPU path, restated in C under oracle/, timed on
+a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -98,6 +101,131 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def precision_form(pkg, steps=500):
+    """1,048,576 atoms in 64 blobs: FP32, mixed (FP32 pair maths, FP64 state and
+    sums) and plain FP64 from the same state; force error of FP32 and mixed
+    against the FP64 GPU forces, energy drift over `steps` steps, and the
+    trajectory RMSD of mixed (and FP32) against FP64 after them."""
+    c = pkg.synth.precision_blobs()
+    n = c["n"]
+    out = {"workload": c["workload"], "n": n, "steps": steps}
+    res = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        spec = pkg.SimSpec(duration=0, stepwidth=c["dt"], rc=c["rc"], rc_coul=c["rc_coul"], skin=c["skin"],
+                           rebuild_every=c["rebuild_every"], lo=0.0, hi=c["L"], precision=prec,
+                           energy_every=10, walls=True, atomsperleaf=SIM_ATOMS_PER_LEAF)
+        sim = pkg.Simulation(n, spec)
+        sim.set_state(c["x"], c["v"], c["q"])
+        sim.run(0)
+        _, _, f0 = sim.get_state()
+        e0 = sim.energies()
+        ms, _ = timed_run(pkg, sim, steps)
+        x, _, _ = sim.get_state()
+        e = sim.energies()
+        E0 = e0[0, 1] + e0[0, 2]
+        E1 = e[-1, 1] + e[-1, 2]
+        res[prec] = dict(f0=f0, x=x, ms=ms, drift=(E1 - E0) / abs(E0))
+        out[prec + "_steps_per_s"] = round(steps / (ms / 1e3), 2)
+        out[prec + "_energy_drift"] = float("%.3e" % res[prec]["drift"])
+        sim.close()
+        log("precision %s: %.4f ms/step, energy drift %.2e" % (prec, ms / steps, res[prec]["drift"]))
+    F = res["fp64"]["f0"]
+    fn = np.maximum(np.linalg.norm(F, axis=1), 1.0)
+    for prec in ("mixed", "fp32"):
+        d = np.linalg.norm(res[prec]["f0"] - F, axis=1)
+        out[prec + "_rel_force_err"] = float("%.3e" % (np.linalg.norm(res[prec]["f0"] - F) / np.linalg.norm(F)))
+        out[prec + "_max_atom_force_err"] = float("%.3e" % (d / fn).max())
+        out[prec + "_rms_atom_force_err"] = float("%.3e" % np.sqrt(np.mean((d / fn) ** 2)))
+        out[prec + "_rmsd_vs_fp64"] = float("%.3e" % np.sqrt(np.mean(np.sum((res[prec]["x"] - res["fp64"]["x"]) ** 2, axis=1))))
+    out["mixed_over_fp64"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
+    out["note"] = ("force errors against the FP64 GPU forces at the initial state (per-atom errors "
+                   "relative to max(|F_i|, 1)); RMSD against the FP64 trajectory")
+    return out
+
+
+def batched_form(pkg, steps=200):
+    """512 replicas of log-uniform size in [256, 16384] (density 0.8, LJ), packed
+    into one launch sequence per step, leaf size and walk autotuned; against one
+    simulation (one launch sequence) per replica."""
+    B = pkg.batched
+    sizes = B.replica_sizes(512, seed=11)
+    reps = B.make_replicas(sizes, seed=11)
+    spec = pkg.SimSpec(duration=0, stepwidth=0.001, rc=2.5, rc_coul=0.0, skin=0.3, rebuild_every=10,
+                       precision="fp32", energy_every=10, atomsperleaf=SIM_ATOMS_PER_LEAF)
+    best, table = pkg.autotune(reps, spec, windows=2, log=log)
+    b = pkg.BatchedSimulation(reps, best)
+    b.set_state()
+    b.run(20)
+    ms, _ = timed_run(pkg, b.sim, steps)
+    b.close()
+    sec = B.run_one_per_replica(reps, best, steps)
+    atoms = int(sizes.sum())
+    out = {"workload": "lj-replicas-512", "replicas": 512, "atoms": atoms, "steps": steps,
+           "autotune": [{"leaf": a, "walk": w, "ms_per_step": round(t, 4)} for a, w, t in table],
+           "chosen": {"leaf": best.atomsperleaf, "walk": best.list_walk},
+           "packed_steps_per_s": round(steps / (ms / 1e3), 2),
+           "packed_atom_steps_per_s": float("%.4e" % (atoms * steps / (ms / 1e3))),
+           "one_per_replica_atom_steps_per_s": float("%.4e" % (atoms * steps / sec.sum())),
+           "speedup": round(sec.sum() / (ms / 1e3), 2), "by_size": []}
+    for lo, hi in [(256, 512), (512, 1024), (1024, 2048), (2048, 4096), (4096, 8192), (8192, 16385)]:
+        m = (sizes >= lo) & (sizes < hi)
+        if m.any():
+            out["by_size"].append({"size": [lo, hi - 1], "replicas": int(m.sum()),
+                                   "one_per_replica_ms_per_step": round(float(np.median(sec[m])) / steps * 1e3, 4),
+                                   "one_per_replica_atom_steps_per_s": float("%.3e" % ((sizes[m] * steps).sum() / sec[m].sum())),
+                                   "packed_atom_steps_per_s": float("%.3e" % ((sizes[m] * steps).sum() / (ms / 1e3)))})
+    log("batched: packed %.4f ms/step, %.1fx one launch sequence per replica" % (ms / steps, out["speedup"]))
+    return out
+
+
+def adjoint_form(pkg, steps=2000):
+    """32,768 atoms, LJ + Coulomb (shifted force), `steps` steps, checkpoints
+    every 50: loss = squared distance of the final smoothed RDF to the RDF of
+    a run with eps = 1.1, sigma = 0.95; gradients by the adjoint against
+    central finite differences of the same forward model."""
+    n = 32768
+    L = pkg.synth.box_side(n)
+    x = pkg.synth.lattice_positions(n, L, 5)
+    v = pkg.synth.gaussian_velocities(n, 1.0, 5)
+    q = pkg.synth.balanced_charges(n, 5)
+    out = {"workload": "ljc-adjoint-32768", "n": n, "steps": steps, "checkpoint_every": 50}
+    for prec in ("fp32", "fp64"):
+        adj = pkg.Adjoint(n, steps, dt=0.001, precision=prec, checkpoint_every=50, rebuild_every=10,
+                          nbins=50, rdf_rmax=2.5, rdf_h=0.1, volume=L ** 3)
+        adj.set_potential(1.1, 0.95)
+        _, target = adj.forward(x, v, q)
+        adj.set_potential(1.0, 1.0)
+        t0 = time.perf_counter()
+        loss, _ = adj.forward(x, v, q, target)
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g = adj.backward()
+        tb = time.perf_counter() - t0
+        r = {"loss": float("%.6e" % loss), "forward_s": round(tf, 3), "backward_s": round(tb, 3),
+             "dL_deps": float("%.6e" % g["deps"]), "dL_dsigma": float("%.6e" % g["dsigma"]),
+             "dL_dq77": float("%.6e" % g["dq"][77]), "dL_dx0_norm": float("%.6e" % np.linalg.norm(g["dx0"]))}
+
+        def lossf(eps=1.0, sigma=1.0, qv=q):
+            adj.set_potential(eps, sigma)
+            return adj.forward(x, v, qv, target)[0]
+        fd = {}
+        for h in ((1e-3, 1e-7) if prec == "fp64" else (1e-3,)):
+            fe = (lossf(eps=1 + h) - lossf(eps=1 - h)) / (2 * h)
+            qp, qm = q.copy(), q.copy()
+            qp[77] += h
+            qm[77] -= h
+            fq = (lossf(qv=qp) - lossf(qv=qm)) / (2 * h)
+            fd["h=%g" % h] = {"dL_deps": float("%.6e" % fe), "rel_eps": float("%.2e" % (abs(fe - g["deps"]) / abs(fe))),
+                              "dL_dq77": float("%.6e" % fq), "rel_q77": float("%.2e" % (abs(fq - g["dq"][77]) / abs(fq)))}
+        r["central_fd"] = fd
+        out[prec] = r
+        adj.close()
+        log("adjoint %s: forward %.3f s, backward %.3f s" % (prec, tf, tb))
+    out["note"] = ("at 2000 steps the trajectory is chaotic: FD with h = 1e-3 leaves the linear regime; "
+                   "FP64 FD with h = 1e-7 is the check of the gradient")
+    return out
 
 
 def cpu_baseline(c, steps, threads):
@@ -202,7 +330,20 @@ def main():
         extra["fp32_masked_key_steps_per_s"] = round(30 / (ms2 / 1e3), 3)
         s2.close()
         log("fp32 masked key: %.3f ms/step" % (ms2 / 30))
+        # full re-sort every 50 steps, box refits of the same tree in between
+        spec = make_spec(pkg, c, "fp32", pkg.MORTON_INTERLEAVE)
+        spec.resort_every = 50
+        s2 = pkg.Simulation(c["n"], spec)
+        s2.set_state(c["x"], c["v"], c["q"])
+        s2.run(args.warmup)
+        ms2, _ = timed_run(pkg, s2, args.steps)
+        extra["fp32_resort50_steps_per_s"] = round(args.steps / (ms2 / 1e3), 3)
+        s2.close()
+        log("fp32 resort every 50: %.4f ms/step" % (ms2 / args.steps))
         result["forms"] = extra
+        result["precision_form"] = precision_form(pkg)
+        result["batched_form"] = batched_form(pkg)
+        result["adjoint_form"] = adjoint_form(pkg)
 
     # ---- CPU baseline: the reference's threaded CPU path (C restatement) ----
     if not args.no_cpu:
