@@ -245,6 +245,13 @@ int nd_sim_time_fused_window(nd_sim *sim, int32_t nsteps, double *avg_fused_ms);
  * Call after nd_sim_create and before nd_sim_run; nrep = 0 clears it. */
 int nd_sim_set_replicas(nd_sim *sim, int32_t nrep, const int32_t *h_rep, const double *h_boxes);
 
+/* Trajectory frame for the poslog of simulate! (Simulator.jl:167, 245):
+ * writes the current positions in original atom order to device memory,
+ * d_out = n x 3 float (as_double = 0) or double (as_double = 1).
+ * Asynchronous on the context's stream (no host copy, no synchronisation),
+ * so frames can be collected in a device ring and copied out in blocks. */
+int nd_sim_record_positions(nd_sim *sim, void *d_out, int32_t as_double);
+
 /* ---- differentiable simulation: adjoint velocity Verlet (no reference
  * counterpart: NaiveDynamics.jl has no gradients; this differentiates the
  * simulate! step of Simulator.jl:198-225 with textbook forces) ----

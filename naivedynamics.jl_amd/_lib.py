@@ -129,6 +129,7 @@ SIGNATURES = {
     "nd_sim_kernel_timing": [_P, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)],
     "nd_sim_time_fused_window": [_P, _i32, ctypes.POINTER(ctypes.c_double)],
     "nd_sim_set_replicas": [_P, _i32, _P, _P],
+    "nd_sim_record_positions": [_P, _P, _i32],
     "nd_adj_create": [_P, _P, ctypes.POINTER(_P)],
     "nd_adj_destroy": [_P],
     "nd_adj_set_potential": [_P, ctypes.c_double, ctypes.c_double],

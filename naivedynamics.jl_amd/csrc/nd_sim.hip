@@ -215,6 +215,8 @@ struct PairAcc {
     }
 };
 
+constexpr int kPairBatch = 4;
+
 // PosS: storage scalar of positions/velocities; CalcS: pair arithmetic;
 // AccS: per-atom accumulation.  FP32 = <float,float,float>,
 // MIXED = <double,float,double>, FP64 = <double,double,double>.
@@ -254,27 +256,25 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         cnt = cnt < P.cap ? cnt : P.cap;
         // this wave's block of the list: row k = 64 consecutive int32
         const int32_t *col = nbr + ((int64_t)(i >> 6) * P.cap) * 64 + (i & 63);
+        // kPairBatch neighbours per iteration: all their index loads, then all
+        // their position gathers, are in flight before the first pair's maths
         int32_t k = 0;
-        for (; k + 4 <= cnt; k += 4) {
-            const int32_t j0 = col[(k + 0) * 64];
-            const int32_t j1 = col[(k + 1) * 64];
-            const int32_t j2 = col[(k + 2) * 64];
-            const int32_t j3 = col[(k + 3) * 64];
+        for (; k + kPairBatch <= cnt; k += kPairBatch) {
+            int32_t j[kPairBatch];
+#pragma unroll
+            for (int u = 0; u < kPairBatch; ++u) j[u] = col[(k + u) * 64];
             if constexpr (FX) {
-                const int4 p0 = fix_in[j0], p1 = fix_in[j1], p2 = fix_in[j2], p3 = fix_in[j3];
-                acc.add(p0);
-                acc.add(p1);
-                acc.add(p2);
-                acc.add(p3);
+                int4 q[kPairBatch];
+#pragma unroll
+                for (int u = 0; u < kPairBatch; ++u) q[u] = fix_in[j[u]];
+#pragma unroll
+                for (int u = 0; u < kPairBatch; ++u) acc.add(q[u]);
             } else {
-                const P4 p0 = pos_in[j0];
-                const P4 p1 = pos_in[j1];
-                const P4 p2 = pos_in[j2];
-                const P4 p3 = pos_in[j3];
-                acc.add(p0);
-                acc.add(p1);
-                acc.add(p2);
-                acc.add(p3);
+                P4 q[kPairBatch];
+#pragma unroll
+                for (int u = 0; u < kPairBatch; ++u) q[u] = pos_in[j[u]];
+#pragma unroll
+                for (int u = 0; u < kPairBatch; ++u) acc.add(q[u]);
             }
         }
         for (; k < cnt; ++k) {
@@ -860,6 +860,20 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         atomicMax(&misc->maxcnt, m);
         if (m > cap) atomicOr(&misc->overflow, 1);
     }
+}
+
+// Trajectory frame: slot s's position to row orig[s] of out (n x 3).
+template <class PosS, class OutT>
+__global__ void __launch_bounds__(nd::kBlock)
+k_record(const typename V4<PosS>::T *__restrict__ pos, const int32_t *__restrict__ orig, int32_t n,
+         OutT *__restrict__ out) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const auto p = pos[s];
+    OutT *o = out + 3 * (int64_t)orig[s];
+    o[0] = (OutT)p.x;
+    o[1] = (OutT)p.y;
+    o[2] = (OutT)p.z;
 }
 
 // FP32 copy of FP64 positions for a box refit without a re-sort.
@@ -1668,6 +1682,26 @@ int nd_sim_set_replicas(nd_sim *s, int32_t nrep, const int32_t *h_rep, const dou
     ND_HIP(hipStreamSynchronize(ctx->stream));
     s->nrep = nrep;
     drop_graphs(s);  // captured launches carry the wall pointers
+    ND_API_END(ctx)
+}
+
+int nd_sim_record_positions(nd_sim *s, void *d_out, int32_t as_double) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s && d_out, "nd_sim_record_positions: null argument");
+    ND_REQUIRE(s->have_state, "nd_sim_record_positions: call nd_sim_set_state first");
+    ND_HIP(hipSetDevice(ctx->device));
+    const unsigned g = nd::grid_for(s->n);
+    hipStream_t st = ctx->stream;
+    const int32_t *orig = s->orig[s->vcur].as<int32_t>();
+    if (s->wide) {
+        if (as_double) k_record<double, double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), orig, s->n, (double *)d_out);
+        else k_record<double, float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), orig, s->n, (float *)d_out);
+    } else {
+        if (as_double) k_record<float, double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), orig, s->n, (double *)d_out);
+        else k_record<float, float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), orig, s->n, (float *)d_out);
+    }
+    ND_HIP(hipGetLastError());
     ND_API_END(ctx)
 }
 

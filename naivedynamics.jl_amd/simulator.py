@@ -151,23 +151,43 @@ class Simulation:
         return ms.value, n.value
 
 
-def simulate_(sys, spec, ctx=None):
+def simulate_(sys, spec, ctx=None, ring=64):
     """simulate!(sys, spec, clct) (Simulator.jl:154-256): advances `sys` by
     spec.duration steps in place; returns poslog, the positions every
-    `record_every` steps (only the start and end if record_every == 0)."""
+    `record_every` steps (only the start and end if record_every == 0).
+
+    Frames are written on the device in original atom order
+    (nd_sim_record_positions) into a ring of up to `ring` frames, which is
+    copied to the host in one transfer whenever it fills and at the end."""
     n = len(sys.position)
     sim = Simulation(n, spec, ctx)
     try:
         sim.set_state(sys.position, sys.velocity, sys.charge, sys.mass)
         poslog = [np.array(sys.position, np.float64, copy=True)]
         stride = spec.record_every if spec.record_every > 0 else spec.duration
+        nframes = (spec.duration + stride - 1) // stride if stride > 0 else 0
+        slots = max(1, min(ring, nframes))
+        buf = _lib.DeviceArray(sim.ctx, (slots, n, 3), np.float64)
+        used = 0
+
+        def drain():
+            if used:
+                frames = buf.to_host(used)
+                poslog.extend(frames[k] for k in range(used))
+
         done = 0
         while done < spec.duration:
             k = min(stride, spec.duration - done)
             sim.run(k)
             done += k
-            x, v, f = sim.get_state()
-            poslog.append(x)
+            check(load().nd_sim_record_positions(sim.handle, ctypes.c_void_p(buf.ptr.value + used * n * 24), 1),
+                  sim.ctx, "nd_sim_record_positions")
+            used += 1
+            if used == slots:
+                drain()
+                used = 0
+        drain()
+        buf.free()
         x, v, f = sim.get_state()
         sys.position[...] = x.astype(np.asarray(sys.position).dtype)
         sys.velocity[...] = v.astype(np.asarray(sys.velocity).dtype)

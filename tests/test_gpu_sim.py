@@ -208,3 +208,22 @@ def test_pair_list_force_api(pkg, O):
     assert abs(e_lj + e_c - U) < 1e-4 * abs(U)
     Flj, Ulj, _, _ = O.forces64(x64, None, O.o64_params(rc_lj=2.5))
     assert _rel(f_lj, Flj) < 1e-4 and abs(e_lj - Ulj) < 1e-4 * abs(Ulj)
+
+
+def test_simulate_poslog_frames_match_the_state(pkg):
+    """poslog frames recorded on the device (ring of 2, drained twice) equal
+    the positions get_state returns after the same number of steps."""
+    x, v, q, L = _cfg(pkg, 4096, 9)
+    spec = _spec(pkg, L, "fp32", steps=40, record_every=10)
+    sysobj = pkg.ObjectCollection(position=x.copy(), velocity=v.copy(), charge=q)
+    poslog = pkg.simulate_(sysobj, spec, ring=2)
+    assert len(poslog) == 5
+    assert np.array_equal(poslog[0], x)
+    for k in (1, 3):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32"))
+        sim.set_state(x, v, q)
+        sim.run(10 * k)
+        xs, _, _ = sim.get_state()
+        sim.close()
+        assert np.array_equal(poslog[k], xs)
+    assert np.array_equal(poslog[-1], sysobj.position)
