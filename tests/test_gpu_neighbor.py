@@ -108,6 +108,38 @@ def test_apl_sweep_vs_oracle(pkg, O, apl):
         _compare_with_oracle(pkg, O, x, apl, 0.1, pkg.PRED_SIMD)
 
 
+@pytest.mark.parametrize("shape", ["duplicates", "line", "plane", "two_clusters"])
+@pytest.mark.parametrize("morton", [0, 1])
+def test_degenerate_inputs_vs_oracle(pkg, O, shape, morton):
+    """Collisions (64 distinct points, each repeated 64 times: equal keys, ties
+    broken by index, pairs at distance 0), points on a line or a plane (keys
+    varying in one or two coordinates only, deep trees) and two far clusters
+    (one huge leaf-to-leaf gap)."""
+    rng = np.random.default_rng(hash(shape) % 1000)
+    n = 4096
+    if shape == "duplicates":
+        x = np.repeat(rng.random((64, 3)).astype(np.float32) * 3.0, 64, axis=0)
+        x = x[rng.permutation(n)]
+    elif shape == "line":
+        x = np.zeros((n, 3), np.float32)
+        x[:, 0] = rng.random(n) * 40.0
+    elif shape == "plane":
+        x = np.zeros((n, 3), np.float32)
+        x[:, :2] = rng.random((n, 2)) * 20.0
+    else:
+        x = (rng.random((n, 3)) * 2.0).astype(np.float32)
+        x[n // 2:] += 500.0
+    lo = sc = None
+    if morton == 1:
+        # the device derives the interleaved key's box from the data's bounding
+        # box (scale 1024 / extent, 0 on a flat axis); give the oracle the same
+        lo = x.min(axis=0)
+        ext = (x.max(axis=0) - lo).astype(np.float32)
+        sc = np.where(ext > 0, np.float32(1024.0) / np.where(ext > 0, ext, 1), 0).astype(np.float32)
+    for pred in (pkg.PRED_SCALAR, pkg.PRED_SIMD):
+        _compare_with_oracle(pkg, O, x, 4, 0.3, pred, morton, lo=lo, scale=sc)
+
+
 @pytest.mark.parametrize("n", [10, 100, 1000, 2000, 3000, 20000])
 def test_size_sweep_vs_oracle_and_bruteforce(pkg, O, n):
     x = _rand(n, 1.0, 12 + n)
