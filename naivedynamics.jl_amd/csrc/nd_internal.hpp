@@ -108,6 +108,8 @@ void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_strid
                        int32_t apl, float r, nd_node *nodes, int32_t *store);
 // Child-box form of the internal nodes (4 float4 per internal node).
 void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *wide);
+// Four-wide records of the internal nodes (8 float4 per internal node).
+void launch_lbvh_wide4(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *rec);
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
                  nd_node *nodes, int32_t *store, int32_t *status);

@@ -272,9 +272,60 @@ k_lbvh_wide(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__ 
     w[3] = make_float4(__int_as_float(L), __int_as_float(R), 0.0f, 0.0f);
 }
 
+// Four-wide records: rec[8k .. 8k+7] for internal node nL+1+k hold the
+// node's grandchildren (a child that is a leaf stands for itself), up to 4
+// entries in the order left child's children, right child's children, as
+// structure-of-arrays float4s (min x, min y, min z, max x, max y, max z; one
+// lane per entry), then their node numbers (int4; 0 = empty slot, whose box
+// is empty so it overlaps nothing), then padding.
+__global__ void __launch_bounds__(nd::kBlock)
+k_lbvh_wide4(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__ rec) {
+    const int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= nL - 1) return;
+    int32_t id[4] = {0, 0, 0, 0};
+    int32_t m = 0;
+    const int32_t nn = 2 * nL - 1;
+    const int32_t L = nodes[nL + k].left;
+    const int32_t R = (L >= 1 && L <= nn) ? nodes[L - 1].skip : 0;
+    const int32_t ch[2] = {L, R};
+    for (int c = 0; c < 2; ++c) {
+        const int32_t x = ch[c];
+        if (x < 1 || x > nn) continue;  // malformed tree: the walk flags the empty record
+        if (x <= nL) {
+            id[m++] = x;
+        } else {
+            const int32_t g0 = nodes[x - 1].left;
+            const int32_t g1 = (g0 >= 1 && g0 <= nn) ? nodes[g0 - 1].skip : 0;
+            if (g0 >= 1 && g0 <= nn) id[m++] = g0;
+            if (g1 >= 1 && g1 <= nn) id[m++] = g1;
+        }
+    }
+    float b[6][4];
+    for (int e = 0; e < 4; ++e) {
+        if (id[e] != 0) {
+            const nd_node &q = nodes[id[e] - 1];
+            b[0][e] = q.min[0]; b[1][e] = q.min[1]; b[2][e] = q.min[2];
+            b[3][e] = q.max[0]; b[4][e] = q.max[1]; b[5][e] = q.max[2];
+        } else {
+            b[0][e] = b[1][e] = b[2][e] = INFINITY;
+            b[3][e] = b[4][e] = b[5][e] = -INFINITY;
+        }
+    }
+    float4 *r = rec + 8 * (int64_t)k;
+    for (int d = 0; d < 6; ++d) r[d] = make_float4(b[d][0], b[d][1], b[d][2], b[d][3]);
+    r[6] = make_float4(__int_as_float(id[0]), __int_as_float(id[1]), __int_as_float(id[2]),
+                       __int_as_float(id[3]));
+    r[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace
 
 namespace nd {
+
+void launch_lbvh_wide4(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *rec) {
+    k_lbvh_wide4<<<grid_for(nL - 1), kBlock, 0, s>>>(nodes, nL, rec);
+    ND_HIP(hipGetLastError());
+}
 
 // Box-only update of an existing tree (same leaf order, same topology):
 // leaf boxes from the current positions, chunk boxes, internal boxes.

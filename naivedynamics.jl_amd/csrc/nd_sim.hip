@@ -689,26 +689,29 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Two-phase child-box walk for leaves of A atoms (A = 4, 8, 16, 32).
-// Phase 1 pops up to kFront internal nodes from the wave's LDS stack and
-// fetches their child-box records together (lane l loads float4 l & 3 of
-// node l >> 2), so one memory latency covers up to 16 nodes; every lane
-// tests its box against both children of each node, overlapped internal
-// children are pushed, and an overlapped leaf is queued for each group of
-// 8 lanes (8 consecutive atoms) with an overlapping lane, together with the
-// group's 8 overlap bits.  Phase 2 drains the 8 group queues side by side:
-// per step each group stages the atoms of its next leaf in its own LDS slot
-// (the following leaf's load already in flight) and each lane whose bit is
-// set tests them, collecting hits in a bit mask that it then appends.  A
-// group of 8 Morton-consecutive atoms overlaps about as many leaves as its
-// busiest lane, so the wave does far fewer leaf steps than a walk that
-// tests the whole wave against every leaf any lane overlaps.  A lane gets
-// exactly the neighbour set of its own single-atom walk.
+// Two-phase walk over four-wide records for leaves of A atoms (A = 4, 8, 16,
+// 32).  Every internal node has a record holding its grandchildren (a child
+// that is a leaf stands for itself): up to 4 entries with boxes and node
+// numbers.  Phase 1 pops up to kFront records from the wave's LDS stack and
+// fetches them together (lane l loads float4 l & 7 of records (l >> 3) + 8 u),
+// so one memory latency covers up to 16 records (two tree levels each); every lane
+// tests its box against each entry, overlapped internal entries are pushed,
+// and an overlapped leaf is queued for each group of 8 lanes (8 consecutive
+// atoms) with an overlapping lane, together with the group's 8 overlap bits.
+// Phase 2 drains the 8 group queues side by side: per step each group stages
+// the atoms of its next leaf in its own LDS slot (the following leaf's load
+// already in flight) and each lane whose bit is set tests them, collecting
+// hits in a bit mask that it then appends.  A group of 8 Morton-consecutive
+// atoms overlaps about as many leaves as its busiest lane, so the wave does
+// far fewer leaf steps than a walk that tests the whole wave against every
+// leaf any lane overlaps.  A lane gets exactly the neighbour set of its own
+// single-atom walk.
 // The stack never overflows for trees of depth <= 63 (any LBVH over 30-bit
-// keys): the frontier shrinks to one node (plain depth-first order) when
-// fewer than 64 free entries remain.  Queue entries pack (leaf << 8) | bits,
-// so leaves must number below 2^23.
-constexpr int kFront = 16, kStack2 = 192, kGroupQ = 64;
+// keys, i.e. <= 32 record levels): when fewer than kStackDfs free entries
+// remain the frontier shrinks to one record (plain depth-first order), which
+// adds at most 3 entries per record level.  Queue entries pack
+// (leaf << 8) | bits, so leaves must number below 2^23.
+constexpr int kFront = 16, kStack2 = 256, kStackDfs = 128, kGroupQ = 128;
 constexpr int32_t kMaxLeaves3 = 1 << 23;
 
 template <int A>
@@ -719,7 +722,7 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
     constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
     constexpr int kSlot = 4 * A + 4;     // floats per group slot (16 B pad: no bank conflicts)
-    constexpr int kBufF = 8 * kSlot > 256 ? 8 * kSlot : 256;  // node records use 64 float4
+    constexpr int kBufF = 8 * kSlot > 32 * kFront ? 8 * kSlot : 32 * kFront;  // records: 8 float4 each
     constexpr int kW = nd::kBlock / 64;
     __shared__ __align__(16) float s_buf[kW][kBufF];
     __shared__ int32_t s_stk[kW][kStack2];
@@ -757,7 +760,7 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     stk[0] = nL + 1;  // the root is internal (nL >= 2)
     wave_lds_sync();
     while (sp > 0 || nqu > 0) {
-        if (nqu > 0 && (sp == 0 || nqu > kGroupQ - 2 * kFront)) {
+        if (nqu > 0 && (sp == 0 || nqu > kGroupQ - 4 * kFront)) {
             // ---- phase 2: drain the group queues ----
             const int32_t len = __shfl(glen, grp, 64);
             int32_t steps = len;
@@ -812,42 +815,53 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
             wave_lds_sync();
             continue;
         }
-        // ---- phase 1: expand up to kFront nodes of the stack top ----
-        const int32_t room = kStack2 - 64 - sp;
+        // ---- phase 1: expand up to kFront records of the stack top ----
+        // (each record pushes at most 4 entries: k pops leave room for them)
+        const int32_t room = (kStack2 - kStackDfs - sp) / 3;
         int32_t k = sp < kFront ? sp : kFront;
         if (k > room) k = room > 1 ? room : 1;
         fetched += k;
-        // more internal nodes than the tree has, or a stack that could overflow
-        // (the k pops leave room for the at most 2k pushes below)
-        if (fetched > nL || sp + k > kStack2) { bad = true; break; }
+        // more records than the tree has internal nodes, or a stack that could overflow
+        if (fetched > nL || sp + 3 * k > kStack2) { bad = true; break; }
         sp -= k;
         {
-            const int32_t slot = lane >> 2;
-            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (slot < k) v = wide[4 * (int64_t)(stk[sp + slot] - nL - 1) + (lane & 3)];
+            // lane l loads float4 l & 7 of records (l >> 3) + 8 u, u < kFront / 8
+            float4 v[kFront / 8];
+#pragma unroll
+            for (int u = 0; u < kFront / 8; ++u) {
+                const int32_t slot = (lane >> 3) + 8 * u;
+                v[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (slot < k) v[u] = wide[8 * (int64_t)(stk[sp + slot] - nL - 1) + (lane & 7)];
+            }
             wave_lds_sync();
-            rec[lane] = v;
+#pragma unroll
+            for (int u = 0; u < kFront / 8; ++u) rec[lane + 64 * u] = v[u];
             wave_lds_sync();
         }
-        for (int32_t s = 0; s < k; ++s) {
-            const float4 a = rec[4 * s], b = rec[4 * s + 1], c = rec[4 * s + 2], d = rec[4 * s + 3];
-            const bool ovL = (qmn0 < a.w) & (qmx0 > a.x) & (qmn1 < b.x) & (qmx1 > a.y) &
-                             (qmn2 < b.y) & (qmx2 > a.z);
-            const bool ovR = (qmn0 < c.y) & (qmx0 > b.z) & (qmn1 < c.z) & (qmx1 > b.w) &
-                             (qmn2 < c.w) & (qmx2 > c.x);
-            const uint64_t mL = __builtin_amdgcn_ballot_w64(ovL), mR = __builtin_amdgcn_ballot_w64(ovR);
-            const int32_t Lc = __builtin_amdgcn_readfirstlane(__float_as_int(d.x));
-            const int32_t Rc = __builtin_amdgcn_readfirstlane(__float_as_int(d.y));
-            const uint32_t nn = (uint32_t)(2 * nL - 1);  // valid node numbers are 1 .. 2 nL - 1
-            if (((uint32_t)(Lc - 1) >= nn) | ((uint32_t)(Rc - 1) >= nn)) { bad = true; break; }
-            if (mL) {
-                if (Lc <= nL) enqueue(Lc, mL);
-                else stk[sp++] = Lc;
+        const uint32_t nn = (uint32_t)(2 * nL - 1);  // valid node numbers are 1 .. 2 nL - 1
+        auto entry = [&](float mnx, float mny, float mnz, float mxx, float mxy, float mxz, float idf) {
+            const int32_t c = __builtin_amdgcn_readfirstlane(__float_as_int(idf));
+            if (c == 0) return;
+            if ((uint32_t)(c - 1) >= nn) { bad = true; return; }
+            const bool ov = (qmn0 < mxx) & (qmx0 > mnx) & (qmn1 < mxy) & (qmx1 > mny) & (qmn2 < mxz) &
+                            (qmx2 > mnz);
+            const uint64_t mk = __builtin_amdgcn_ballot_w64(ov);
+            if (mk) {
+                if (c <= nL) enqueue(c, mk);
+                else stk[sp++] = c;
             }
-            if (mR) {
-                if (Rc <= nL) enqueue(Rc, mR);
-                else stk[sp++] = Rc;
-            }
+        };
+        for (int32_t r = 0; r < k; ++r) {
+            const float4 mnx = rec[8 * r], mny = rec[8 * r + 1], mnz = rec[8 * r + 2];
+            const float4 mxx = rec[8 * r + 3], mxy = rec[8 * r + 4], mxz = rec[8 * r + 5];
+            const float4 ids = rec[8 * r + 6];
+            // an internal node has two children, so a record has >= 2 entries
+            if (__builtin_amdgcn_readfirstlane(__float_as_int(ids.y)) == 0) { bad = true; break; }
+            entry(mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, ids.x);
+            entry(mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, ids.y);
+            entry(mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, ids.z);
+            entry(mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, ids.w);
+            if (bad) break;
         }
         if (bad) break;
         wave_lds_sync();
@@ -1002,8 +1016,10 @@ void launch_rebuild(nd_sim *s, bool full) {
         }
     } else {
         float4 *kids = s->kids.as<float4>();
-        nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
-        const bool two_phase = s->p.list_walk == 0 && s->nL < kMaxLeaves3;
+        const bool two_phase = s->p.list_walk == 0 && s->nL < kMaxLeaves3 &&
+                               (s->apl == 4 || s->apl == 8 || s->apl == 16 || s->apl == 32);
+        if (two_phase) nd::launch_lbvh_wide4(st, s->nodes.as<nd_node>(), s->nL, kids);
+        else nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
         if (two_phase && s->apl == 4)
             k_nlist3<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
         else if (two_phase && s->apl == 8)
@@ -1393,7 +1409,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->perm.reserve(sizeof(int32_t) * n);
         s->nodes.reserve(sizeof(nd_node) * (size_t)(2 * s->nL - 1));
         s->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(s->nL));
-        s->kids.reserve(4 * sizeof(float4) * (size_t)(s->nL - 1));
+        s->kids.reserve(8 * sizeof(float4) * (size_t)(s->nL - 1));  // four-wide records
         s->ncount.reserve(sizeof(int32_t) * n);
         s->misc.reserve(sizeof(SimMisc));
         s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
