@@ -25,6 +25,7 @@
 // / velocity accumulation in FP64), FP64 (everything in double).  Energies and the virial
 // are reduced per workgroup in FP64 and summed with Neumaier compensation.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "nd_internal.hpp"
@@ -164,47 +165,76 @@ __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nblocks) {
 
 // One neighbour's contribution: LJ (r < rc) + Coulomb (r < rc_coul), both
 // evaluated without branches and masked, so the loads of the next neighbours
-// can be issued before this one finishes.
+// can be issued before this one finishes.  One reciprocal square root per
+// pair gives 1/r, 1/r^2 and 1/r^6; with c12 = 48 eps sigma^12 and
+// c6 = 24 eps sigma^6
+//   F/r = (1/r^2) [ (1/r^6)(c12/r^6 - c6) + k qi qj / r ]
+// (the same force as 24 eps (2 (sigma/r)^12 - (sigma/r)^6) / r^2 + k qi qj / r^3).
+// ONE: both cutoffs are equal, so a single compare masks the sum.
 template <class PosS, class CalcS, class AccS, bool ENERGY>
 struct PairAcc {
     using P4 = typename V4<PosS>::T;
-    CalcS rc2, rcc2, sigma2, eps24, eps4, kcq, eljs, ecs;
+    CalcS rc2, rcc2, c12, c6, e12, e6, kcq, eljs, ecs;
     P4 pi;
     int4 pfx;     // MIXED: fixed-point self position
     float inv;    // MIXED: fixed-point unit
     AccS fx = 0, fy = 0, fz = 0, e = 0, w = 0;
     int32_t np = 0;
-    __device__ __forceinline__ void add(const P4 &pj) {
-        add_d((CalcS)(pi.x - pj.x), (CalcS)(pi.y - pj.y), (CalcS)(pi.z - pj.z), (CalcS)pj.w);
+    __device__ __forceinline__ void init(const StepParams &P) {
+        using Q = PP<CalcS>;
+        rc2 = Q::rc2(P);
+        rcc2 = Q::rcc2(P);
+        const CalcS s2 = Q::sigma2(P), s6 = s2 * s2 * s2;
+        c6 = Q::eps24(P) * s6;
+        c12 = (CalcS)2 * c6 * s6;
+        e6 = Q::eps4(P) * s6;
+        e12 = e6 * s6;
+        kcq = Q::kc(P) * (CalcS)pi.w;
+        eljs = Q::eljs(P);
+        ecs = Q::ecs(P);
+    }
+    // `valid` false: a padding slot (the neighbour is the atom itself), which
+    // contributes nothing
+    template <bool ONE> __device__ __forceinline__ void add(const P4 &pj, bool valid) {
+        add_d<ONE>((CalcS)(pi.x - pj.x), (CalcS)(pi.y - pj.y), (CalcS)(pi.z - pj.z), (CalcS)pj.w,
+                   valid);
     }
     // fixed-point neighbour (MIXED): exact integer difference, then FP32
-    __device__ __forceinline__ void add(const int4 &pj) {
+    template <bool ONE> __device__ __forceinline__ void add(const int4 &pj, bool valid) {
         const CalcS dx = (CalcS)(float)(int32_t)((uint32_t)pfx.x - (uint32_t)pj.x) * (CalcS)inv;
         const CalcS dy = (CalcS)(float)(int32_t)((uint32_t)pfx.y - (uint32_t)pj.y) * (CalcS)inv;
         const CalcS dz = (CalcS)(float)(int32_t)((uint32_t)pfx.z - (uint32_t)pj.z) * (CalcS)inv;
-        add_d(dx, dy, dz, (CalcS)__int_as_float(pj.w));
+        add_d<ONE>(dx, dy, dz, (CalcS)__int_as_float(pj.w), valid);
     }
-    __device__ __forceinline__ void add_d(const CalcS dx, const CalcS dy, const CalcS dz, const CalcS qj) {
+    template <bool ONE>
+    __device__ __forceinline__ void add_d(const CalcS dx, const CalcS dy, const CalcS dz, const CalcS qj,
+                                          const bool valid) {
         const CalcS r2 = dx * dx + dy * dy + dz * dz;
-        const CalcS ir2 = rcp_(r2);
-        const CalcS sr2 = sigma2 * ir2;
-        const CalcS sr6 = sr2 * sr2 * sr2;
-        const bool in_lj = r2 < rc2, in_c = r2 < rcc2;
+        const CalcS ir = rsq_(r2);
+        const CalcS ir2 = ir * ir;
+        const CalcS ir6 = ir2 * ir2 * ir2;
+        const CalcS qq = kcq * qj;
+        const bool in_lj = (r2 < rc2) & valid, in_c = ONE ? in_lj : (r2 < rcc2) & valid;
         // both terms are computed for every neighbour and then masked with a
         // select (a conditional expression over unevaluated operands would
         // become a branch per pair)
-        const CalcS flj_all = eps24 * ir2 * sr6 * (sr6 + sr6 - (CalcS)1);
-        const CalcS flj = in_lj ? flj_all : (CalcS)0;
-        const CalcS qq = kcq * qj;
-        const CalcS ir = rsq_(r2);
-        const CalcS fc_all = qq * ir * ir2;
-        const CalcS fc = in_c ? fc_all : (CalcS)0;
-        const CalcS fs = flj + fc;
+        const CalcS lj_all = ir6 * (c12 * ir6 - c6);
+        CalcS fs;
+        if (ONE) {
+            const CalcS t = (qq * ir + lj_all) * ir2;
+            fs = in_lj ? t : (CalcS)0;
+        } else {
+            // the outer select also clears padding slots (r = 0: 0 * inf)
+            const CalcS lj = in_lj ? lj_all : (CalcS)0;
+            const CalcS qm = in_c ? qq : (CalcS)0;
+            const CalcS t = (qm * ir + lj) * ir2;
+            fs = (in_lj | in_c) ? t : (CalcS)0;
+        }
         fx += (AccS)(fs * dx);
         fy += (AccS)(fs * dy);
         fz += (AccS)(fs * dz);
         if (ENERGY) {
-            const CalcS elj_all = eps4 * sr6 * (sr6 - (CalcS)1) - eljs;
+            const CalcS elj_all = ir6 * (e12 * ir6 - e6) - eljs;
             const CalcS ec_all = qq * (ir - ecs);
             const CalcS elj = in_lj ? elj_all : (CalcS)0;
             const CalcS ec = in_c ? ec_all : (CalcS)0;
@@ -216,6 +246,9 @@ struct PairAcc {
 };
 
 constexpr int kPairBatch = 4;
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format;
+// untyped loads ignore it otherwise).
+constexpr int32_t kBufWord3 = 0x00020000;
 
 // PosS: storage scalar of positions/velocities; CalcS: pair arithmetic;
 // AccS: per-atom accumulation.  FP32 = <float,float,float>,
@@ -228,7 +261,6 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
        double *__restrict__ partials, SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in,
        int4 *__restrict__ fix_out, Walls W) {
     using P4 = typename V4<PosS>::T;
-    using Q = PP<CalcS>;
     // MIXED gathers the fixed-point copy (16 B per neighbour)
     constexpr bool FX = sizeof(PosS) == 8 && sizeof(CalcS) == 4;
     __shared__ double sh[nd::kBlock / 64][4];
@@ -244,43 +276,76 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
             acc.pfx = fix_in[i];
             acc.inv = P.fx_inv;
         }
-        acc.rc2 = Q::rc2(P);
-        acc.rcc2 = Q::rcc2(P);
-        acc.sigma2 = Q::sigma2(P);
-        acc.eps24 = Q::eps24(P);
-        acc.eps4 = Q::eps4(P);
-        acc.kcq = Q::kc(P) * (CalcS)acc.pi.w;
-        acc.eljs = Q::eljs(P);
-        acc.ecs = Q::ecs(P);
+        acc.init(P);
         int32_t cnt = ncount[i];
         cnt = cnt < P.cap ? cnt : P.cap;
         // this wave's block of the list: row k = 64 consecutive int32
-        const int32_t *col = nbr + ((int64_t)(i >> 6) * P.cap) * 64 + (i & 63);
-        // kPairBatch neighbours per iteration: all their index loads, then all
-        // their position gathers, are in flight before the first pair's maths
-        int32_t k = 0;
-        for (; k + kPairBatch <= cnt; k += kPairBatch) {
-            int32_t j[kPairBatch];
-#pragma unroll
-            for (int u = 0; u < kPairBatch; ++u) j[u] = col[(k + u) * 64];
-            if constexpr (FX) {
-                int4 q[kPairBatch];
-#pragma unroll
-                for (int u = 0; u < kPairBatch; ++u) q[u] = fix_in[j[u]];
-#pragma unroll
-                for (int u = 0; u < kPairBatch; ++u) acc.add(q[u]);
+        // Software pipeline over batches of kPairBatch neighbours: while
+        // batch b's pairs are computed, the positions of batch b + 1 and the
+        // indices of batch b + 2 are in flight, so one memory latency (not a
+        // list load followed by a gather) separates consecutive batches.
+        // Both go through buffer loads: the row offset is a scalar operand and
+        // the gather offset is j times the record size.  Slots past the
+        // atom's count gather index -1, past the end of the buffer, which the
+        // hardware range check answers with zeros and no memory access; the
+        // pair maths masks them.
+        using G = std::conditional_t<FX, int4, P4>;
+        const int32_t last = P.cap - 1;
+        const int32_t wave = __builtin_amdgcn_readfirstlane(i >> 6);
+        const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(nbr + (int64_t)wave * P.cap * 64), 0, P.cap * 256, kBufWord3);
+        const uint32_t lane4 = (uint32_t)(i & 63) * 4u;
+        auto idx = [&](int32_t r) {
+            const uint32_t row = (uint32_t)(r < last ? r : last);
+            const int32_t v = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(lrs, lane4, row * 256u, 0);
+            return r < cnt ? v : -1;
+        };
+        const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
+            FX ? (void *)fix_in : (void *)pos_in, 0, n * (int32_t)sizeof(G), kBufWord3);
+        auto gather = [&](int32_t j) -> G {
+            const uint32_t off = (uint32_t)j * (uint32_t)sizeof(G);
+            G g;
+            if constexpr (sizeof(G) == 16) {
+                const auto a = __builtin_amdgcn_raw_buffer_load_b128(prs, off, 0, 0);
+                __builtin_memcpy(&g, &a, 16);
             } else {
-                P4 q[kPairBatch];
-#pragma unroll
-                for (int u = 0; u < kPairBatch; ++u) q[u] = pos_in[j[u]];
-#pragma unroll
-                for (int u = 0; u < kPairBatch; ++u) acc.add(q[u]);
+                const auto a = __builtin_amdgcn_raw_buffer_load_b128(prs, off, 0, 0);
+                const auto b = __builtin_amdgcn_raw_buffer_load_b128(prs, off + 16u, 0, 0);
+                __builtin_memcpy(&g, &a, 16);
+                __builtin_memcpy(reinterpret_cast<char *>(&g) + 16, &b, 16);
             }
-        }
-        for (; k < cnt; ++k) {
-            if constexpr (FX) acc.add(fix_in[col[k * 64]]);
-            else acc.add(pos_in[col[k * 64]]);
-        }
+            return g;
+        };
+        // Two register sets (a, b) take turns, so no batch is copied.
+        auto pairs = [&](auto one) {
+            constexpr bool ONE = decltype(one)::value;
+            constexpr int B = kPairBatch;
+            int32_t ja[B], jb[B];
+            G qa[B], qb[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) qa[u] = gather(idx(u));
+#pragma unroll
+            for (int u = 0; u < B; ++u) jb[u] = idx(B + u);
+            for (int32_t k = 0;; k += 2 * B) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) qb[u] = gather(jb[u]);
+#pragma unroll
+                for (int u = 0; u < B; ++u) ja[u] = idx(k + 2 * B + u);
+#pragma unroll
+                for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], k + u < cnt);
+                if (k + B >= cnt) break;
+#pragma unroll
+                for (int u = 0; u < B; ++u) qa[u] = gather(ja[u]);
+#pragma unroll
+                for (int u = 0; u < B; ++u) jb[u] = idx(k + 3 * B + u);
+#pragma unroll
+                for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], k + B + u < cnt);
+                if (k + 2 * B >= cnt) break;
+            }
+        };
+        // equal cutoffs (uniform over the launch): one mask per pair
+        if (acc.rc2 == acc.rcc2) pairs(std::true_type{});
+        else pairs(std::false_type{});
         fx = acc.fx;
         fy = acc.fy;
         fz = acc.fz;
