@@ -124,10 +124,13 @@ def precision_form(pkg, steps=500):
         ms, _ = timed_run(pkg, sim, steps)
         x, _, _ = sim.get_state()
         e = sim.energies()
+        W = c["rebuild_every"]
+        res_k = float(np.mean([sim.time_fused_window(W) for _ in range(5)]))
         E0 = e0[0, 1] + e0[0, 2]
         E1 = e[-1, 1] + e[-1, 2]
-        res[prec] = dict(f0=f0, x=x, ms=ms, drift=(E1 - E0) / abs(E0))
+        res[prec] = dict(f0=f0, x=x, ms=ms, drift=(E1 - E0) / abs(E0), kms=res_k)
         out[prec + "_steps_per_s"] = round(steps / (ms / 1e3), 2)
+        out[prec + "_force_kernel_ms"] = round(res_k, 5)
         out[prec + "_energy_drift"] = float("%.3e" % res[prec]["drift"])
         sim.close()
         log("precision %s: %.4f ms/step, energy drift %.2e" % (prec, ms / steps, res[prec]["drift"]))
@@ -140,6 +143,7 @@ def precision_form(pkg, steps=500):
         out[prec + "_rms_atom_force_err"] = float("%.3e" % np.sqrt(np.mean((d / fn) ** 2)))
         out[prec + "_rmsd_vs_fp64"] = float("%.3e" % np.sqrt(np.mean(np.sum((res[prec]["x"] - res["fp64"]["x"]) ** 2, axis=1))))
     out["mixed_over_fp64"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
+    out["mixed_over_fp64_force_kernel"] = round(res["fp64"]["kms"] / res["mixed"]["kms"], 3)
     out["note"] = ("force errors against the FP64 GPU forces at the initial state (per-atom errors "
                    "relative to max(|F_i|, 1)); RMSD against the FP64 trajectory")
     return out
@@ -323,9 +327,13 @@ def main():
         for prec in ("mixed", "fp64"):
             s2, ms2, _, _ = gpu_form(pkg, c, prec, pkg.MORTON_INTERLEAVE, 20, k)
             extra[prec + "_steps_per_s"] = round(k / (ms2 / 1e3), 3)
+            W = c["rebuild_every"]
+            extra[prec + "_force_kernel_ms"] = round(float(np.mean([s2.time_fused_window(W) for _ in range(10)])), 5)
             s2.close()
-            log("%s: %.3f ms/step" % (prec, ms2 / k))
+            log("%s: %.3f ms/step, force kernel %.4f ms" % (prec, ms2 / k, extra[prec + "_force_kernel_ms"]))
         extra["mixed_over_fp64"] = round(extra["mixed_steps_per_s"] / extra["fp64_steps_per_s"], 3)
+        # the pair-force + integrate kernel alone (the list build is shared)
+        extra["mixed_over_fp64_force_kernel"] = round(extra["fp64_force_kernel_ms"] / extra["mixed_force_kernel_ms"], 3)
         s2, ms2, _, _ = gpu_form(pkg, c, "fp32", pkg.MORTON_MASKED, 10, 30)
         extra["fp32_masked_key_steps_per_s"] = round(30 / (ms2 / 1e3), 3)
         s2.close()

@@ -179,7 +179,19 @@ struct PairAcc {
     int4 pfx;     // MIXED: fixed-point self position
     float inv;    // MIXED: fixed-point unit
     AccS fx = 0, fy = 0, fz = 0, e = 0, w = 0;
+    // MIXED: FP32 force sums over one batch of neighbours, added to the FP64
+    // sums by flush() (one conversion and FP64 add per batch, not per pair)
+    static constexpr bool kBatchSums = sizeof(AccS) != sizeof(CalcS);
+    CalcS bx = 0, by = 0, bz = 0;
     int32_t np = 0;
+    __device__ __forceinline__ void flush() {
+        if constexpr (kBatchSums) {
+            fx += (AccS)bx;
+            fy += (AccS)by;
+            fz += (AccS)bz;
+            bx = by = bz = 0;
+        }
+    }
     __device__ __forceinline__ void init(const StepParams &P) {
         using Q = PP<CalcS>;
         rc2 = Q::rc2(P);
@@ -230,9 +242,15 @@ struct PairAcc {
             const CalcS t = (qm * ir + lj) * ir2;
             fs = (in_lj | in_c) ? t : (CalcS)0;
         }
-        fx += (AccS)(fs * dx);
-        fy += (AccS)(fs * dy);
-        fz += (AccS)(fs * dz);
+        if constexpr (kBatchSums) {
+            bx += fs * dx;
+            by += fs * dy;
+            bz += fs * dz;
+        } else {
+            fx += fs * dx;
+            fy += fs * dy;
+            fz += fs * dz;
+        }
         if (ENERGY) {
             const CalcS elj_all = ir6 * (e12 * ir6 - e6) - eljs;
             const CalcS ec_all = qq * (ir - ecs);
@@ -269,6 +287,13 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     const int32_t n = P.n;
     AccS fx = 0, fy = 0, fz = 0;
     double pe = 0.0, vir = 0.0, ke = 0.0, np = 0.0;
+    // list length (the wave's block of the list holds rows of 64 int32), and
+    // the wave's longest list: rows past it are never needed, so the
+    // look-ahead re-reads the last needed row (a cache hit) instead
+    const int32_t cnt = i < n ? min(ncount[i], P.cap) : 0;
+    int32_t wmax = cnt;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    const int32_t last = max(__builtin_amdgcn_readfirstlane(wmax) - 1, 0);
     if (i < n) {
         PairAcc<PosS, CalcS, AccS, ENERGY> acc;
         acc.pi = pos_in[i];
@@ -277,9 +302,6 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
             acc.inv = P.fx_inv;
         }
         acc.init(P);
-        int32_t cnt = ncount[i];
-        cnt = cnt < P.cap ? cnt : P.cap;
-        // this wave's block of the list: row k = 64 consecutive int32
         // Software pipeline over batches of kPairBatch neighbours: while
         // batch b's pairs are computed, the positions of batch b + 1 and the
         // indices of batch b + 2 are in flight, so one memory latency (not a
@@ -290,7 +312,6 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         // hardware range check answers with zeros and no memory access; the
         // pair maths masks them.
         using G = std::conditional_t<FX, int4, P4>;
-        const int32_t last = P.cap - 1;
         const int32_t wave = __builtin_amdgcn_readfirstlane(i >> 6);
         const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(nbr + (int64_t)wave * P.cap * 64), 0, P.cap * 256, kBufWord3);
@@ -333,6 +354,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 for (int u = 0; u < B; ++u) ja[u] = idx(k + 2 * B + u);
 #pragma unroll
                 for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], k + u < cnt);
+                acc.flush();
                 if (k + B >= cnt) break;
 #pragma unroll
                 for (int u = 0; u < B; ++u) qa[u] = gather(ja[u]);
@@ -340,6 +362,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 for (int u = 0; u < B; ++u) jb[u] = idx(k + 3 * B + u);
 #pragma unroll
                 for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], k + B + u < cnt);
+                acc.flush();
                 if (k + 2 * B >= cnt) break;
             }
         };
