@@ -1143,7 +1143,7 @@ void launch_sort_and_build(nd_sim *s) {
         k_sim_keys<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), n, s->p.morton_mode,
                                                     lo, scale, codes, perm, misc);
     // Both keys are < 2^30, so bits 30..31 of the sign-flipped key are constant.
-    nd::radix_sort_pairs(st, s->sort, codes, perm, n, 0, 30);
+    nd::radix_sort_keys30(st, s->sort, codes, perm, n, &codes, &perm);
     const int a = s->cur ^ 1, va = s->vcur ^ 1;
     if (s->wide)
         k_permute<double><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
@@ -1505,7 +1505,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         const int32_t ntiles = (int32_t)nd::radix_scratch_blocks(s->n);
         s->sort.keys_alt.reserve(sizeof(int32_t) * n);
         s->sort.vals_alt.reserve(sizeof(int32_t) * n);
-        s->sort.hist.reserve(sizeof(int32_t) * (256 * (size_t)ntiles + 256));
+        s->sort.hist.reserve(sizeof(int32_t) * (1024 * (size_t)ntiles + 1024));  // 10-bit digits
         ND_HIP(hipMemsetAsync(s->misc.p, 0, sizeof(SimMisc), ctx->stream));
         int32_t cap = p->max_neighbors;
         if (cap <= 0) {

@@ -1,12 +1,13 @@
 // nd_sort.hip -- K2: stable LSD radix sort of (int32 key, int32 value) pairs in
-// signed-key order, 8-bit digits, for sortperm (BVHTraverse.jl:570-573).
+// signed-key order, for sortperm (BVHTraverse.jl:570-573): 8-bit digits in
+// general, three 10-bit digits for the simulation's 30-bit keys.
 //
 // Per pass: (1) per-tile digit histograms, (2) one exclusive scan over the
 // digit-major [digit][tile] table, (3) stable scatter.  The scatter ranks
-// each 64-key round of a wave with eight ballots (lanes holding the same
-// digit form a peer mask; rank = popcount of lower peers), carries running
-// per-wave digit counters in LDS, stages the tile in LDS in sorted order and
-// writes every digit run to global memory contiguously.
+// each 64-key round of a wave with one ballot per digit bit (lanes holding
+// the same digit form a peer mask; rank = popcount of lower peers), carries
+// running per-wave digit counters in LDS, stages the tile in LDS in sorted
+// order and writes every digit run to global memory contiguously.
 #include "nd_internal.hpp"
 
 namespace {
@@ -15,25 +16,31 @@ constexpr int kWaves = nd::kBlock / 64;     // 4
 constexpr int kRounds = 4;                  // 64-key rounds per wave
 constexpr int kTile = nd::kBlock * kRounds; // 1024 keys per workgroup
 
+template <int BITS>
 __device__ __forceinline__ uint32_t digit_of(int32_t key, int shift) {
-    return (((uint32_t)key ^ 0x80000000u) >> shift) & 0xFFu;
+    return (((uint32_t)key ^ 0x80000000u) >> shift) & ((1u << BITS) - 1u);
 }
 
+// Per-tile histogram of one digit (2^BITS bins).
+template <int BITS>
 __global__ void __launch_bounds__(nd::kBlock)
 k_radix_hist(const int32_t *__restrict__ keys, int32_t n, int shift, int32_t ntiles,
              int32_t *__restrict__ hist) {
-    __shared__ int32_t cnt[256];
+    constexpr int R = 1 << BITS, DPT = R / nd::kBlock;
+    __shared__ int32_t cnt[R];
     const int t = threadIdx.x;
-    cnt[t] = 0;
+#pragma unroll
+    for (int c = 0; c < DPT; ++c) cnt[t * DPT + c] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kTile;
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         int64_t i = base + r * nd::kBlock + t;
-        if (i < n) atomicAdd(&cnt[digit_of(keys[i], shift)], 1);
+        if (i < n) atomicAdd(&cnt[digit_of<BITS>(keys[i], shift)], 1);
     }
     __syncthreads();
-    hist[(int64_t)t * ntiles + blockIdx.x] = cnt[t];
+#pragma unroll
+    for (int c = 0; c < DPT; ++c) hist[(int64_t)(t * DPT + c) * ntiles + blockIdx.x] = cnt[t * DPT + c];
 }
 
 // Exclusive scan of v over the 256 lanes of a workgroup; returns the prefix
@@ -74,14 +81,18 @@ k_radix_scan_rows(int32_t *__restrict__ hist, int32_t ntiles, int32_t *__restric
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+// Stable scatter of one tile for one digit.  Thread t owns digits
+// t * DPT .. t * DPT + DPT - 1 in the per-digit bookkeeping.
+template <int BITS>
 __global__ void __launch_bounds__(nd::kBlock)
 k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
                 int32_t n, int shift, int32_t ntiles, const int32_t *__restrict__ hist,
                 const int32_t *__restrict__ totals, int32_t *__restrict__ keys_out,
                 int32_t *__restrict__ vals_out) {
-    __shared__ int32_t wcnt[kWaves][256];
-    __shared__ int32_t bpre[256];
-    __shared__ int32_t gofs[256];
+    constexpr int R = 1 << BITS, DPT = R / nd::kBlock;
+    __shared__ int32_t wcnt[kWaves][R];
+    __shared__ int32_t bpre[R];
+    __shared__ int32_t gofs[R];
     __shared__ int32_t wsum[kWaves];
     __shared__ int32_t tot0;
     __shared__ int32_t sk[kTile];
@@ -93,14 +104,29 @@ k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__
     const int64_t base = (int64_t)blockIdx.x * kTile;
     const int32_t nvalid = (int32_t)((n - base) < kTile ? (n - base) : kTile);
 
-    // global start of digit t in this tile = (keys with smaller digits) +
-    // (keys with digit t in earlier tiles)
-    const int32_t dbase = block_exclusive_scan256(totals[t], wsum, &tot0);
-    gofs[t] = dbase + hist[(int64_t)t * ntiles + blockIdx.x];
+    // global start of digit d in this tile = (keys with smaller digits) +
+    // (keys with digit d in earlier tiles)
+    {
+        int32_t tv[DPT], sum = 0;
 #pragma unroll
-    for (int k = 0; k < kWaves; ++k) wcnt[k][t] = 0;
+        for (int c = 0; c < DPT; ++c) { tv[c] = totals[t * DPT + c]; sum += tv[c]; }
+        int32_t run = block_exclusive_scan256(sum, wsum, &tot0);
+#pragma unroll
+        for (int c = 0; c < DPT; ++c) {
+            const int d = t * DPT + c;
+            gofs[d] = run + hist[(int64_t)d * ntiles + blockIdx.x];
+            run += tv[c];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k)
+#pragma unroll
+        for (int c = 0; c < DPT; ++c) wcnt[k][t * DPT + c] = 0;
     __syncthreads();
 
+    // rank each 64-key round of a wave: lanes with the same digit (found with
+    // one ballot per digit bit) form a peer mask; rank = running count of the
+    // digit in this wave + number of lower peers
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int32_t key[kRounds], val[kRounds], rank[kRounds];
     uint32_t dig[kRounds];
@@ -110,11 +136,11 @@ k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__
         const bool valid = li < nvalid;
         key[r] = valid ? keys_in[base + li] : 0;
         val[r] = valid ? vals_in[base + li] : 0;
-        const uint32_t d = digit_of(key[r], shift);
+        const uint32_t d = digit_of<BITS>(key[r], shift);
         dig[r] = d;
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
@@ -126,26 +152,29 @@ k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__
     }
     __syncthreads();
 
-    // Per digit: offsets of each wave's run and the block-level digit prefix.
-    int32_t tot = 0;
+    // per digit: offsets of each wave's run, and the block-level digit prefix
+    {
+        int32_t tv[DPT], sum = 0;
 #pragma unroll
-    for (int k = 0; k < kWaves; ++k) {
-        int32_t c = wcnt[k][t];
-        wcnt[k][t] = tot;
-        tot += c;
-    }
-    // exclusive scan of tot over the 256 digits
-    int32_t inc = tot;
+        for (int c = 0; c < DPT; ++c) {
+            const int d = t * DPT + c;
+            int32_t tot = 0;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        int32_t v = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += v;
+            for (int k = 0; k < kWaves; ++k) {
+                const int32_t cw = wcnt[k][d];
+                wcnt[k][d] = tot;
+                tot += cw;
+            }
+            tv[c] = tot;
+            sum += tot;
+        }
+        int32_t run = block_exclusive_scan256(sum, wsum, &tot0);
+#pragma unroll
+        for (int c = 0; c < DPT; ++c) {
+            bpre[t * DPT + c] = run;
+            run += tv[c];
+        }
     }
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    int32_t wbase = 0;
-    for (int k = 0; k < w; ++k) wbase += wsum[k];
-    bpre[t] = wbase + inc - tot;
     __syncthreads();
 
 #pragma unroll
@@ -159,12 +188,13 @@ k_radix_scatter(const int32_t *__restrict__ keys_in, const int32_t *__restrict__
     }
     __syncthreads();
 
+    // every digit run leaves contiguously
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         const int32_t k = r * nd::kBlock + t;
         if (k < nvalid) {
             const int32_t kk = sk[k];
-            const uint32_t d = digit_of(kk, shift);
+            const uint32_t d = digit_of<BITS>(kk, shift);
             const int64_t g = (int64_t)gofs[d] + (k - bpre[d]);
             if (g >= 0 && g < n) {
                 keys_out[g] = kk;
@@ -202,29 +232,53 @@ namespace nd {
 
 size_t radix_scratch_blocks(int32_t n) { return (size_t)((n + kTile - 1) / kTile); }
 
-void radix_sort_pairs(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
-                      int32_t first_bit, int32_t end_bit) {
-    if (n <= 1 || end_bit <= first_bit) return;
+namespace {
+
+// One LSD pass per BITS-bit digit from first_bit up to end_bit; returns the
+// number of passes (odd: the result is in the alternate buffers).
+template <int BITS>
+int radix_passes(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
+                 int32_t first_bit, int32_t end_bit) {
+    constexpr int R = 1 << BITS;
     const int32_t ntiles = (int32_t)radix_scratch_blocks(n);
     b.keys_alt.reserve(sizeof(int32_t) * (size_t)n);
     b.vals_alt.reserve(sizeof(int32_t) * (size_t)n);
-    b.hist.reserve(sizeof(int32_t) * (256 * (size_t)ntiles + 256));
-    int32_t *hist = b.hist.as<int32_t>(), *totals = hist + 256 * (size_t)ntiles;
+    b.hist.reserve(sizeof(int32_t) * ((size_t)R * (size_t)ntiles + R));
+    int32_t *hist = b.hist.as<int32_t>(), *totals = hist + (size_t)R * (size_t)ntiles;
     int32_t *ki = keys, *vi = vals, *ko = b.keys_alt.as<int32_t>(), *vo = b.vals_alt.as<int32_t>();
     int passes = 0;
-    for (int shift = first_bit; shift < end_bit; shift += 8) {
-        k_radix_hist<<<ntiles, kBlock, 0, s>>>(ki, n, shift, ntiles, hist);
-        k_radix_scan_rows<<<256, kBlock, 0, s>>>(hist, ntiles, totals);
-        k_radix_scatter<<<ntiles, kBlock, 0, s>>>(ki, vi, n, shift, ntiles, hist, totals, ko, vo);
+    for (int shift = first_bit; shift < end_bit; shift += BITS) {
+        k_radix_hist<BITS><<<ntiles, kBlock, 0, s>>>(ki, n, shift, ntiles, hist);
+        k_radix_scan_rows<<<R, kBlock, 0, s>>>(hist, ntiles, totals);
+        k_radix_scatter<BITS><<<ntiles, kBlock, 0, s>>>(ki, vi, n, shift, ntiles, hist, totals, ko, vo);
         std::swap(ki, ko);
         std::swap(vi, vo);
         ++passes;
     }
     ND_HIP(hipGetLastError());
+    return passes;
+}
+
+}  // namespace
+
+void radix_sort_pairs(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
+                      int32_t first_bit, int32_t end_bit) {
+    if (n <= 1 || end_bit <= first_bit) return;
+    const int passes = radix_passes<8>(s, b, keys, vals, n, first_bit, end_bit);
     if (passes & 1) {
-        ND_HIP(hipMemcpyAsync(keys, ki, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
-        ND_HIP(hipMemcpyAsync(vals, vi, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        ND_HIP(hipMemcpyAsync(keys, b.keys_alt.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        ND_HIP(hipMemcpyAsync(vals, b.vals_alt.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
     }
+}
+
+void radix_sort_keys30(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
+                       int32_t **keys_sorted, int32_t **vals_sorted) {
+    *keys_sorted = keys;
+    *vals_sorted = vals;
+    if (n <= 1) return;
+    radix_passes<10>(s, b, keys, vals, n, 0, 30);  // three passes
+    *keys_sorted = b.keys_alt.as<int32_t>();
+    *vals_sorted = b.vals_alt.as<int32_t>();
 }
 
 // Sorts keys/vals in place over only the 8-bit digit windows whose bits vary
