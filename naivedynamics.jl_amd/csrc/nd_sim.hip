@@ -823,14 +823,17 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
 
     const int32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = i < n;
-    int32_t *lst = nbr + ((int64_t)(i >> 6) * cap) * 64 + (i & 63);
+    // this wave's block of the list as a buffer resource: a row at or past cap
+    // lies outside it, and the hardware drops such a store (no branch per hit)
+    const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(nbr + (int64_t)__builtin_amdgcn_readfirstlane(i >> 6) * cap * 64), 0, cap * 256, kBufWord3);
+    uint32_t voff = (uint32_t)lane * 4u;  // byte offset of row cnt in this lane's column
     const float4 p = active ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     // lanes without an atom get an empty query box (it overlaps nothing)
     const float qmn0 = active ? p.x - h : INFINITY, qmn1 = active ? p.y - h : INFINITY,
                 qmn2 = active ? p.z - h : INFINITY;
     const float qmx0 = active ? p.x + h : -INFINITY, qmx1 = active ? p.y + h : -INFINITY,
                 qmx2 = active ? p.z + h : -INFINITY;
-    int32_t cnt = 0;
     bool bad = false;
 
     int32_t glen = 0;  // lanes 0..7: length of queue `lane`
@@ -893,8 +896,8 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
                     while (hits) {
                         const int32_t m = __builtin_ctz(hits);
                         hits &= hits - 1u;
-                        if (cnt < cap) lst[cnt * 64] = base + m;
-                        ++cnt;
+                        __builtin_amdgcn_raw_buffer_store_b32(base + m, lrs, voff, 0, 0);
+                        voff += 256u;
                     }
                 }
             }
@@ -955,6 +958,7 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         wave_lds_sync();
     }
     if (bad && lane == 0) atomicOr(&misc->status, nd::kStatusTree);
+    const int32_t cnt = (int32_t)((voff - (uint32_t)lane * 4u) >> 8);
     if (active) ncount[i] = cnt;
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
