@@ -1048,6 +1048,10 @@ void drop_graphs(nd_sim *s) {
 }
 
 void alloc_lists(nd_sim *s, int32_t cap) {
+    // a wave's block of the list (cap rows of 256 B) is addressed through a
+    // buffer resource with a 31-bit byte range
+    if (cap > (1 << 23))
+        throw nd::Error(ND_ERR_CAPACITY, "simulation: more than 2^23 neighbours per atom");
     s->cap = cap;
     s->sp.cap = cap;
     // one block of cap rows x 64 slots per 64 atoms (n rounded up to 64)
@@ -1456,6 +1460,9 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     ND_REQUIRE(ctx && p && out, "nd_sim_create: null argument");
     *out = nullptr;
     ND_REQUIRE(p->n >= 2, "nd_sim_create: need at least two atoms");
+    // the pair kernel addresses positions through a buffer resource whose
+    // byte range (32 B per atom in FP64) must fit in 31 bits
+    ND_REQUIRE(p->n <= (1 << 26), "nd_sim_create: at most 2^26 atoms");
     ND_REQUIRE(p->atomsperleaf >= 1 && p->n % p->atomsperleaf == 0 && p->n / p->atomsperleaf >= 2,
                "nd_sim_create: atomsperleaf must divide n and give >= 2 leaves");
     ND_REQUIRE(p->precision >= ND_PREC_FP32 && p->precision <= ND_PREC_FP64, "nd_sim_create: precision");
