@@ -104,8 +104,6 @@ template <class S> struct V4;
 template <> struct V4<float> { using T = float4; };
 template <> struct V4<double> { using T = double4; };
 
-__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
 __device__ __forceinline__ float rsq_(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
 

@@ -164,6 +164,14 @@ def test_list_overflow_retry(pkg):
         assert np.abs(a - b).max() < 1e-12
 
 
+def test_size_limit_is_an_argument_error(pkg):
+    # positions are addressed through a buffer resource with a 31-bit byte
+    # range: more than 2^26 atoms is refused before anything is allocated
+    n = 2 ** 26 + 16
+    with pytest.raises(pkg._lib.SpecError):
+        pkg.Simulation(n, _spec(pkg, 1000.0, "fp32"))
+
+
 def test_thermostat_moves_temperature(pkg):
     x, v, q, L = _cfg(pkg, 4096, 6, coul=False)
     sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", coul=False, thermostat_every=10,
