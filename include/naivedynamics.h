@@ -175,7 +175,7 @@ int nd_sum_forces(nd_ctx *ctx, float *d_force, const float *d_f1, const float *d
 
 /* ---- simulation (simulate! / simulate_bvh!, Simulator.jl:154-256, 327-379) */
 typedef struct {
-    int32_t n;            /* atoms */
+    int32_t n;            /* atoms (2 .. 2^26) */
     int32_t precision;    /* nd_precision */
     int32_t morton_mode;  /* key for the simulation's LBVH (nd_morton_mode) */
     int32_t atomsperleaf; /* LBVH leaf size (must divide n) */
