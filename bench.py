@@ -257,6 +257,9 @@ def main():
                     help="only the FP32 timed region (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 5 x 256 MiB device-to-device (counter calibration run)")
+    ap.add_argument("--form", choices=("precision", "batched", "adjoint"),
+                    help="only this form, with --form-steps steps (for profiler runs)")
+    ap.add_argument("--form-steps", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -267,6 +270,11 @@ def main():
     pkg = ge.load_package()
     if args.copy_calibration:
         copy_calibration(pkg)
+        return
+    if args.form:
+        fn = {"precision": precision_form, "batched": batched_form, "adjoint": adjoint_form}[args.form]
+        out = fn(pkg, args.form_steps) if args.form_steps > 0 else fn(pkg)
+        print(json.dumps({"form": args.form, "result": out}, default=float), flush=True)
         return
     c = pkg.synth.plain_scale(seed=2)
     n = c["n"]
