@@ -51,6 +51,25 @@ def test_initial_forces_vs_oracle(pkg, O, precision, tol):
     sim.close()
 
 
+@pytest.mark.parametrize("rc,rcc", [(2.5, 2.0), (2.0, 2.5)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-4), ("mixed", 1e-5), ("fp64", 1e-10)])
+def test_unequal_cutoffs_vs_oracle(pkg, O, precision, tol, rc, rcc):
+    # LJ and Coulomb masked separately (the equal-cutoff case uses one mask)
+    x, v, q, L = _cfg(pkg, 16384, 3)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision, rc=rc, rc_coul=rcc))
+    sim.set_state(x, v, q)
+    sim.run(0)
+    _, _, f = sim.get_state()
+    F, U, W, npairs = O.forces64(x, q, O.o64_params(rc_lj=rc, rc_coul=rcc))
+    assert _rel(f, F) < tol, _rel(f, F)
+    e = sim.energies()
+    etol = {"fp32": 1e-4, "mixed": 1e-6, "fp64": 1e-12}[precision]
+    assert abs(e[0, 2] - U) < etol * abs(U)
+    assert abs(e[0, 3] - W) < etol * abs(W)
+    assert abs(int(e[0, 4]) - npairs) <= (0 if precision == "fp64" else 2)
+    sim.close()
+
+
 @pytest.mark.parametrize("precision,xtol", [("fp32", 2e-4), ("mixed", 1e-7), ("fp64", 1e-10)])
 def test_trajectory_vs_oracle(pkg, O, precision, xtol):
     x, v, q, L = _cfg(pkg, 4096, 2)
