@@ -281,7 +281,7 @@ def main():
     log("workload ljc-262144: N=%d L=%.4f" % (n, c["L"]))
 
     # ---- headline: FP32 plain path, interleaved LBVH key ----
-    sim, ms, wall, pairs = gpu_form(pkg, c, "fp32", pkg.MORTON_INTERLEAVE, args.warmup, args.steps)
+    sim, ms, wall, pairs = gpu_form(pkg, c, "fp32", pkg.MORTON_HILBERT, args.warmup, args.steps)
     steps_per_s = args.steps / (ms / 1e3)
     log("fp32: %.3f ms/step (%.1f steps/s), wall %.3f s, %.0f pairs in cutoff" %
         (ms / args.steps, steps_per_s, wall, pairs))
@@ -296,7 +296,7 @@ def main():
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "ljc-262144", "n": n, "density": 0.8, "rc_lj": c["rc"],
                    "rc_coul": c["rc_coul"], "skin": c["skin"], "dt": c["dt"],
-                   "rebuild_every": c["rebuild_every"], "lbvh_key": "interleave30",
+                   "rebuild_every": c["rebuild_every"], "lbvh_key": "hilbert30",
                    "atoms_per_leaf": SIM_ATOMS_PER_LEAF, "precision": "fp32", "walls": True},
         "pair_evals_per_s": round(pairs * steps_per_s, 1),
         "pairs_in_cutoff_per_step": pairs,
@@ -333,7 +333,7 @@ def main():
         extra = {}
         k = min(args.steps, 300)
         for prec in ("mixed", "fp64"):
-            s2, ms2, _, _ = gpu_form(pkg, c, prec, pkg.MORTON_INTERLEAVE, 20, k)
+            s2, ms2, _, _ = gpu_form(pkg, c, prec, pkg.MORTON_HILBERT, 20, k)
             extra[prec + "_steps_per_s"] = round(k / (ms2 / 1e3), 3)
             W = c["rebuild_every"]
             extra[prec + "_force_kernel_ms"] = round(float(np.mean([s2.time_fused_window(W) for _ in range(10)])), 5)
@@ -347,7 +347,7 @@ def main():
         s2.close()
         log("fp32 masked key: %.3f ms/step" % (ms2 / 30))
         # full re-sort every 50 steps, box refits of the same tree in between
-        spec = make_spec(pkg, c, "fp32", pkg.MORTON_INTERLEAVE)
+        spec = make_spec(pkg, c, "fp32", pkg.MORTON_HILBERT)
         spec.resort_every = 50
         s2 = pkg.Simulation(c["n"], spec)
         s2.set_state(c["x"], c["v"], c["q"])

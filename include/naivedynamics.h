@@ -37,7 +37,7 @@ typedef enum {
 } nd_status;
 
 typedef enum { ND_PRED_SCALAR = 0, ND_PRED_SIMD = 1 } nd_predicate;
-typedef enum { ND_MORTON_MASKED = 0, ND_MORTON_INTERLEAVE = 1 } nd_morton_mode;
+typedef enum { ND_MORTON_MASKED = 0, ND_MORTON_INTERLEAVE = 1, ND_MORTON_HILBERT = 2 } nd_morton_mode;
 typedef enum { ND_PREC_FP32 = 0, ND_PREC_MIXED = 1, ND_PREC_FP64 = 2 } nd_precision;
 
 /* GridKey{Float32,Int32} (BVHTraverse.jl:102-109), same 32-byte layout. */
@@ -99,7 +99,10 @@ int nd_bvh_spec_init(nd_bvh_spec *spec, float neighbor_distance, int64_t atom_co
  * bit for bit and fails with ND_ERR_DOMAIN where the reference throws
  * InexactError.  ND_MORTON_INTERLEAVE is this library's 30-bit interleaved key
  * over the box [lo, lo + 1024/scale) (lo3/scale3 may be NULL: the bounding
- * box of the input is used).  d_xyz is N x 3 float (AoS). */
+ * box of the input is used); ND_MORTON_HILBERT is the 30-bit Hilbert index of
+ * the same 10-bit cells (Skilling's transform), the simulation's default key:
+ * its fixed-size leaves are compact runs of space.  d_xyz is N x 3 float
+ * (AoS).  The one-call neighbour list takes the first two modes. */
 int nd_mortoncodes(nd_ctx *ctx, const float *d_xyz, int32_t n, int32_t mode,
                    const float *h_lo3, const float *h_scale3, int32_t *d_codes);
 

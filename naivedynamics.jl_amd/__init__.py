@@ -9,7 +9,7 @@ The directory name is not a Python identifier; load the package with
         submodule_search_locations=[".../naivedynamics.jl_amd"])
 """
 from . import _lib, adjoint, batched, bvh, forces, simulator, synth
-from ._lib import (MORTON_INTERLEAVE, MORTON_MASKED, NODE_DTYPE, PAIR_DTYPE, PRED_SCALAR,
+from ._lib import (MORTON_HILBERT, MORTON_INTERLEAVE, MORTON_MASKED, NODE_DTYPE, PAIR_DTYPE, PRED_SCALAR,
                    PRED_SIMD, PREC_FP32, PREC_FP64, PREC_MIXED, CapacityError, Context,
                    DeviceArray, DomainError, NDError, SpecError, default_context)
 from .bvh import (SpheresBVHSpecs, binwidth, bounding_volume_hierarchy, canonical,

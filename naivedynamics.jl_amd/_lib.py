@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libnaivehip.so")
 
 ND_OK, ND_ERR_ARG, ND_ERR_DOMAIN, ND_ERR_CAPACITY, ND_ERR_HIP, ND_ERR_STATE = 0, -1, -2, -3, -4, -5
 PRED_SCALAR, PRED_SIMD = 0, 1
-MORTON_MASKED, MORTON_INTERLEAVE = 0, 1
+MORTON_MASKED, MORTON_INTERLEAVE, MORTON_HILBERT = 0, 1, 2
 PREC_FP32, PREC_MIXED, PREC_FP64 = 0, 1, 2
 
 NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,)), ("left", "<i4"),

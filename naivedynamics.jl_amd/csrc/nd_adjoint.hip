@@ -399,7 +399,7 @@ size_t state_bytes(const nd_adj *s) { return s->esz * 3 * (size_t)s->p.n; }
 
 template <class T> T *at(nd::DevBuf &b, size_t off_elems = 0) { return b.as<T>() + off_elems; }
 
-// neighbour rows (radius rlist, interleaved key LBVH) at positions xp (device, T)
+// neighbour rows (radius rlist, Hilbert key LBVH) at positions xp (device, T)
 template <class T>
 void build_list(nd_adj *s, const T *xp) {
     nd_ctx *ctx = s->ctx;
@@ -409,8 +409,8 @@ void build_list(nd_adj *s, const T *xp) {
     int32_t *codes = s->codes.as<int32_t>(), *perm = s->perm.as<int32_t>(), *flags = s->flags.as<int32_t>();
     k_adj_to_f32<T><<<nd::grid_for(m), nd::kBlock, 0, st>>>(xp, m, s->xf.as<float>());
     nd::launch_fill_i32(st, flags, 4, 0);
-    nd::launch_morton(ctx, s->xf.as<float>(), n, ND_MORTON_INTERLEAVE, nullptr, nullptr, codes, perm, flags + 2);
-    nd::radix_sort_pairs(st, s->sort, codes, perm, n, 0, 30);  // interleaved keys are < 2^30
+    nd::launch_morton(ctx, s->xf.as<float>(), n, ND_MORTON_HILBERT, nullptr, nullptr, codes, perm, flags + 2);
+    nd::radix_sort_pairs(st, s->sort, codes, perm, n, 0, 30);  // Hilbert keys are < 2^30
     nd::launch_gather_xyz(st, s->xf.as<float>(), perm, n, s->xs.as<float>());
     nd::launch_lbvh(st, s->xs.as<float>(), 3, codes, nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
                     s->store.as<int32_t>(), flags + 1);

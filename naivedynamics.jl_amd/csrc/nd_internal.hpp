@@ -150,6 +150,43 @@ __device__ __forceinline__ uint32_t spread10(uint32_t x) {
     return x;
 }
 
+// 30-bit Hilbert index of a cell (x, y, z), 10 bits per axis, by Skilling's
+// transform (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc.
+// 707, 2004): undo the excess work of the inverse transform from the top bit
+// down, Gray-encode, then interleave the transposed coordinates with x's bit
+// the most significant of each triple.  Consecutive indices are face-adjacent
+// cells, and every 3k-bit prefix is a cube of the level-k octree, so an LBVH
+// over these keys is an octree hierarchy like the interleaved key's but its
+// fixed-size leaves are compact runs.
+__device__ __forceinline__ uint32_t hilbert30(uint32_t x, uint32_t y, uint32_t z) {
+#pragma unroll
+    for (uint32_t q = 1u << 9; q > 1u; q >>= 1) {
+        const uint32_t p = q - 1u;
+        x = (x & q) ? (x ^ p) : x;
+        {
+            const uint32_t t = (x ^ y) & p;
+            const bool inv = (y & q) != 0u;
+            x = inv ? (x ^ p) : (x ^ t);
+            y = inv ? y : (y ^ t);
+        }
+        {
+            const uint32_t t = (x ^ z) & p;
+            const bool inv = (z & q) != 0u;
+            x = inv ? (x ^ p) : (x ^ t);
+            z = inv ? z : (z ^ t);
+        }
+    }
+    y ^= x;
+    z ^= y;
+    uint32_t t = 0u;
+#pragma unroll
+    for (uint32_t q = 1u << 9; q > 1u; q >>= 1) t = (z & q) ? (t ^ (q - 1u)) : t;
+    x ^= t;
+    y ^= t;
+    z ^= t;
+    return (spread10(x) << 2) | (spread10(y) << 1) | spread10(z);
+}
+
 // q = round(Int32, v, RoundDown): defined for -2^31 <= v < 2^31 only.
 __device__ __forceinline__ bool floor_i32(float v, int32_t &q) {
     if (!(v >= -2147483648.0f && v < 2147483648.0f)) return false;

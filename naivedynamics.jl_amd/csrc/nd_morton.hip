@@ -29,6 +29,7 @@ k_morton_masked(const float *__restrict__ xyz, int32_t n, int32_t *__restrict__ 
     if (!ok) atomicOr(status, nd::kStatusDomain);
 }
 
+template <bool HILBERT>
 __global__ void __launch_bounds__(nd::kBlock)
 k_morton_interleave(const float *__restrict__ xyz, int32_t n, const float *__restrict__ box,
                     int32_t *__restrict__ codes, int32_t *__restrict__ idx,
@@ -39,8 +40,9 @@ k_morton_interleave(const float *__restrict__ xyz, int32_t n, const float *__res
     const float y = xyz[3 * (int64_t)i + 1];
     const float z = xyz[3 * (int64_t)i + 2];
     bool ok = __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
-    uint32_t c = spread10(cell10(x, box[0], box[3])) | (spread10(cell10(y, box[1], box[4])) << 1) |
-                 (spread10(cell10(z, box[2], box[5])) << 2);
+    const uint32_t cx = cell10(x, box[0], box[3]), cy = cell10(y, box[1], box[4]),
+                   cz = cell10(z, box[2], box[5]);
+    const uint32_t c = HILBERT ? nd::hilbert30(cx, cy, cz) : (spread10(cx) | (spread10(cy) << 1) | (spread10(cz) << 2));
     codes[i] = (int32_t)c;
     if (idx) idx[i] = i;
     if (!ok) atomicOr(status, nd::kStatusDomain);
@@ -156,8 +158,10 @@ void launch_morton(nd_ctx *ctx, const float *xyz, int32_t n, int32_t mode, const
         } else {
             launch_bbox(ctx, xyz, n, box);
         }
-        k_morton_interleave<<<grid_for(n), kBlock, 0, ctx->stream>>>(xyz, n, box, codes, idx,
-                                                                      status);
+        if (mode == ND_MORTON_HILBERT)
+            k_morton_interleave<true><<<grid_for(n), kBlock, 0, ctx->stream>>>(xyz, n, box, codes, idx, status);
+        else
+            k_morton_interleave<false><<<grid_for(n), kBlock, 0, ctx->stream>>>(xyz, n, box, codes, idx, status);
     }
     ND_HIP(hipGetLastError());
 }
@@ -168,7 +172,7 @@ extern "C" int nd_mortoncodes(nd_ctx *ctx, const float *d_xyz, int32_t n, int32_
                               const float *h_lo3, const float *h_scale3, int32_t *d_codes) {
     ND_API_BEGIN
     ND_REQUIRE(ctx && d_xyz && d_codes && n >= 0, "nd_mortoncodes: bad argument");
-    ND_REQUIRE(mode == ND_MORTON_MASKED || mode == ND_MORTON_INTERLEAVE,
+    ND_REQUIRE(mode == ND_MORTON_MASKED || mode == ND_MORTON_INTERLEAVE || mode == ND_MORTON_HILBERT,
                "nd_mortoncodes: unknown key mode");
     ND_REQUIRE((h_lo3 == nullptr) == (h_scale3 == nullptr),
                "nd_mortoncodes: pass both lo and scale, or neither");

@@ -547,9 +547,11 @@ k_sim_keys(const typename V4<PosS>::T *__restrict__ pos, int32_t n, int32_t mode
         code = (qx & nd::kMagic0) | (qy & nd::kMagic1) | (qz & nd::kMagic2);
     } else {
         ok = __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
-        code = (int32_t)(nd::spread10(nd::cell10(x, lo, scale)) |
-                         (nd::spread10(nd::cell10(y, lo, scale)) << 1) |
-                         (nd::spread10(nd::cell10(z, lo, scale)) << 2));
+        const uint32_t cx = nd::cell10(x, lo, scale), cy = nd::cell10(y, lo, scale),
+                       cz = nd::cell10(z, lo, scale);
+        code = (int32_t)(mode == ND_MORTON_HILBERT
+                             ? nd::hilbert30(cx, cy, cz)
+                             : (nd::spread10(cx) | (nd::spread10(cy) << 1) | (nd::spread10(cz) << 2)));
     }
     codes[i] = code;
     idx[i] = i;
@@ -1464,7 +1466,8 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     ND_REQUIRE(p->atomsperleaf >= 1 && p->n % p->atomsperleaf == 0 && p->n / p->atomsperleaf >= 2,
                "nd_sim_create: atomsperleaf must divide n and give >= 2 leaves");
     ND_REQUIRE(p->precision >= ND_PREC_FP32 && p->precision <= ND_PREC_FP64, "nd_sim_create: precision");
-    ND_REQUIRE(p->morton_mode == ND_MORTON_MASKED || p->morton_mode == ND_MORTON_INTERLEAVE,
+    ND_REQUIRE(p->morton_mode == ND_MORTON_MASKED || p->morton_mode == ND_MORTON_INTERLEAVE ||
+                   p->morton_mode == ND_MORTON_HILBERT,
                "nd_sim_create: key mode");
     ND_REQUIRE(p->rebuild_every >= 1, "nd_sim_create: rebuild_every >= 1");
     ND_REQUIRE(p->resort_every >= 0 && p->resort_every % p->rebuild_every == 0,

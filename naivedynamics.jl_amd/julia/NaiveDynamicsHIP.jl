@@ -139,7 +139,8 @@ end
 function NaiveDynamics.simulate!(b::HIPBackend, sys, spec, clct; dt = 0.001, rc = 2.5, rc_coul = 2.5,
                                  skin = 0.3, rebuild_every = 10, precision = 0, lo = 0.0, hi = 1.0)
     n = length(sys.position)
-    p = Ref(NDSimParams(n, precision, 1, 16, rebuild_every, 1, 10, 0, 1, 0, 0, 0,
+    # key 2: Hilbert index (ND_MORTON_HILBERT), 16 atoms per leaf
+    p = Ref(NDSimParams(n, precision, 2, 16, rebuild_every, 1, 10, 0, 1, 0, 0, 0,
                         1.0, 1.0, rc, 1.0, rc_coul, skin, dt, lo, hi, 1.0, 1.0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(b, ccall((:nd_sim_create, libnaivehip), Cint, (Ptr{Cvoid}, Ref{NDSimParams}, Ref{Ptr{Cvoid}}), b.handle, p, h), "simulate!")

@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (MORTON_INTERLEAVE, PREC_FP32, PREC_FP64, PREC_MIXED, SimParams, SimStats,
+from ._lib import (MORTON_HILBERT, PREC_FP32, PREC_FP64, PREC_MIXED, SimParams, SimStats,
                    check, load)
 
 PRECISIONS = {"fp32": PREC_FP32, "mixed": PREC_MIXED, "fp64": PREC_FP64}
@@ -35,7 +35,7 @@ class SimSpec:
     rebuild_every: int = 10
     atomsperleaf: int = 4
     precision: str = "fp32"
-    morton: int = MORTON_INTERLEAVE
+    morton: int = MORTON_HILBERT
     walls: bool = True
     lo: float = 0.0
     hi: float = 1.0

@@ -18,7 +18,7 @@ NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,)),
 PAIR_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("d", "<f4")])
 
 PRED_SCALAR, PRED_SIMD = 0, 1
-MORTON_MASKED, MORTON_INTERLEAVE = 0, 1
+MORTON_MASKED, MORTON_INTERLEAVE, MORTON_HILBERT = 0, 1, 2
 
 
 class Adj64Params(ctypes.Structure):
@@ -68,6 +68,9 @@ def lib():
         L.ref_binwidth.restype = f32
         L.ref_morton_masked.argtypes = [P, i64, P]
         L.ref_morton_interleave.argtypes = [P, i64, P, P, P]
+        L.ref_morton_hilbert.argtypes = [P, i64, P, P, P]
+        L.ref_hilbert_cells.argtypes = [P, i64, P]
+        L.ref_hilbert_cells.restype = None
         L.ref_sortperm.argtypes = [P, i64, P]
         L.ref_sortperm.restype = None
         L.ref_leaf_boxes.argtypes = [P, i32, i32, f32, P]
@@ -127,6 +130,25 @@ def morton_interleave(xyz, lo, scale):
     lo, scale = _f32(lo), _f32(scale)
     out = np.empty(len(xyz), np.int32)
     rc = lib().ref_morton_interleave(_p(xyz), len(xyz), _p(lo), _p(scale), _p(out))
+    if rc != 0:
+        raise ValueError("oracle: non-finite coordinate")
+    return out
+
+
+def hilbert_cells(cells):
+    """Hilbert index of integer cells (n x 3, each in [0, 1024))."""
+    c = np.ascontiguousarray(cells, np.uint32).reshape(-1, 3)
+    out = np.empty(len(c), np.int32)
+    lib().ref_hilbert_cells(_p(c), len(c), _p(out))
+    return out
+
+
+def morton_hilbert(xyz, lo, scale):
+    """Hilbert key of the interleaved key's 10-bit cells."""
+    xyz = _f32(xyz).reshape(-1, 3)
+    lo, scale = _f32(lo), _f32(scale)
+    out = np.empty(len(xyz), np.int32)
+    rc = lib().ref_morton_hilbert(_p(xyz), len(xyz), _p(lo), _p(scale), _p(out))
     if rc != 0:
         raise ValueError("oracle: non-finite coordinate")
     return out

@@ -90,6 +90,57 @@ int ref_morton_interleave(const float *xyz, int64_t n, const float lo[3],
     return REF_OK;
 }
 
+/* Hilbert index of a 10-bit cell triple: Skilling's transform (J. Skilling,  */
+/* "Programming the Hilbert curve", AIP Conf. Proc. 707, 2004), written as   */
+/* the paper's loops over the axes: undo excess work from the top bit down,  */
+/* Gray encode, then read the transposed coordinates' bits MSB first with    */
+/* axis 0 leading each triple.                                               */
+uint32_t ref_hilbert_cell(uint32_t cx, uint32_t cy, uint32_t cz) {
+    uint32_t X[3] = {cx & 1023u, cy & 1023u, cz & 1023u};
+    const uint32_t M = 1u << 9;
+    for (uint32_t Q = M; Q > 1u; Q >>= 1) {
+        const uint32_t P = Q - 1u;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;                 /* invert */
+            } else {
+                uint32_t t = (X[0] ^ X[i]) & P;  /* exchange */
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1u; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1u;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint32_t key = 0;
+    for (int b = 9; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+    return key;
+}
+
+void ref_hilbert_cells(const uint32_t *cells, int64_t n, int32_t *codes) {
+    for (int64_t i = 0; i < n; ++i)
+        codes[i] = (int32_t)ref_hilbert_cell(cells[3 * i], cells[3 * i + 1], cells[3 * i + 2]);
+}
+
+int ref_morton_hilbert(const float *xyz, int64_t n, const float lo[3], const float scale[3],
+                       int32_t *codes) {
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t c[3];
+        for (int d = 0; d < 3; ++d) {
+            float x = xyz[3 * i + d];
+            if (!isfinite(x)) return REF_ERR_DOMAIN;
+            float t = (x - lo[d]) * scale[d];
+            c[d] = (t <= 0.0f) ? 0u : (t >= 1023.0f ? 1023u : (uint32_t)floorf(t));
+        }
+        codes[i] = (int32_t)ref_hilbert_cell(c[0], c[1], c[2]);
+    }
+    return REF_OK;
+}
+
 /* ------------------------------------------------------------------------- */
 /* A6: sortperm on signed Int32 codes (BVHTraverse.jl:570-573).  Julia's      */
 /* sortperm is stable, so ties keep ascending original index.  Stable LSD    */

@@ -37,6 +37,10 @@ typedef struct {
 
 float ref_binwidth(void);
 int ref_morton_masked(const float *xyz, int64_t n, int32_t *codes);
+uint32_t ref_hilbert_cell(uint32_t cx, uint32_t cy, uint32_t cz);
+void ref_hilbert_cells(const uint32_t *cells, int64_t n, int32_t *codes);
+int ref_morton_hilbert(const float *xyz, int64_t n, const float lo[3], const float scale[3],
+                       int32_t *codes);
 int ref_morton_interleave(const float *xyz, int64_t n, const float lo[3],
                           const float scale[3], int32_t *codes);
 void ref_sortperm(const int32_t *codes, int64_t n, int32_t *perm);

@@ -42,6 +42,18 @@ def test_interleave_keys_bitexact(pkg, O):
     assert np.array_equal(got, O.morton_interleave(x, lo, sc))
 
 
+def test_hilbert_keys_bitexact(pkg, O):
+    x = _rand(70000, 68.94, 41)
+    lo, sc = O.interleave_scale([0, 0, 0], [68.94, 68.94, 68.94])
+    got = pkg.mortoncodes(x, mode=pkg.MORTON_HILBERT, lo=lo, scale=sc)
+    assert np.array_equal(got, O.morton_hilbert(x, lo, sc))
+    # box from the data (no lo/scale): same rule as the interleaved key
+    y = _rand(5000, 3.0, 42)
+    ylo = y.min(axis=0)
+    ysc = (np.float32(1024.0) / (y.max(axis=0) - ylo)).astype(np.float32)
+    assert np.array_equal(pkg.mortoncodes(y, mode=pkg.MORTON_HILBERT), O.morton_hilbert(y, ylo, ysc))
+
+
 @pytest.mark.parametrize("n", [1, 2, 1000, 1024, 1025, 262144, 1000003])
 def test_sortperm_stable(pkg, O, n):
     rng = np.random.default_rng(n)

@@ -89,15 +89,17 @@ def test_trajectory_vs_oracle(pkg, O, precision, xtol):
 
 
 @pytest.mark.parametrize("precision,xtol", [("fp32", 2e-4), ("fp64", 1e-10)])
-@pytest.mark.parametrize("resort_every,list_walk", [(0, 1), (50, 0), (50, 1)])
-def test_refit_windows_and_walks_vs_oracle(pkg, O, precision, xtol, resort_every, list_walk):
+@pytest.mark.parametrize("resort_every,list_walk,key", [(0, 1, 2), (50, 0, 2), (50, 1, 2), (0, 0, 1),
+                                                        (50, 0, 1)])
+def test_refit_windows_and_walks_vs_oracle(pkg, O, precision, xtol, resort_every, list_walk, key):
     # resort_every = 50 with rebuild_every = 10: rebuilds at steps 10..40 and
     # 60..90 refit the boxes of the tree built at 0 and 50 (atom order and
     # topology kept); the rope walk and the child-box walk must both give
     # every pair inside the list radius
     x, v, q, L = _cfg(pkg, 4096, 7)
+    # key 1: interleaved (Morton) cells, 2: Hilbert index of the same cells
     sim = pkg.Simulation(len(x), _spec(pkg, L, precision, steps=100, resort_every=resort_every,
-                                       list_walk=list_walk))
+                                       list_walk=list_walk, morton=key))
     sim.set_state(x, v, q)
     sim.run(100)
     xg, vg, fg = sim.get_state()
@@ -115,17 +117,20 @@ def test_refit_windows_and_walks_vs_oracle(pkg, O, precision, xtol, resort_every
 @pytest.mark.parametrize("apl", [2, 4, 8, 16, 32])
 def test_walks_give_the_same_list(pkg, apl):
     # 0: two-phase child-box walk (leaf sizes 4..32; others use walk 2),
-    # 1: rope walk, 2: one-node child-box walk
+    # 1: rope walk, 2: one-node child-box walk; over both tree keys (the list
+    # is the set of pairs within the list radius, whatever the atom order)
     x, v, q, L = _cfg(pkg, 16384, 8)
     counts = []
-    for walk in (0, 1, 2):
-        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", list_walk=walk, atomsperleaf=apl))
-        sim.set_state(x, v, q)
-        sim.run(0)
-        st = sim.stats()
-        counts.append((st["list_entries"], st["max_neighbors"]))
-        sim.close()
-    assert counts[0] == counts[1] == counts[2] and counts[0][0] > 0
+    for key in (1, 2):
+        for walk in (0, 1, 2):
+            sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", list_walk=walk, atomsperleaf=apl,
+                                               morton=key))
+            sim.set_state(x, v, q)
+            sim.run(0)
+            st = sim.stats()
+            counts.append((st["list_entries"], st["max_neighbors"]))
+            sim.close()
+    assert len(set(counts)) == 1 and counts[0][0] > 0
 
 
 @pytest.mark.parametrize("resort_every", [0, 20])

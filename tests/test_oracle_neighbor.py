@@ -233,3 +233,57 @@ def test_margin_predicates_differ(O):
 def test_canonical_rows_helper():
     a = np.array([[3, 1, 7], [0, 2, 5]])
     assert rows_canonical(a).tolist() == [[0, 2, 5], [1, 3, 7]]
+
+
+# ---- Hilbert key (the simulation's key): pinned by the curve's defining
+# properties, not by a restatement of the transform ----
+
+def _grid(k):
+    """All cells of a 2^k grid, scaled to 10-bit cell coordinates."""
+    g = np.arange(2 ** k)
+    c = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    return c << (10 - k)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_hilbert_full_grid_is_a_face_adjacent_path(O, k):
+    c = _grid(k)
+    h = O.hilbert_cells(c).astype(np.int64)
+    # on a 2^k grid the top 3k bits of the index run through 0 .. 8^k - 1 once
+    top = h >> (3 * (10 - k))
+    assert np.array_equal(np.sort(top), np.arange(8 ** k))
+    # consecutive cells along the curve share a face (one unit step)
+    path = c[np.argsort(h)] >> (10 - k)
+    steps = np.abs(np.diff(path, axis=0)).sum(1)
+    assert np.all(steps == 1)
+
+
+def test_hilbert_prefixes_are_octree_cubes(O):
+    # cells with the same 3k-bit prefix lie in one cube of side 2^(10-k)
+    rng = np.random.default_rng(8)
+    c = rng.integers(0, 1024, (50000, 3))
+    h = O.hilbert_cells(c).astype(np.int64)
+    for k in (1, 3, 5):
+        pre = h >> (3 * (10 - k))
+        cube = c >> (10 - k)
+        for p in np.unique(pre)[:200]:
+            m = pre == p
+            assert (cube[m] == cube[m][0]).all()
+
+
+def test_hilbert_is_a_bijection_and_starts_at_the_origin(O):
+    rng = np.random.default_rng(9)
+    c = np.unique(rng.integers(0, 1024, (200000, 3)), axis=0)
+    h = O.hilbert_cells(c)
+    assert len(np.unique(h)) == len(c)
+    assert O.hilbert_cells(np.zeros((1, 3), np.int64))[0] == 0
+    assert np.all((h >= 0) & (h < 2 ** 30))
+
+
+def test_morton_hilbert_uses_the_interleaved_cells(O):
+    rng = np.random.default_rng(10)
+    x = (rng.random((30000, 3)) * 68.94).astype(np.float32)
+    lo, sc = O.interleave_scale([0, 0, 0], [68.94, 68.94, 68.94])
+    t = (x - lo) * sc
+    cells = np.where(t <= 0, 0, np.where(t >= 1023, 1023, np.floor(t))).astype(np.int64)
+    assert np.array_equal(O.morton_hilbert(x, lo, sc), O.hilbert_cells(cells))
