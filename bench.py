@@ -44,7 +44,10 @@ def log(msg):
 
 # Leaf size of the simulation's internal LBVH (a tuning parameter of this
 # implementation; the reference neighbour-list entry points keep their own).
-SIM_ATOMS_PER_LEAF = 16
+# LBVH leaf size per workload, from sweeps over 8 / 16 / 32 with the Hilbert
+# key: 32 on the uniform liquid (ljc-262144), 16 on the dense blobs
+SIM_ATOMS_PER_LEAF = 32
+BLOB_ATOMS_PER_LEAF = 16
 
 
 def make_spec(pkg, c, precision, key, energy_every=10, use_graph=True):
@@ -115,7 +118,7 @@ def precision_form(pkg, steps=500):
     for prec in ("fp64", "mixed", "fp32"):
         spec = pkg.SimSpec(duration=0, stepwidth=c["dt"], rc=c["rc"], rc_coul=c["rc_coul"], skin=c["skin"],
                            rebuild_every=c["rebuild_every"], lo=0.0, hi=c["L"], precision=prec,
-                           energy_every=10, walls=True, atomsperleaf=SIM_ATOMS_PER_LEAF)
+                           energy_every=10, walls=True, atomsperleaf=BLOB_ATOMS_PER_LEAF)
         sim = pkg.Simulation(n, spec)
         sim.set_state(c["x"], c["v"], c["q"])
         sim.run(0)
