@@ -48,34 +48,17 @@ __device__ __forceinline__ int32_t cas_relaxed(int32_t *p, int32_t desired) {
 }
 
 __global__ void __launch_bounds__(nd::kBlock)
-k_lbvh_topology(const float *__restrict__ xyz, int32_t stride, const int32_t *__restrict__ codes,
-                int32_t nL, int32_t apl, float h, nd_node *__restrict__ nodes,
+k_lbvh_topology(const int32_t *__restrict__ codes, int32_t nL, int32_t apl, nd_node *__restrict__ nodes,
                 int32_t *__restrict__ store, int2 *__restrict__ ranges,
                 int32_t *__restrict__ status) {
     int32_t i = 1 + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i > nL) return;
 
-    // ---- leaf box: min(x - r/2), max(x + r/2) over the leaf's atoms ----
-    float mn[3], mx[3];
-    const float *p0 = xyz + (int64_t)stride * (int64_t)(i - 1) * apl;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) { mn[d] = p0[d] - h; mx[d] = p0[d] + h; }
-    for (int32_t a = 0; a < apl; ++a) {
-        const float *p = p0 + (int64_t)stride * a;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            mn[d] = jl_min(mn[d], p[d] - h);
-            mx[d] = jl_max(mx[d], p[d] + h);
-        }
-    }
+    // the leaf's box is written by launch_leaf_boxes; here its left (0) and rope
     int32_t rangel = i, ranger = i;
     int32_t dell = delta(rangel - 1, codes, nL, apl);
     int32_t delr = delta(ranger, codes, nL, apl);
-    {
-        float4 *q = reinterpret_cast<float4 *>(nodes + (i - 1));
-        q[0] = make_float4(mn[0], mn[1], mn[2], mx[0]);
-        q[1] = make_float4(mx[1], mx[2], __int_as_float(0), __int_as_float(rope(i, delr, codes, nL, apl)));
-    }
+    *reinterpret_cast<int2 *>(&nodes[i - 1].left) = make_int2(0, rope(i, delr, codes, nL, apl));
 
     // Every climb step strictly widens [rangel, ranger], so a walker takes at
     // most nL steps; the bound and the index checks only matter for codes
@@ -225,8 +208,42 @@ k_lbvh_refit(nd_node *__restrict__ nodes, const int2 *__restrict__ ranges,
     }
 }
 
-// Leaf boxes only (the tree topology is kept): box of each leaf's atoms
-// +- r/2, written into the leaf node without touching left/skip.
+// Leaf boxes: box of each leaf's atoms +- r/2 (leafcluster_primitives,
+// BVHTraverse.jl:376-410), written into the leaf node without touching
+// left/skip.  One lane per atom and a butterfly over the leaf's SEG lanes
+// (SEG = atoms per leaf, a power of two up to 64: each leaf's lanes are an
+// aligned segment of one wave); Julia's min/max are associative and
+// commutative on non-NaN floats, so the box is the serial loop's bit for bit.
+template <int SEG>
+__global__ void __launch_bounds__(nd::kBlock)
+k_leaf_boxes_seg(const float *__restrict__ xyz, int32_t stride, int32_t natoms, float h,
+                 nd_node *__restrict__ nodes) {
+    const int32_t a = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool act = a < natoms;
+    const float inf = __int_as_float(0x7F800000);
+    float mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;
+    if (act) {
+        const float *p = xyz + (int64_t)stride * a;
+        mn0 = p[0] - h; mn1 = p[1] - h; mn2 = p[2] - h;
+        mx0 = p[0] + h; mx1 = p[1] + h; mx2 = p[2] + h;
+    }
+#pragma unroll
+    for (int off = 1; off < SEG; off <<= 1) {
+        mn0 = jl_min(mn0, __shfl_xor(mn0, off, 64));
+        mn1 = jl_min(mn1, __shfl_xor(mn1, off, 64));
+        mn2 = jl_min(mn2, __shfl_xor(mn2, off, 64));
+        mx0 = jl_max(mx0, __shfl_xor(mx0, off, 64));
+        mx1 = jl_max(mx1, __shfl_xor(mx1, off, 64));
+        mx2 = jl_max(mx2, __shfl_xor(mx2, off, 64));
+    }
+    if (act && (a & (SEG - 1)) == 0) {
+        float *q = reinterpret_cast<float *>(nodes + a / SEG);
+        *reinterpret_cast<float4 *>(q) = make_float4(mn0, mn1, mn2, mx0);
+        *reinterpret_cast<float2 *>(q + 4) = make_float2(mx1, mx2);
+    }
+}
+
+// Leaf boxes for any leaf size: one lane per leaf, serial over its atoms.
 __global__ void __launch_bounds__(nd::kBlock)
 k_leaf_boxes(const float *__restrict__ xyz, int32_t stride, int32_t nL, int32_t apl, float h,
              nd_node *__restrict__ nodes) {
@@ -328,6 +345,22 @@ void launch_lbvh_wide4(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *
 }
 
 // Box-only update of an existing tree (same leaf order, same topology):
+// Leaf boxes: the one-lane-per-atom butterfly for power-of-two leaf sizes up
+// to 64, the serial loop otherwise.
+void launch_leaf_boxes(hipStream_t s, const float *xyz, int32_t stride, int32_t nL, int32_t apl,
+                       float h, nd_node *nodes) {
+    const int32_t na = nL * apl;
+    switch (apl) {
+        case 2: k_leaf_boxes_seg<2><<<grid_for(na), kBlock, 0, s>>>(xyz, stride, na, h, nodes); break;
+        case 4: k_leaf_boxes_seg<4><<<grid_for(na), kBlock, 0, s>>>(xyz, stride, na, h, nodes); break;
+        case 8: k_leaf_boxes_seg<8><<<grid_for(na), kBlock, 0, s>>>(xyz, stride, na, h, nodes); break;
+        case 16: k_leaf_boxes_seg<16><<<grid_for(na), kBlock, 0, s>>>(xyz, stride, na, h, nodes); break;
+        case 32: k_leaf_boxes_seg<32><<<grid_for(na), kBlock, 0, s>>>(xyz, stride, na, h, nodes); break;
+        case 64: k_leaf_boxes_seg<64><<<grid_for(na), kBlock, 0, s>>>(xyz, stride, na, h, nodes); break;
+        default: k_leaf_boxes<<<grid_for(nL), kBlock, 0, s>>>(xyz, stride, nL, apl, h, nodes);
+    }
+}
+
 // leaf boxes from the current positions, chunk boxes, internal boxes.
 void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride, int32_t nL,
                        int32_t apl, float r, nd_node *nodes, int32_t *store) {
@@ -335,7 +368,7 @@ void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_strid
     const size_t off_c = (off_r + 2 * (size_t)nL + 3) & ~(size_t)3;
     const int2 *ranges = reinterpret_cast<const int2 *>(store + off_r);
     float4 *chunks = reinterpret_cast<float4 *>(store + off_c);
-    k_leaf_boxes<<<grid_for(nL), kBlock, 0, s>>>(xyz_sorted, xyz_stride, nL, apl, r / 2.0f, nodes);
+    launch_leaf_boxes(s, xyz_sorted, xyz_stride, nL, apl, r / 2.0f, nodes);
     const int64_t nchunks = (nL + kChunk - 1) / kChunk;
     k_lbvh_chunks<<<grid_for(nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);
     k_lbvh_refit<<<grid_for((int64_t)(nL - 1) * kGroup), kBlock, 0, s>>>(nodes, ranges, chunks, nL, 0);
@@ -358,8 +391,8 @@ void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
     int2 *ranges = reinterpret_cast<int2 *>(store + off_r);
     float4 *chunks = reinterpret_cast<float4 *>(store + off_c);
     const float h = r / 2.0f;  // neighbor_distance/2 in Float32
-    k_lbvh_topology<<<grid_for(nL), kBlock, 0, s>>>(xyz_sorted, xyz_stride, codes_sorted, nL, apl, h,
-                                                   nodes, store, ranges, status);
+    launch_leaf_boxes(s, xyz_sorted, xyz_stride, nL, apl, h, nodes);
+    k_lbvh_topology<<<grid_for(nL), kBlock, 0, s>>>(codes_sorted, nL, apl, nodes, store, ranges, status);
     const int64_t nchunks = (nL + kChunk - 1) / kChunk;
     k_lbvh_chunks<<<grid_for(nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);
     k_lbvh_refit<<<grid_for((int64_t)(nL - 1) * kGroup), kBlock, 0, s>>>(nodes, ranges, chunks, nL,
