@@ -96,10 +96,11 @@ void launch_gather_xyz(hipStream_t s, const float *xyz, const int32_t *perm, int
 // key_bits = 32 for a full sort).  Results land in keys/vals.
 void radix_sort_pairs(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
                       int32_t first_bit, int32_t end_bit);
-// Keys below 2^30 in three 10-bit passes; the sorted pairs are left in
-// *keys_sorted / *vals_sorted (b's alternate buffers, keys/vals for n <= 1).
-void radix_sort_keys30(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
-                       int32_t **keys_sorted, int32_t **vals_sorted);
+// Keys below 2^key_bits in 10-bit passes; the sorted pairs are left in
+// *keys_sorted / *vals_sorted (keys/vals after an even number of passes,
+// b's alternate buffers after an odd one).
+void radix_sort_keys10(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
+                       int32_t key_bits, int32_t **keys_sorted, int32_t **vals_sorted);
 size_t radix_scratch_blocks(int32_t n);
 
 // nd_lbvh.hip

@@ -530,7 +530,7 @@ k_energy_finalize(const double *__restrict__ partials, int32_t nblocks, double *
 template <class PosS>
 __global__ void __launch_bounds__(nd::kBlock)
 k_sim_keys(const typename V4<PosS>::T *__restrict__ pos, int32_t n, int32_t mode, float lo,
-           float scale, int32_t *__restrict__ codes, int32_t *__restrict__ idx,
+           float scale, int32_t drop, int32_t *__restrict__ codes, int32_t *__restrict__ idx,
            SimMisc *__restrict__ misc) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) { misc->maxcnt = 0; }
@@ -549,9 +549,10 @@ k_sim_keys(const typename V4<PosS>::T *__restrict__ pos, int32_t n, int32_t mode
         ok = __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
         const uint32_t cx = nd::cell10(x, lo, scale), cy = nd::cell10(y, lo, scale),
                        cz = nd::cell10(z, lo, scale);
-        code = (int32_t)(mode == ND_MORTON_HILBERT
-                             ? nd::hilbert30(cx, cy, cz)
-                             : (nd::spread10(cx) | (nd::spread10(cy) << 1) | (nd::spread10(cz) << 2)));
+        code = (int32_t)((mode == ND_MORTON_HILBERT
+                              ? nd::hilbert30(cx, cy, cz)
+                              : (nd::spread10(cx) | (nd::spread10(cy) << 1) | (nd::spread10(cz) << 2))) >>
+                         drop);
     }
     codes[i] = code;
     idx[i] = i;
@@ -1144,14 +1145,20 @@ void launch_sort_and_build(nd_sim *s) {
     const float lo = s->p.lo;
     const float scale = s->p.hi > s->p.lo ? 1024.0f / (s->p.hi - s->p.lo) : 0.0f;
     int32_t *codes = s->codes.as<int32_t>(), *perm = s->perm.as<int32_t>();
+    // Key width: with at most 2^20 atoms the top 20 bits of the interleaved or
+    // Hilbert key (cells of 1/101 of the box side, about one atom each at the
+    // densities here) order the atoms as well as all 30 do for the tree and
+    // the walk, and take two radix passes instead of three; the tree is built
+    // on the same truncated keys.  The masked key keeps its 30 bits.
+    const int32_t key_bits = (s->p.morton_mode != ND_MORTON_MASKED && n <= (1 << 20)) ? 20 : 30;
     if (s->wide)
         k_sim_keys<double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), n, s->p.morton_mode,
-                                                     lo, scale, codes, perm, misc);
+                                                     lo, scale, 30 - key_bits, codes, perm, misc);
     else
         k_sim_keys<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), n, s->p.morton_mode,
-                                                    lo, scale, codes, perm, misc);
-    // Both keys are < 2^30, so bits 30..31 of the sign-flipped key are constant.
-    nd::radix_sort_keys30(st, s->sort, codes, perm, n, &codes, &perm);
+                                                    lo, scale, 30 - key_bits, codes, perm, misc);
+    // keys are < 2^key_bits, so the sign-flipped key's higher bits are constant
+    nd::radix_sort_keys10(st, s->sort, codes, perm, n, key_bits, &codes, &perm);
     const int a = s->cur ^ 1, va = s->vcur ^ 1;
     if (s->wide)
         k_permute<double><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),

@@ -271,14 +271,16 @@ void radix_sort_pairs(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, 
     }
 }
 
-void radix_sort_keys30(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
-                       int32_t **keys_sorted, int32_t **vals_sorted) {
+void radix_sort_keys10(hipStream_t s, SortBufs &b, int32_t *keys, int32_t *vals, int32_t n,
+                       int32_t key_bits, int32_t **keys_sorted, int32_t **vals_sorted) {
     *keys_sorted = keys;
     *vals_sorted = vals;
     if (n <= 1) return;
-    radix_passes<10>(s, b, keys, vals, n, 0, 30);  // three passes
-    *keys_sorted = b.keys_alt.as<int32_t>();
-    *vals_sorted = b.vals_alt.as<int32_t>();
+    const int passes = radix_passes<10>(s, b, keys, vals, n, 0, key_bits);
+    if (passes & 1) {
+        *keys_sorted = b.keys_alt.as<int32_t>();
+        *vals_sorted = b.vals_alt.as<int32_t>();
+    }
 }
 
 // Sorts keys/vals in place over only the 8-bit digit windows whose bits vary
