@@ -115,9 +115,10 @@ void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_strid
 void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *wide);
 // Four-wide records of the internal nodes (8 float4 per internal node).
 void launch_lbvh_wide4(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *rec);
+// slots_cleared: the caller has already zeroed store[0 .. nL-2] (the CAS slots).
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
-                 nd_node *nodes, int32_t *store, int32_t *status);
+                 nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared = false);
 
 // nd_traverse.hip
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,

@@ -382,10 +382,10 @@ void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *w
 
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
-                 nd_node *nodes, int32_t *store, int32_t *status) {
+                 nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared) {
     // store (lbvh_scratch_ints(nL) ints): nL-1 CAS slots | nL-1 int2 leaf
     // ranges | one 32-byte box per 64-leaf chunk
-    launch_fill_i32(s, store, nL - 1, 0);
+    if (!slots_cleared) launch_fill_i32(s, store, nL - 1, 0);
     const size_t off_r = ((size_t)nL + 3) & ~(size_t)3;
     const size_t off_c = (off_r + 2 * (size_t)nL + 3) & ~(size_t)3;
     int2 *ranges = reinterpret_cast<int2 *>(store + off_r);

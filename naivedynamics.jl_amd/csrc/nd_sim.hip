@@ -531,9 +531,10 @@ template <class PosS>
 __global__ void __launch_bounds__(nd::kBlock)
 k_sim_keys(const typename V4<PosS>::T *__restrict__ pos, int32_t n, int32_t mode, float lo,
            float scale, int32_t drop, int32_t *__restrict__ codes, int32_t *__restrict__ idx,
-           SimMisc *__restrict__ misc) {
+           SimMisc *__restrict__ misc, int32_t *__restrict__ slots, int32_t nslots) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) { misc->maxcnt = 0; }
+    if (i < nslots) slots[i] = 0;  // the tree build's CAS slots (nslots < n)
     if (i >= n) return;
     const auto p = pos[i];
     const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
@@ -1153,10 +1154,12 @@ void launch_sort_and_build(nd_sim *s) {
     const int32_t key_bits = (s->p.morton_mode != ND_MORTON_MASKED && n <= (1 << 20)) ? 20 : 30;
     if (s->wide)
         k_sim_keys<double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), n, s->p.morton_mode,
-                                                     lo, scale, 30 - key_bits, codes, perm, misc);
+                                                     lo, scale, 30 - key_bits, codes, perm, misc,
+                                                     s->store.as<int32_t>(), s->nL - 1);
     else
         k_sim_keys<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), n, s->p.morton_mode,
-                                                    lo, scale, 30 - key_bits, codes, perm, misc);
+                                                    lo, scale, 30 - key_bits, codes, perm, misc,
+                                                    s->store.as<int32_t>(), s->nL - 1);
     // keys are < 2^key_bits, so the sign-flipped key's higher bits are constant
     nd::radix_sort_keys10(st, s->sort, codes, perm, n, key_bits, &codes, &perm);
     const int a = s->cur ^ 1, va = s->vcur ^ 1;
@@ -1175,7 +1178,7 @@ void launch_sort_and_build(nd_sim *s) {
     const float *p32 = s->wide ? reinterpret_cast<const float *>(s->pos32.p)
                                : reinterpret_cast<const float *>(s->pos[s->cur].p);
     nd::launch_lbvh(st, p32, 4, codes, s->nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
-                    s->store.as<int32_t>(), &misc->status);
+                    s->store.as<int32_t>(), &misc->status, true);
     ND_HIP(hipGetLastError());
 }
 
