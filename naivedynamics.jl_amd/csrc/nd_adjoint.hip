@@ -167,19 +167,25 @@ k_adj_nlist(const float *__restrict__ xs, const int32_t *__restrict__ perm,
     if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&flags[0], mx);
 }
 
+// Rows are split over kRowLanes consecutive lanes of a wave (lane s of an
+// atom takes entries s, s + kRowLanes, ...): at N = 32,768 one lane per atom
+// leaves 2 waves per CU and every row a long chain of dependent loads.
+constexpr int kRowLanes = 4;
+
 // F_i = sum_j s(r_ij) r_ij over row i, summed in 2^-40 fixed point: the sum
-// is the same whatever the order of the row.
+// is the same whatever the order of the row (and however it is split).
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
 k_adj_force(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
             const T *__restrict__ x, const double *__restrict__ q, AdjP P, unsigned long long *__restrict__ F) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int32_t i = (int32_t)(t / kRowLanes), sub = (int32_t)(t % kRowLanes);
+    long long fx = 0, fy = 0, fz = 0;
+    if (i < P.n) {
     const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
     const T xi = x[3 * (int64_t)i], yi = x[3 * (int64_t)i + 1], zi = x[3 * (int64_t)i + 2];
     const double qi = q[i];
-    long long fx = 0, fy = 0, fz = 0;
-    for (int64_t e = e0; e < e1; ++e) {
+    for (int64_t e = e0 + sub; e < e1; e += kRowLanes) {
         const int32_t j = nbr[e];
         const T dx = xi - x[3 * (int64_t)j], dy = yi - x[3 * (int64_t)j + 1], dz = zi - x[3 * (int64_t)j + 2];
         const T r2 = dx * dx + dy * dy + dz * dz;
@@ -189,9 +195,19 @@ k_adj_force(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restr
         fy += llrint((double)(s * dy) * kFix);
         fz += llrint((double)(s * dz) * kFix);
     }
-    F[3 * (int64_t)i] = (unsigned long long)fx;
-    F[3 * (int64_t)i + 1] = (unsigned long long)fy;
-    F[3 * (int64_t)i + 2] = (unsigned long long)fz;
+    }
+    // exact integer sums over the atom's lanes (all lanes take part)
+#pragma unroll
+    for (int off = 1; off < kRowLanes; off <<= 1) {
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+        fz += __shfl_xor(fz, off, 64);
+    }
+    if (i < P.n && sub == 0) {
+        F[3 * (int64_t)i] = (unsigned long long)fx;
+        F[3 * (int64_t)i + 1] = (unsigned long long)fy;
+        F[3 * (int64_t)i + 2] = (unsigned long long)fz;
+    }
 }
 
 __device__ __forceinline__ double fix_val(unsigned long long v) { return (double)(long long)v * kInvFix; }
@@ -321,16 +337,17 @@ k_adj_hvp(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restric
           const T *__restrict__ x, const double *__restrict__ q, AdjP P, const double *__restrict__ a,
           double *__restrict__ lam, double *__restrict__ dq, double *__restrict__ dpar) {
     __shared__ double sh[2][nd::kBlock / 64];
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int32_t i = (int32_t)(t / kRowLanes), sub = (int32_t)(t % kRowLanes);
     double ge = 0.0, gs = 0.0;
+    double mx = 0.0, my = 0.0, mz = 0.0, gq = 0.0;
     if (i < P.n) {
         const double xi = (double)x[3 * (int64_t)i], yi = (double)x[3 * (int64_t)i + 1],
                      zi = (double)x[3 * (int64_t)i + 2];
         const double ai = a[3 * (int64_t)i], bi = a[3 * (int64_t)i + 1], ci = a[3 * (int64_t)i + 2];
         const double qi = q[i];
-        double mx = 0.0, my = 0.0, mz = 0.0, gq = 0.0;
         const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
-        for (int64_t e = e0; e < e1; ++e) {
+        for (int64_t e = e0 + sub; e < e1; e += kRowLanes) {
             const int32_t j = nbr[e];
             const double dx = xi - (double)x[3 * (int64_t)j], dy = yi - (double)x[3 * (int64_t)j + 1],
                          dz = zi - (double)x[3 * (int64_t)j + 2];
@@ -347,6 +364,16 @@ k_adj_hvp(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restric
             ge += 0.5 * d.s_eps * rda;
             gs += 0.5 * d.s_sig * rda;
         }
+    }
+    // the atom's lanes combine their partial sums in a fixed order
+#pragma unroll
+    for (int off = 1; off < kRowLanes; off <<= 1) {
+        mx += __shfl_xor(mx, off, 64);
+        my += __shfl_xor(my, off, 64);
+        mz += __shfl_xor(mz, off, 64);
+        gq += __shfl_xor(gq, off, 64);
+    }
+    if (i < P.n && sub == 0) {
         lam[3 * (int64_t)i] += mx;
         lam[3 * (int64_t)i + 1] += my;
         lam[3 * (int64_t)i + 2] += mz;
@@ -438,7 +465,7 @@ void compute_force(nd_adj *s, const T *xs) {
     hipStream_t st = s->ctx->stream;
     const int64_t m = 3 * (int64_t)s->p.n;
     (void)m;
-    k_adj_force<T><<<nd::grid_for(s->p.n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+    k_adj_force<T><<<nd::grid_for((int64_t)s->p.n * kRowLanes), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
                                                                  xs, s->q.as<double>(), s->P,
                                                                  s->F.as<unsigned long long>());
     ND_HIP(hipGetLastError());
@@ -530,7 +557,7 @@ void backward_impl(nd_adj *s) {
             }
             k_adj_vec<<<nd::grid_for(m), nd::kBlock, 0, st>>>(lx, lv, a, m, s->P.dt,
                                                                 k == 0 ? s->P.c : 2.0 * s->P.c);
-            k_adj_hvp<T><<<nd::grid_for(n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+            k_adj_hvp<T><<<nd::grid_for((int64_t)n * kRowLanes), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
                                                                   xk, s->q.as<double>(), s->P, a, lx,
                                                                   s->dq.as<double>(), s->dpar.as<double>());
             ND_HIP(hipGetLastError());
