@@ -267,7 +267,7 @@ int nd_sim_record_positions(nd_sim *sim, void *d_out, int32_t as_double);
  * rdf_rmax / nbins, g_b = V / (n(n-1)/2 4 pi r_b^2 dr) sum_{i<j} K(r_ij - r_b),
  * K(d) = (1 - (d/h)^2)^3 for |d| < h = rdf_h, V = volume. */
 typedef struct {
-    int32_t n;               /* atoms (even, >= 4) */
+    int32_t n;               /* atoms (a multiple of 4, >= 8) */
     int32_t precision;       /* ND_PREC_FP32 or ND_PREC_FP64 (state and forward pair maths) */
     int32_t nsteps;
     int32_t checkpoint_every;

@@ -110,61 +110,38 @@ __device__ __forceinline__ PairD pair_d(double r2, double qi, double qj, const A
     return o;
 }
 
-// Neighbour rows from the LBVH: one wave per 64 consecutive sorted atoms walks
-// the tree from the root with ropes as a unit (the node is wave-uniform; each
-// lane tests its own box, the wave descends where any lane overlaps), and
-// each lane appends every sorted slot t != s within the list radius to the
-// row of its original atom, as original indices: row i = nbr[i cap ..
-// i cap + cnt[i]).  flags[0] receives the longest row, flags[1] a tree error.
+// Sorted FP32 positions (w = 0) for the list walk.
 __global__ void __launch_bounds__(nd::kBlock)
-k_adj_nlist(const float *__restrict__ xs, const int32_t *__restrict__ perm,
-            const nd_node *__restrict__ nodes, int32_t n, int32_t nL, int32_t apl, float h, float rl2,
-            int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ cnt,
-            int32_t *__restrict__ flags) {
+k_adj_pos4(const float *__restrict__ xf, const int32_t *__restrict__ perm, int32_t n,
+           float4 *__restrict__ out) {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = s < n;
-    const float px = active ? xs[3 * (int64_t)s] : 0.0f, py = active ? xs[3 * (int64_t)s + 1] : 0.0f,
-                pz = active ? xs[3 * (int64_t)s + 2] : 0.0f;
-    const int32_t row = active ? perm[s] : 0;
-    int32_t *out = nbr + (int64_t)row * cap;
-    int32_t c = 0;
-    int32_t t = nL + 1;  // root
-    int32_t visits = 0;
-    while (t != 0) {
-        t = __builtin_amdgcn_readfirstlane(t);
-        if (++visits > 2 * nL || t < 1 || t > 2 * nL - 1) {
-            if ((threadIdx.x & 63) == 0) atomicOr(&flags[1], 1);
-            break;
-        }
-        const float4 *np4 = reinterpret_cast<const float4 *>(nodes + (t - 1));
-        const float4 a = np4[0], b = np4[1];
-        const bool ov = active & (px - h < a.w) & (px + h > a.x) & (py - h < b.x) & (py + h > a.y) &
-                        (pz - h < b.y) & (pz + h > a.z);
-        const int32_t left = __float_as_int(b.z), skip = __float_as_int(b.w);
-        if (__ballot(ov) == 0ull) {
-            t = skip;
-        } else if (left != 0) {
-            t = left;
-        } else {
-            if (ov) {
-                const int32_t base = (t - 1) * apl;
-                for (int32_t m = 0; m < apl; ++m) {
-                    const int32_t u = base + m;
-                    const float dx = px - xs[3 * (int64_t)u], dy = py - xs[3 * (int64_t)u + 1],
-                                dz = pz - xs[3 * (int64_t)u + 2];
-                    if (u != s && dx * dx + dy * dy + dz * dz < rl2) {
-                        if (c < cap) out[c] = perm[u];
-                        ++c;
-                    }
-                }
-            }
-            t = skip;
-        }
+    if (s >= n) return;
+    const int64_t o = 3 * (int64_t)perm[s];
+    out[s] = make_float4(xf[o], xf[o + 1], xf[o + 2], 0.0f);
+}
+
+// Neighbour rows in the list walk's layout: sorted slot s (atom perm[s])
+// keeps its entries in column s & 63 of the block of its 64 slots (stride
+// 64), as original atom ids once k_adj_map_ids has run; lcnt[s] entries.
+struct Rows {
+    const int32_t *lists, *lcnt, *perm;
+    int32_t cap;
+    __device__ __forceinline__ const int32_t *col(int32_t s) const {
+        return lists + ((int64_t)(s >> 6) * cap) * 64 + (s & 63);
     }
-    if (active) cnt[row] = c < cap ? c : cap;
-    int32_t mx = c;
-    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_down(mx, o, 64));
-    if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&flags[0], mx);
+    __device__ __forceinline__ int32_t count(int32_t s) const { return lcnt[s] < cap ? lcnt[s] : cap; }
+};
+
+// The walk writes sorted indices; the adjoint's kernels want original ids.
+// One lane per slot and list row (blockIdx.y = row): no per-lane loop.
+__global__ void __launch_bounds__(nd::kBlock)
+k_adj_map_ids(const int32_t *__restrict__ perm, int32_t n, int32_t cap, int32_t *__restrict__ lists,
+              const int32_t *__restrict__ lcnt) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t k = blockIdx.y;
+    if (s >= n || k >= (lcnt[s] < cap ? lcnt[s] : cap)) return;
+    int32_t *e = lists + ((int64_t)(s >> 6) * cap + k) * 64 + (s & 63);
+    *e = perm[*e];
 }
 
 // Rows are split over kRowLanes consecutive lanes of a wave (lane s of an
@@ -176,17 +153,19 @@ constexpr int kRowLanes = 4;
 // is the same whatever the order of the row (and however it is split).
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_adj_force(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
-            const T *__restrict__ x, const double *__restrict__ q, AdjP P, unsigned long long *__restrict__ F) {
+k_adj_force(Rows R, const T *__restrict__ x, const double *__restrict__ q, AdjP P,
+            unsigned long long *__restrict__ F) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int32_t i = (int32_t)(t / kRowLanes), sub = (int32_t)(t % kRowLanes);
+    const int32_t sl = (int32_t)(t / kRowLanes), sub = (int32_t)(t % kRowLanes);
+    const int32_t i = sl < P.n ? R.perm[sl] : 0;
     long long fx = 0, fy = 0, fz = 0;
-    if (i < P.n) {
-    const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
+    if (sl < P.n) {
+    const int32_t *col = R.col(sl);
+    const int32_t c = R.count(sl);
     const T xi = x[3 * (int64_t)i], yi = x[3 * (int64_t)i + 1], zi = x[3 * (int64_t)i + 2];
     const double qi = q[i];
-    for (int64_t e = e0 + sub; e < e1; e += kRowLanes) {
-        const int32_t j = nbr[e];
+    for (int32_t k = sub; k < c; k += kRowLanes) {
+        const int32_t j = col[(int64_t)k * 64];
         const T dx = xi - x[3 * (int64_t)j], dy = yi - x[3 * (int64_t)j + 1], dz = zi - x[3 * (int64_t)j + 2];
         const T r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 < (T)P.rc2 || r2 < (T)P.rcc2)) continue;
@@ -203,7 +182,7 @@ k_adj_force(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restr
         fy += __shfl_xor(fy, off, 64);
         fz += __shfl_xor(fz, off, 64);
     }
-    if (i < P.n && sub == 0) {
+    if (sl < P.n && sub == 0) {
         F[3 * (int64_t)i] = (unsigned long long)fx;
         F[3 * (int64_t)i + 1] = (unsigned long long)fy;
         F[3 * (int64_t)i + 2] = (unsigned long long)fz;
@@ -256,18 +235,19 @@ constexpr int kMaxBins = 1024;
 
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_rdf_hist(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
-           const T *__restrict__ x, AdjP P, double *__restrict__ hist) {
+k_rdf_hist(Rows R, const T *__restrict__ x, AdjP P, double *__restrict__ hist) {
     __shared__ double hb[kMaxBins];
     for (int b = threadIdx.x; b < P.nbins; b += blockDim.x) hb[b] = 0.0;
     __syncthreads();
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < P.n) {
+    const int32_t sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl < P.n) {
+        const int32_t i = R.perm[sl];
         const double xi = (double)x[3 * (int64_t)i], yi = (double)x[3 * (int64_t)i + 1],
                      zi = (double)x[3 * (int64_t)i + 2];
-        const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
-        for (int64_t e = e0; e < e1; ++e) {
-            const int32_t j = nbr[e];
+        const int32_t *col = R.col(sl);
+        const int32_t c = R.count(sl);
+        for (int32_t k = 0; k < c; ++k) {
+            const int32_t j = col[(int64_t)k * 64];
             if (j < i) continue;  // each unordered pair once
             const double dx = xi - (double)x[3 * (int64_t)j], dy = yi - (double)x[3 * (int64_t)j + 1],
                          dz = zi - (double)x[3 * (int64_t)j + 2];
@@ -289,16 +269,18 @@ k_rdf_hist(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restri
 // lambda_x += dL/dx with L = sum_b (g_b - t_b)^2: wb[b] = 2 (g_b - t_b) norm_b
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_rdf_grad(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
-           const T *__restrict__ x, AdjP P, const double *__restrict__ wb, double *__restrict__ lam) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
+k_rdf_grad(Rows R, const T *__restrict__ x, AdjP P, const double *__restrict__ wb,
+           double *__restrict__ lam) {
+    const int32_t sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= P.n) return;
+    const int32_t i = R.perm[sl];
+    const int32_t *col = R.col(sl);
+    const int32_t c = R.count(sl);
     const double xi = (double)x[3 * (int64_t)i], yi = (double)x[3 * (int64_t)i + 1],
                  zi = (double)x[3 * (int64_t)i + 2];
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    for (int64_t e = e0; e < e1; ++e) {
-        const int32_t j = nbr[e];
+    for (int32_t k = 0; k < c; ++k) {
+        const int32_t j = col[(int64_t)k * 64];
         const double dx = xi - (double)x[3 * (int64_t)j], dy = yi - (double)x[3 * (int64_t)j + 1],
                      dz = zi - (double)x[3 * (int64_t)j + 2];
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -333,22 +315,24 @@ k_adj_vec(const double *__restrict__ lx, double *__restrict__ lv, double *__rest
 // term is taken with weight 1/2 from each.
 template <class T>
 __global__ void __launch_bounds__(nd::kBlock)
-k_adj_hvp(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restrict__ nbr,
-          const T *__restrict__ x, const double *__restrict__ q, AdjP P, const double *__restrict__ a,
-          double *__restrict__ lam, double *__restrict__ dq, double *__restrict__ dpar) {
+k_adj_hvp(Rows R, const T *__restrict__ x, const double *__restrict__ q, AdjP P,
+          const double *__restrict__ a, double *__restrict__ lam, double *__restrict__ dq,
+          double *__restrict__ dpar) {
     __shared__ double sh[2][nd::kBlock / 64];
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int32_t i = (int32_t)(t / kRowLanes), sub = (int32_t)(t % kRowLanes);
+    const int32_t sl = (int32_t)(t / kRowLanes), sub = (int32_t)(t % kRowLanes);
+    const int32_t i = sl < P.n ? R.perm[sl] : 0;
     double ge = 0.0, gs = 0.0;
     double mx = 0.0, my = 0.0, mz = 0.0, gq = 0.0;
-    if (i < P.n) {
+    if (sl < P.n) {
         const double xi = (double)x[3 * (int64_t)i], yi = (double)x[3 * (int64_t)i + 1],
                      zi = (double)x[3 * (int64_t)i + 2];
         const double ai = a[3 * (int64_t)i], bi = a[3 * (int64_t)i + 1], ci = a[3 * (int64_t)i + 2];
         const double qi = q[i];
-        const int64_t e0 = (int64_t)i * cap, e1 = e0 + cnt[i];
-        for (int64_t e = e0 + sub; e < e1; e += kRowLanes) {
-            const int32_t j = nbr[e];
+        const int32_t *col = R.col(sl);
+        const int32_t c = R.count(sl);
+        for (int32_t k = sub; k < c; k += kRowLanes) {
+            const int32_t j = col[(int64_t)k * 64];
             const double dx = xi - (double)x[3 * (int64_t)j], dy = yi - (double)x[3 * (int64_t)j + 1],
                          dz = zi - (double)x[3 * (int64_t)j + 2];
             const double r2 = dx * dx + dy * dy + dz * dz;
@@ -373,7 +357,7 @@ k_adj_hvp(int32_t cap, const int32_t *__restrict__ cnt, const int32_t *__restric
         mz += __shfl_xor(mz, off, 64);
         gq += __shfl_xor(gq, off, 64);
     }
-    if (i < P.n && sub == 0) {
+    if (sl < P.n && sub == 0) {
         lam[3 * (int64_t)i] += mx;
         lam[3 * (int64_t)i + 1] += my;
         lam[3 * (int64_t)i + 2] += mz;
@@ -411,7 +395,7 @@ struct nd_adj {
     float rlist = 0.0f;
     // state and scratch
     nd::DevBuf x, v, F, xf, xs, codes, perm, nodes, store, flags, q, ckx, ckv, seg, lx, lv, a, dq,
-        dpar, hist, wb, cnt, nbr;
+        dpar, hist, wb, rec, lists, lcnt;
     nd::SortBufs sort;
     int32_t cap = 0;  // row capacity
     int32_t nck = 0;
@@ -427,6 +411,16 @@ size_t state_bytes(const nd_adj *s) { return s->esz * 3 * (size_t)s->p.n; }
 template <class T> T *at(nd::DevBuf &b, size_t off_elems = 0) { return b.as<T>() + off_elems; }
 
 // neighbour rows (radius rlist, Hilbert key LBVH) at positions xp (device, T)
+// the walk's blocks of [cap][64] slots per 64 sorted atoms
+void reserve_rows(nd_adj *s) {
+    const size_t n64 = ((size_t)s->p.n + 63) & ~(size_t)63;
+    s->lists.reserve(sizeof(int32_t) * (size_t)s->cap * n64);
+}
+
+Rows rows_of(nd_adj *s) {
+    return Rows{s->lists.as<int32_t>(), s->lcnt.as<int32_t>(), s->perm.as<int32_t>(), s->cap};
+}
+
 template <class T>
 void build_list(nd_adj *s, const T *xp) {
     nd_ctx *ctx = s->ctx;
@@ -435,27 +429,28 @@ void build_list(nd_adj *s, const T *xp) {
     const int64_t m = 3 * (int64_t)n;
     int32_t *codes = s->codes.as<int32_t>(), *perm = s->perm.as<int32_t>(), *flags = s->flags.as<int32_t>();
     k_adj_to_f32<T><<<nd::grid_for(m), nd::kBlock, 0, st>>>(xp, m, s->xf.as<float>());
-    nd::launch_fill_i32(st, flags, 4, 0);
+    nd::launch_fill_i32(st, flags, 8, 0);
     nd::launch_morton(ctx, s->xf.as<float>(), n, ND_MORTON_HILBERT, nullptr, nullptr, codes, perm, flags + 2);
     nd::radix_sort_pairs(st, s->sort, codes, perm, n, 0, 30);  // Hilbert keys are < 2^30
-    nd::launch_gather_xyz(st, s->xf.as<float>(), perm, n, s->xs.as<float>());
-    nd::launch_lbvh(st, s->xs.as<float>(), 3, codes, nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
+    k_adj_pos4<<<nd::grid_for(n), nd::kBlock, 0, st>>>(s->xf.as<float>(), perm, n, s->xs.as<float4>());
+    nd::launch_lbvh(st, s->xs.as<float>(), 4, codes, nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
                     s->store.as<int32_t>(), flags + 1);
-    const float hl = s->rlist / 2.0f, rl2 = s->rlist * s->rlist;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        k_adj_nlist<<<nd::grid_for(n), nd::kBlock, 0, st>>>(s->xs.as<float>(), perm, s->nodes.as<nd_node>(), n, nL,
-                                                            s->apl, hl, rl2, s->cap, s->nbr.as<int32_t>(),
-                                                            s->cnt.as<int32_t>(), flags);
+        // the simulation's list walk over the sorted atoms, then original ids
+        nd::launch_list_walk(st, s->nodes.as<nd_node>(), s->rec.as<float4>(), s->xs.as<float4>(), n, nL,
+                             s->apl, s->rlist, s->cap, s->lists.as<int32_t>(), s->lcnt.as<int32_t>(), flags + 4);
+        k_adj_map_ids<<<dim3(nd::grid_for(n), (unsigned)s->cap), nd::kBlock, 0, st>>>(
+            perm, n, s->cap, s->lists.as<int32_t>(), s->lcnt.as<int32_t>());
         ND_HIP(hipGetLastError());
-        int32_t h[4];
+        int32_t h[8];
         ND_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, st));
         ND_HIP(hipStreamSynchronize(st));
         if (h[2]) throw nd::Error(ND_ERR_DOMAIN, "adjoint: non-finite coordinate");
-        if (h[1]) throw nd::Error(ND_ERR_STATE, "adjoint: malformed neighbour tree");
-        if (h[0] <= s->cap) return;
-        s->cap = h[0] + h[0] / 4 + 8;
-        s->nbr.reserve(sizeof(int32_t) * (size_t)s->cap * (size_t)n);
-        nd::launch_fill_i32(st, flags, 1, 0);
+        if (h[1] || h[6]) throw nd::Error(ND_ERR_STATE, "adjoint: malformed neighbour tree");
+        if (h[4] <= s->cap) return;
+        s->cap = h[4] + h[4] / 4 + 8;
+        reserve_rows(s);
+        nd::launch_fill_i32(st, flags + 4, 3, 0);
     }
     throw nd::Error(ND_ERR_CAPACITY, "adjoint: neighbour rows kept overflowing");
 }
@@ -465,7 +460,7 @@ void compute_force(nd_adj *s, const T *xs) {
     hipStream_t st = s->ctx->stream;
     const int64_t m = 3 * (int64_t)s->p.n;
     (void)m;
-    k_adj_force<T><<<nd::grid_for((int64_t)s->p.n * kRowLanes), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+    k_adj_force<T><<<nd::grid_for((int64_t)s->p.n * kRowLanes), nd::kBlock, 0, st>>>(rows_of(s),
                                                                  xs, s->q.as<double>(), s->P,
                                                                  s->F.as<unsigned long long>());
     ND_HIP(hipGetLastError());
@@ -506,7 +501,7 @@ void forward_impl(nd_adj *s) {
     if (T_ % s->p.rebuild_every != 0) build_list<T>(s, x);
     const int32_t nb = s->p.nbins;
     k_zero_f64<<<nd::grid_for(nb), nd::kBlock, 0, st>>>(s->hist.as<double>(), nb);
-    k_rdf_hist<T><<<nd::grid_for(s->p.n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+    k_rdf_hist<T><<<nd::grid_for(s->p.n), nd::kBlock, 0, st>>>(rows_of(s),
                                                                 x, s->P, s->hist.as<double>());
     ND_HIP(hipGetLastError());
     std::vector<double> h(nb);
@@ -535,7 +530,7 @@ void backward_impl(nd_adj *s) {
     std::vector<double> wb(s->p.nbins);
     for (int32_t b = 0; b < s->p.nbins; ++b) wb[b] = 2.0 * (s->rdf[b] - s->target[b]) * s->norm[b];
     ND_HIP(hipMemcpyAsync(s->wb.p, wb.data(), sizeof(double) * wb.size(), hipMemcpyHostToDevice, st));
-    k_rdf_grad<T><<<nd::grid_for(n), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+    k_rdf_grad<T><<<nd::grid_for(n), nd::kBlock, 0, st>>>(rows_of(s),
                                                            s->x.as<T>(), s->P, s->wb.as<double>(), lx);
     ND_HIP(hipGetLastError());
     T *seg = s->seg.as<T>();
@@ -557,7 +552,7 @@ void backward_impl(nd_adj *s) {
             }
             k_adj_vec<<<nd::grid_for(m), nd::kBlock, 0, st>>>(lx, lv, a, m, s->P.dt,
                                                                 k == 0 ? s->P.c : 2.0 * s->P.c);
-            k_adj_hvp<T><<<nd::grid_for((int64_t)n * kRowLanes), nd::kBlock, 0, st>>>(s->cap, s->cnt.as<int32_t>(), s->nbr.as<int32_t>(),
+            k_adj_hvp<T><<<nd::grid_for((int64_t)n * kRowLanes), nd::kBlock, 0, st>>>(rows_of(s),
                                                                   xk, s->q.as<double>(), s->P, a, lx,
                                                                   s->dq.as<double>(), s->dpar.as<double>());
             ND_HIP(hipGetLastError());
@@ -573,10 +568,10 @@ int nd_adj_create(nd_ctx *ctx, const nd_adj_params *p, nd_adj **out) {
     ND_API_BEGIN
     ND_REQUIRE(ctx && p && out, "nd_adj_create: null argument");
     *out = nullptr;
-    ND_REQUIRE(p->n >= 4 && p->n % 2 == 0 && p->nsteps >= 1 && p->checkpoint_every >= 1 &&
+    ND_REQUIRE(p->n >= 8 && p->n % 4 == 0 && p->nsteps >= 1 && p->checkpoint_every >= 1 &&
                    p->rebuild_every >= 1,
-               "nd_adj_create: even n >= 4 (pair-list leaves of >= 2 atoms); nsteps, checkpoint_every, "
-               "rebuild_every >= 1");
+               "nd_adj_create: n >= 8 and a multiple of 4 (list leaves of 4, 8 or 16 atoms); nsteps, "
+               "checkpoint_every, rebuild_every >= 1");
     ND_REQUIRE(p->checkpoint_every % p->rebuild_every == 0,
                "nd_adj_create: checkpoint_every must be a multiple of rebuild_every");
     ND_REQUIRE(p->precision == ND_PREC_FP32 || p->precision == ND_PREC_FP64,
@@ -593,8 +588,8 @@ int nd_adj_create(nd_ctx *ctx, const nd_adj_params *p, nd_adj **out) {
         s->p = *p;
         s->dbl = p->precision == ND_PREC_FP64;
         s->esz = s->dbl ? 8 : 4;
-        s->apl = 2;
-        for (int32_t a : {16, 8, 4, 2})
+        s->apl = 4;
+        for (int32_t a : {16, 8, 4})
             if (p->n % a == 0 && p->n / a >= 2) { s->apl = a; break; }
         const double reach = std::max(std::max(p->rc, p->rc_coul), p->rdf_rmax + p->rdf_h);
         // FP32 copies of the coordinates feed the list: pad by their rounding
@@ -616,16 +611,17 @@ int nd_adj_create(nd_ctx *ctx, const nd_adj_params *p, nd_adj **out) {
         s->dq.reserve(8 * n);
         s->dpar.reserve(16);
         s->hist.reserve(8 * (size_t)p->nbins);
-        s->cnt.reserve(4 * n);
-        s->xs.reserve(4 * m);
+        s->xs.reserve(sizeof(float4) * n);
+        s->lcnt.reserve(4 * n);
+        s->rec.reserve(8 * sizeof(float4) * (size_t)(p->n / s->apl - 1));
         s->codes.reserve(4 * n);
         s->perm.reserve(4 * n);
         s->nodes.reserve(sizeof(nd_node) * (size_t)(2 * (p->n / s->apl) - 1));
         s->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(p->n / s->apl));
-        s->flags.reserve(64);
+        s->flags.reserve(64);  // [0..3] tree build status words, [4..6] the walk's counters
         s->wb.reserve(8 * (size_t)p->nbins);
         s->cap = 96;
-        s->nbr.reserve(sizeof(int32_t) * (size_t)s->cap * n);
+        reserve_rows(s);
         // RDF normalisation: ideal-gas pair count in each bin's shell
         const double dr = p->rdf_rmax / p->nbins, npair = 0.5 * (double)p->n * (p->n - 1);
         s->norm.resize(p->nbins);
@@ -648,9 +644,9 @@ int nd_adj_destroy(nd_adj *s) {
         (void)hipSetDevice(s->ctx->device);
         (void)hipStreamSynchronize(s->ctx->stream);
     }
-    nd::DevBuf *bufs[] = {&s->x, &s->v, &s->F, &s->xf, &s->xs, &s->codes, &s->perm, &s->nodes,
+    nd::DevBuf *bufs[] = {&s->rec, &s->lists, &s->lcnt, &s->x, &s->v, &s->F, &s->xf, &s->xs, &s->codes, &s->perm, &s->nodes,
                           &s->store, &s->flags, &s->q, &s->ckx, &s->ckv, &s->seg, &s->lx, &s->lv,
-                          &s->a, &s->dq, &s->dpar, &s->hist, &s->wb, &s->cnt, &s->nbr,
+                          &s->a, &s->dq, &s->dpar, &s->hist, &s->wb,
                           &s->sort.keys_alt, &s->sort.vals_alt, &s->sort.hist, &s->sort.flags};
     for (auto *b : bufs) b->release();
     delete s;

@@ -120,6 +120,17 @@ void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
                  nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared = false);
 
+// nd_sim.hip: the two-phase list walk over an LBVH of float4 positions
+// (sorted; leaf boxes +- rlist/2) with 4, 8, 16 or 32 atoms per leaf.  It
+// writes the four-wide records into rec (8 float4 per internal node), one
+// block of [cap][64] int32 neighbour slots per 64 sorted atoms into nbr (the
+// neighbours' sorted indices), each atom's count into ncount, and into ctr:
+// [0] the largest count (atomic max), [1] 1 if any count exceeds cap,
+// [2] kStatusTree if the tree was malformed.
+void launch_list_walk(hipStream_t st, const nd_node *nodes, float4 *rec, const float4 *pos, int32_t n,
+                      int32_t nL, int32_t apl, float rlist, int32_t cap, int32_t *nbr, int32_t *ncount,
+                      int32_t *ctr);
+
 // nd_traverse.hip
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
                       const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t predicate,

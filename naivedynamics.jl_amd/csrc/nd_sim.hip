@@ -25,6 +25,7 @@
 // / velocity accumulation in FP64), FP64 (everything in double).  Energies and the virial
 // are reduced per workgroup in FP64 and summed with Neumaier compensation.
 #include <cmath>
+#include <cstddef>
 #include <type_traits>
 #include <vector>
 
@@ -36,6 +37,8 @@ constexpr int kModeForce = 0;  // F(x_n), store F, energy at n
 constexpr int kModeFused = 1;  // F(x_n), finish v_n, [energy], start step: v_{n+1/2}, x_{n+1}
 constexpr int kModeFinal = 2;  // F(x_n), finish v_n, [energy], store F
 
+// The first three words are the list walk's counters (k_nlist3 takes them as
+// an int32 triple: maximum count, overflow flag, status bits).
 struct SimMisc {
     int32_t maxcnt;
     int32_t overflow;
@@ -46,6 +49,9 @@ struct SimMisc {
     double ke;
     double pad[3];
 };
+static_assert(offsetof(SimMisc, maxcnt) == 0 && offsetof(SimMisc, overflow) == 4 &&
+                  offsetof(SimMisc, status) == 8,
+              "the walk's counter triple");
 
 struct StepParams {
     int32_t n, cap, walls, pad;
@@ -808,7 +814,7 @@ template <int A>
 __global__ void __launch_bounds__(nd::kBlock)
 k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
          float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
-         SimMisc *__restrict__ misc) {
+         int32_t *__restrict__ ctr) {
     static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
     constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
     constexpr int kSlot = 4 * A + 4;     // floats per group slot (16 B pad: no bank conflicts)
@@ -959,14 +965,14 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         if (bad) break;
         wave_lds_sync();
     }
-    if (bad && lane == 0) atomicOr(&misc->status, nd::kStatusTree);
+    if (bad && lane == 0) atomicOr(&ctr[2], nd::kStatusTree);
     const int32_t cnt = (int32_t)((voff - (uint32_t)lane * 4u) >> 8);
     if (active) ncount[i] = cnt;
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
     if (lane == 0 && m > 0) {
-        atomicMax(&misc->maxcnt, m);
-        if (m > cap) atomicOr(&misc->overflow, 1);
+        atomicMax(&ctr[0], m);
+        if (m > cap) atomicOr(&ctr[1], 1);
     }
 }
 
@@ -998,6 +1004,27 @@ __global__ void k_misc_reset_counters(SimMisc *misc) {
 }
 
 }  // namespace
+
+namespace nd {
+
+void launch_list_walk(hipStream_t st, const nd_node *nodes, float4 *rec, const float4 *pos, int32_t n,
+                      int32_t nL, int32_t apl, float rlist, int32_t cap, int32_t *nbr, int32_t *ncount,
+                      int32_t *ctr) {
+    if (nL >= kMaxLeaves3) throw Error(ND_ERR_ARG, "list walk: too many leaves");
+    launch_lbvh_wide4(st, nodes, nL, rec);
+    const float hl = rlist / 2.0f, rl2 = rlist * rlist;
+    const unsigned g = grid_for(n);
+    switch (apl) {
+        case 4: k_nlist3<4><<<g, kBlock, 0, st>>>(pos, rec, n, nL, hl, rl2, cap, nbr, ncount, ctr); break;
+        case 8: k_nlist3<8><<<g, kBlock, 0, st>>>(pos, rec, n, nL, hl, rl2, cap, nbr, ncount, ctr); break;
+        case 16: k_nlist3<16><<<g, kBlock, 0, st>>>(pos, rec, n, nL, hl, rl2, cap, nbr, ncount, ctr); break;
+        case 32: k_nlist3<32><<<g, kBlock, 0, st>>>(pos, rec, n, nL, hl, rl2, cap, nbr, ncount, ctr); break;
+        default: throw Error(ND_ERR_ARG, "list walk: leaf size must be 4, 8, 16 or 32");
+    }
+    ND_HIP(hipGetLastError());
+}
+
+}  // namespace nd
 
 struct nd_sim {
     nd_ctx *ctx = nullptr;
@@ -1103,6 +1130,7 @@ void launch_rebuild(nd_sim *s, bool full) {
     const float4 *p4 = s->wide ? s->pos32.as<float4>() : s->pos[s->cur].as<float4>();
     const float hl = s->rlist / 2.0f, rl2 = s->rlist * s->rlist;
     int32_t *nb = s->nbr.as<int32_t>(), *nc = s->ncount.as<int32_t>();
+    int32_t *ctr = reinterpret_cast<int32_t *>(misc);  // maxcnt, overflow, status
     if (s->p.list_walk == 1) {
         const nd_node *nodes = s->nodes.as<nd_node>();
         switch (s->apl) {
@@ -1119,13 +1147,13 @@ void launch_rebuild(nd_sim *s, bool full) {
         if (two_phase) nd::launch_lbvh_wide4(st, s->nodes.as<nd_node>(), s->nL, kids);
         else nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
         if (two_phase && s->apl == 4)
-            k_nlist3<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+            k_nlist3<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
         else if (two_phase && s->apl == 8)
-            k_nlist3<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+            k_nlist3<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
         else if (two_phase && s->apl == 16)
-            k_nlist3<16><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+            k_nlist3<16><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
         else if (two_phase && s->apl == 32)
-            k_nlist3<32><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, misc);
+            k_nlist3<32><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
         else switch (s->apl) {
             case 4: k_nlist2<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 4, hl, rl2, s->cap, nb, nc, misc); break;
             case 8: k_nlist2<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 8, hl, rl2, s->cap, nb, nc, misc); break;

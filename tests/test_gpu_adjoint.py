@@ -112,3 +112,9 @@ def test_fp32_gradient_close_to_fp64(pkg, O):
     assert abs(a["deps"] - b["deps"]) < 1e-3 * abs(b["deps"])
     assert abs(a["dsigma"] - b["dsigma"]) < 1e-3 * abs(b["dsigma"])
     assert np.linalg.norm(a["dx0"] - b["dx0"]) < 1e-3 * np.linalg.norm(b["dx0"])
+
+
+def test_size_rule(pkg):
+    # the adjoint's list walk uses leaves of 4, 8 or 16 atoms
+    with pytest.raises(pkg._lib.SpecError):
+        pkg.Adjoint(50, 10)
