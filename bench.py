@@ -106,6 +106,33 @@ def measured_traffic():
     return None
 
 
+def search_comparison(pkg, c):
+    """The one-call LBVH neighbour list (interleaved key, 4 atoms per leaf)
+    against the cell list (nd_cell_neighborlist) on the same 262,144 positions,
+    r = r_list, unique pairs with s < r^2; median wall time of 5 calls each
+    (inputs resident on the device, results left there)."""
+    x = np.ascontiguousarray(c["x"], np.float32)
+    ctx = pkg.default_context()
+    d = pkg.DeviceArray.from_host(ctx, x)
+    r = float(np.float32(c["rc"] + c["skin"]))
+    spec = pkg.SpheresBVHSpecs(neighbor_distance=r, atom_count=len(x), atomsperleaf=4)
+
+    def timed(fn):
+        count = fn()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3, count
+
+    ms_l, n_l = timed(lambda: pkg.neighborlist_device(d, spec, pkg.PRED_SCALAR, pkg.MORTON_INTERLEAVE)[1])
+    ms_c, n_c = timed(lambda: pkg.cells.neighborlist_device(d, r, pkg.PRED_SCALAR)[1])
+    log("search: LBVH %.3f ms (%d pairs), cell list %.3f ms (%d pairs)" % (ms_l, n_l, ms_c, n_c))
+    return {"workload": "ljc-262144 initial positions", "r": r, "lbvh_ms": round(ms_l, 3),
+            "lbvh_pairs": int(n_l), "cell_list_ms": round(ms_c, 3), "cell_list_pairs": int(n_c)}
+
+
 def precision_form(pkg, steps=500):
     """1,048,576 atoms in 64 blobs: FP32, mixed (FP32 pair maths, FP64 state and
     sums) and plain FP64 from the same state; force error of FP32 and mixed
@@ -359,6 +386,18 @@ def main():
         extra["fp32_resort50_steps_per_s"] = round(args.steps / (ms2 / 1e3), 3)
         s2.close()
         log("fp32 resort every 50: %.4f ms/step" % (ms2 / args.steps))
+        # the cell-list comparison point: the same steps with lists from a
+        # uniform cell list instead of the LBVH walk
+        spec = make_spec(pkg, c, "fp32", pkg.MORTON_HILBERT)
+        spec.list_walk = 3
+        s2 = pkg.Simulation(c["n"], spec)
+        s2.set_state(c["x"], c["v"], c["q"])
+        s2.run(args.warmup)
+        ms2, _ = timed_run(pkg, s2, args.steps)
+        extra["fp32_cell_list_steps_per_s"] = round(args.steps / (ms2 / 1e3), 3)
+        s2.close()
+        log("fp32 cell-list walk: %.4f ms/step" % (ms2 / args.steps))
+        extra["search"] = search_comparison(pkg, c)
         result["forms"] = extra
         result["precision_form"] = precision_form(pkg)
         result["batched_form"] = batched_form(pkg)

@@ -152,6 +152,19 @@ int nd_tree_export(nd_ctx *ctx, nd_node *h_nodes, int32_t *h_perm, int32_t *h_co
 int nd_unique_threshold_pairs(nd_ctx *ctx, const float *d_xyz, int32_t n, float threshold,
                               nd_pair *d_out, int64_t cap, int64_t *count);
 
+/* neighborlist(x, cutoff) of CellListMap, the cell-list search the reference
+ * benchmarks its LBVH against (assets/BVHBenchSuite.jl:307; src/MDInput.jl:290):
+ * every unordered pair of d_xyz (N x 3 float, AoS) accepted by predicate
+ * (ND_PRED_SCALAR: s < cutoff^2, ND_PRED_SIMD: sqrt(s) < cutoff, with s folded
+ * as ((dx^2 + dy^2) + dz^2)), once, as (min index, max index, Float32
+ * distance) with 0-based original indices.  Uniform cells of side >= cutoff
+ * over the bounding box; atoms with a non-finite coordinate pair with none.
+ * The pair set and distances equal nd_unique_threshold_pairs' (ND_PRED_SIMD)
+ * bit for bit; the order is deterministic (cell order of the earlier atom).
+ * Capacity protocol as above. */
+int nd_cell_neighborlist(nd_ctx *ctx, const float *d_xyz, int32_t n, float cutoff,
+                         int32_t predicate, nd_pair *d_out, int64_t cap, int64_t *count);
+
 /* ---- forces over a pair list --------------------------------------------- */
 typedef struct {
     float eps, sigma; /* Lennard-Jones */
@@ -193,7 +206,9 @@ typedef struct {
                              keep the atom order and tree topology and refit the
                              boxes to the current positions. */
     int32_t list_walk;    /* list walk: 0 = two-phase child-box walk (node frontier, then
-                             queued leaves), 1 = rope walk, 2 = one-node child-box walk */
+                             queued leaves), 1 = rope walk, 2 = one-node child-box walk,
+                             3 = uniform cell list over [lo, hi]^3 (no LBVH; the
+                             comparison point; needs hi > lo) */
     double eps, sigma, rc; /* LJ */
     double k_coul, rc_coul;/* Coulomb (rc_coul = 0 disables) */
     double skin;           /* list radius = max(rc, rc_coul) + skin */

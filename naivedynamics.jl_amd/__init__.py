@@ -8,7 +8,7 @@ The directory name is not a Python identifier; load the package with
         "naivedynamics_amd", ".../naivedynamics.jl_amd/__init__.py",
         submodule_search_locations=[".../naivedynamics.jl_amd"])
 """
-from . import _lib, adjoint, batched, bvh, forces, simulator, synth
+from . import _lib, adjoint, batched, bvh, cells, forces, simulator, synth
 from ._lib import (MORTON_HILBERT, MORTON_INTERLEAVE, MORTON_MASKED, NODE_DTYPE, PAIR_DTYPE, PRED_SCALAR,
                    PRED_SIMD, PREC_FP32, PREC_FP64, PREC_MIXED, CapacityError, Context,
                    DeviceArray, DomainError, NDError, SpecError, default_context)
@@ -16,6 +16,7 @@ from .bvh import (SpheresBVHSpecs, binwidth, bounding_volume_hierarchy, canonica
                   exptsimdbuild_traverse_bvh, gpubvh_neighborlist, leafbuild_traverse_bvh,
                   leafneighbor_traverse, leafTreeData, mortoncodes, neighborlist_device,
                   simdbuild_traverse_bvh, sortperm, threshold_unique_pairs)
+from .cells import neighborlist as cell_neighborlist
 from .forces import force_coulomb_, force_lennardjones_, sum_forces_
 from .simulator import ObjectCollection, SimSpec, Simulation, simulate_, simulate_bvh_
 from .batched import BatchedSimulation, autotune

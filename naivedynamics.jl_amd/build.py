@@ -17,7 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnaivehip.so")
 ARCH = os.environ.get("ND_OFFLOAD_ARCH", "gfx950")
 
-EXACT = ["nd_ctx.hip", "nd_morton.hip", "nd_sort.hip", "nd_lbvh.hip", "nd_traverse.hip"]
+EXACT = ["nd_ctx.hip", "nd_morton.hip", "nd_sort.hip", "nd_lbvh.hip", "nd_traverse.hip",
+         "nd_cells.hip"]
 FAST = ["nd_force.hip", "nd_sim.hip", "nd_adjoint.hip"]
 
 COMMON = [

@@ -60,7 +60,8 @@ int nd_ctx_destroy(nd_ctx *ctx) {
                           &ctx->xyz_sorted, &ctx->nodes, &ctx->store, &ctx->counts,
                           &ctx->offsets, &ctx->xyz_in, &ctx->pairs_out, &ctx->bbox,
                           &ctx->sort.keys_alt, &ctx->sort.vals_alt, &ctx->sort.hist,
-                          &ctx->sort.flags};
+                          &ctx->sort.flags, &ctx->cell_keys, &ctx->cell_vals, &ctx->cell_pos,
+                          &ctx->cell_start};
     for (auto *b : bufs) b->release();
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);

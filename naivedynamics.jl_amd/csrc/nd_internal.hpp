@@ -70,6 +70,7 @@ struct nd_ctx {
     nd::DevBuf status;  // int32 status word + int64 counters
     nd::DevBuf codes, codes_sorted, perm, xyz_sorted, nodes, store, counts, offsets, xyz_in,
         pairs_out, bbox;
+    nd::DevBuf cell_keys, cell_vals, cell_pos, cell_start;  // nd_cell_neighborlist
     nd::SortBufs sort;
     // last tree built by nd_leafbuild_traverse_bvh
     int32_t tree_n = 0, tree_apl = 0;
@@ -135,6 +136,37 @@ void launch_list_walk(hipStream_t st, const nd_node *nodes, float4 *rec, const f
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
                       const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t predicate,
                       nd_pair *out, int64_t cap);
+// Exclusive scan of ctx->counts (n int64) into ctx->offsets (n + 1); synchronises
+// and returns the total.
+int64_t scan_counts(nd_ctx *ctx, int64_t n);
+
+// Uniform cell grid: cell d-coordinate of x is clamp(floor((x - lo[d]) * inv[d]),
+// 0, g[d] - 1), computed in double so that it is exact to far below a cell;
+// NaN maps to 0.  Cells are numbered (cz * g[1] + cy) * g[0] + cx (x fastest).
+struct CellGrid {
+    double lo[3], inv[3];
+    int32_t g[3], ncell;
+};
+// Grid over [lo, lo + ext] (per axis) with cells of side >= r (1 + 1e-4), at most
+// 1024 per axis and max_cells in total (larger cells when capped).
+CellGrid make_cell_grid(const double lo[3], const double ext[3], double r, int64_t max_cells);
+
+// nd_cells.hip: cell index of n positions (stride 3 or 4 floats): keys (cell
+// number; ncell for a non-finite coordinate, so such atoms sort last and lie
+// in no cell), a stable sort by key with vals = 0..n-1, cpos[t] = (x, y, z,
+// bits of vals[t]) for sorted position t, and cstart[k] = first sorted
+// position with key >= k for k = 0..ncell.  keys/vals must hold n int32;
+// cstart ncell + 1.  Returns the sorted keys (keys, or sb's alternate buffer).
+const int32_t *launch_cell_index(hipStream_t st, SortBufs &sb, const float *xyz, int32_t stride,
+                                 int32_t n, const CellGrid &cg, int32_t *keys, int32_t *vals,
+                                 float4 *cpos, int32_t *cstart);
+
+__device__ __forceinline__ int32_t cell_coord(const CellGrid &cg, float x, int d) {
+    const double t = ((double)x - cg.lo[d]) * cg.inv[d];
+    if (!(t > 0.0)) return 0;
+    if (t >= (double)(cg.g[d] - 1)) return cg.g[d] - 1;
+    return (int32_t)t;
+}
 
 void set_error(nd_ctx *ctx, const std::string &m);
 int32_t read_status(nd_ctx *ctx);  // synchronises; returns and clears the status word
@@ -198,6 +230,28 @@ __device__ __forceinline__ uint32_t hilbert30(uint32_t x, uint32_t y, uint32_t z
     y ^= t;
     z ^= t;
     return (spread10(x) << 2) | (spread10(y) << 1) | spread10(z);
+}
+
+// s = sum((a - b).^2) as StaticArrays folds it: ((dx*dx + dy*dy) + dz*dz).
+// Exact only in files compiled with -ffp-contract=off (no multiply-add fused).
+__device__ __forceinline__ float sqdist(float ax, float ay, float az, float bx, float by,
+                                        float bz) {
+    const float dx = ax - bx, dy = ay - by, dz = az - bz;
+    const float sx = dx * dx, sy = dy * dy, sz = dz * dz;
+    float s = sx + sy;
+    s = s + sz;
+    return s;
+}
+
+// ND_PRED_SCALAR: s < fl(r*r) then d = sqrt(s) (BVHTraverse.jl:1027,1048).
+// ND_PRED_SIMD:   d = sqrt(s), d < r            (NaiveSIMD.jl:529,589).
+__device__ __forceinline__ bool accept(float s, float r, float r2, int32_t pred, float &d) {
+    if (pred == ND_PRED_SCALAR) {
+        if (s < r2) { d = sqrtf(s); return true; }
+        return false;
+    }
+    d = sqrtf(s);
+    return d < r;
 }
 
 // q = round(Int32, v, RoundDown): defined for -2^31 <= v < 2^31 only.

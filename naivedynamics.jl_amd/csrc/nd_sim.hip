@@ -976,6 +976,57 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     }
 }
 
+// Cell-list walk (list_walk 3), the comparison point for the LBVH walks: the
+// same list (every other atom with r^2 < r_list^2, as sorted indices) from the
+// 27 cells of side >= r_list around the atom's own cell, scanned as 9 runs of
+// 3 cells that are contiguous in the cell-sorted copy cpos (w = sorted index,
+// cstart = first position of each cell).  One lane per atom; appends go to
+// the same [cap][64] blocks as the LBVH walks', with the same counters.
+__global__ void __launch_bounds__(nd::kBlock)
+k_cell_rows(const float4 *__restrict__ pos, const float4 *__restrict__ cpos,
+            const int32_t *__restrict__ cstart, int32_t n, nd::CellGrid cg, float rl2, int32_t cap,
+            int32_t *__restrict__ nbr, int32_t *__restrict__ ncount, int32_t *__restrict__ ctr) {
+    const int32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool active = i < n;
+    // this wave's block of the list as a buffer resource: a row at or past cap
+    // lies outside it, and the hardware drops such a store
+    const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(nbr + (int64_t)__builtin_amdgcn_readfirstlane(i >> 6) * cap * 64), 0, cap * 256, kBufWord3);
+    uint32_t voff = (uint32_t)lane * 4u;
+    if (active) {
+        const float4 p = pos[i];
+        const int32_t cx = nd::cell_coord(cg, p.x, 0), cy = nd::cell_coord(cg, p.y, 1),
+                      cz = nd::cell_coord(cg, p.z, 2);
+        const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < cg.g[0] ? cx + 1 : cg.g[0] - 1;
+        const int32_t y0 = cy > 0 ? cy - 1 : 0, y1 = cy + 1 < cg.g[1] ? cy + 1 : cg.g[1] - 1;
+        const int32_t z0 = cz > 0 ? cz - 1 : 0, z1 = cz + 1 < cg.g[2] ? cz + 1 : cg.g[2] - 1;
+        for (int32_t z = z0; z <= z1; ++z)
+            for (int32_t y = y0; y <= y1; ++y) {
+                const int32_t b = (z * cg.g[1] + y) * cg.g[0];
+                const int32_t e = cstart[b + x1 + 1];
+                for (int32_t t = cstart[b + x0]; t < e; ++t) {
+                    const float4 q = cpos[t];
+                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    const float s = dx * dx + dy * dy + dz * dz;
+                    const int32_t j = __float_as_int(q.w);
+                    if ((s < rl2) & (j != i)) {
+                        __builtin_amdgcn_raw_buffer_store_b32(j, lrs, voff, 0, 0);
+                        voff += 256u;
+                    }
+                }
+            }
+    }
+    const int32_t cnt = (int32_t)((voff - (uint32_t)lane * 4u) >> 8);
+    if (active) ncount[i] = cnt;
+    int32_t m = cnt;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
+    if (lane == 0 && m > 0) {
+        atomicMax(&ctr[0], m);
+        if (m > cap) atomicOr(&ctr[1], 1);
+    }
+}
+
 // Trajectory frame: slot s's position to row orig[s] of out (n x 3).
 template <class PosS, class OutT>
 __global__ void __launch_bounds__(nd::kBlock)
@@ -1033,6 +1084,8 @@ struct nd_sim {
     bool wide = false;  // FP64 storage (MIXED, FP64)
     nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, fix[2], codes, perm, nodes, store, kids, nbr,
         ncount, misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig, rep, boxes;
+    nd::DevBuf cpos, cstart;  // cell-list walk: cell-sorted positions, cell starts
+    nd::CellGrid cg{};
     int32_t nrep = 0;  // packed replicas (0: one system)
     nd::SortBufs sort;
     int cur = 0;   // position buffer (flips every drift and every re-sort)
@@ -1116,6 +1169,26 @@ void launch_rebuild(nd_sim *s, bool full) {
     const int32_t n = s->n;
     const unsigned g = nd::grid_for(n);
     SimMisc *misc = s->misc.as<SimMisc>();
+    if (s->p.list_walk == 3) {
+        // cell list: the atom order (for the force kernel) still follows the
+        // sort key; the cell index is rebuilt from the current positions
+        if (full) {
+            launch_sort_and_build(s);
+            s->tree_valid = true;
+        } else if (s->wide) {
+            k_pos32<<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), n, s->pos32.as<float4>());
+        }
+        const float4 *p4 = s->wide ? s->pos32.as<float4>() : s->pos[s->cur].as<float4>();
+        float4 *cpos = s->cpos.as<float4>();
+        int32_t *cstart = s->cstart.as<int32_t>();
+        nd::launch_cell_index(st, s->sort, reinterpret_cast<const float *>(p4), 4, n, s->cg,
+                              s->codes.as<int32_t>(), s->perm.as<int32_t>(), cpos, cstart);
+        k_cell_rows<<<g, nd::kBlock, 0, st>>>(p4, cpos, cstart, n, s->cg, s->rlist * s->rlist, s->cap,
+                                              s->nbr.as<int32_t>(), s->ncount.as<int32_t>(),
+                                              reinterpret_cast<int32_t *>(misc));
+        ND_HIP(hipGetLastError());
+        return;
+    }
     if (full) {
         launch_sort_and_build(s);
         s->tree_valid = true;
@@ -1203,6 +1276,10 @@ void launch_sort_and_build(nd_sim *s) {
                                                    nullptr, nullptr, s->sp, misc);
     s->cur = a;
     s->vcur = va;
+    if (s->p.list_walk == 3) {  // the cell list needs no tree
+        ND_HIP(hipGetLastError());
+        return;
+    }
     const float *p32 = s->wide ? reinterpret_cast<const float *>(s->pos32.p)
                                : reinterpret_cast<const float *>(s->pos[s->cur].p);
     nd::launch_lbvh(st, p32, 4, codes, s->nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
@@ -1510,7 +1587,8 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     ND_REQUIRE(p->rebuild_every >= 1, "nd_sim_create: rebuild_every >= 1");
     ND_REQUIRE(p->resort_every >= 0 && p->resort_every % p->rebuild_every == 0,
                "nd_sim_create: resort_every must be a multiple of rebuild_every (or 0)");
-    ND_REQUIRE(p->list_walk >= 0 && p->list_walk <= 2, "nd_sim_create: list_walk must be 0, 1 or 2");
+    ND_REQUIRE(p->list_walk >= 0 && p->list_walk <= 3, "nd_sim_create: list_walk must be 0, 1, 2 or 3");
+    ND_REQUIRE(p->list_walk != 3 || p->hi > p->lo, "nd_sim_create: the cell-list walk needs a box (hi > lo)");
     ND_REQUIRE(p->dt > 0.0f && p->rc >= 0.0f && p->rc_coul >= 0.0f && p->skin >= 0.0f,
                "nd_sim_create: dt > 0, cutoffs and skin >= 0");
     ND_REQUIRE(p->rc > 0.0f || p->rc_coul > 0.0f, "nd_sim_create: no interaction enabled");
@@ -1556,6 +1634,13 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->sort.keys_alt.reserve(sizeof(int32_t) * n);
         s->sort.vals_alt.reserve(sizeof(int32_t) * n);
         s->sort.hist.reserve(sizeof(int32_t) * (1024 * (size_t)ntiles + 1024));  // 10-bit digits
+        if (p->list_walk == 3) {
+            const double lo3[3] = {p->lo, p->lo, p->lo}, ext = p->hi - p->lo;
+            const double ext3[3] = {ext, ext, ext};
+            s->cg = nd::make_cell_grid(lo3, ext3, (double)s->rlist, std::min<int64_t>(1 << 22, 2 * (int64_t)n + 64));
+            s->cpos.reserve(sizeof(float4) * n);
+            s->cstart.reserve(sizeof(int32_t) * ((size_t)s->cg.ncell + 1));
+        }
         ND_HIP(hipMemsetAsync(s->misc.p, 0, sizeof(SimMisc), ctx->stream));
         int32_t cap = p->max_neighbors;
         if (cap <= 0) {
@@ -1589,7 +1674,7 @@ int nd_sim_destroy(nd_sim *s) {
                           &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids, &s->rep, &s->boxes,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
                           &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
-                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags};
+                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos, &s->cstart};
     for (auto *b : bufs) b->release();
     delete s;
     return ND_OK;

@@ -14,27 +14,8 @@
 
 namespace {
 
-// s = sum((a - b).^2) as StaticArrays folds it: ((dx*dx + dy*dy) + dz*dz).
-// The file is compiled with -ffp-contract=off, so no multiply-add is fused.
-__device__ __forceinline__ float sqdist(float ax, float ay, float az, float bx, float by,
-                                        float bz) {
-    const float dx = ax - bx, dy = ay - by, dz = az - bz;
-    const float sx = dx * dx, sy = dy * dy, sz = dz * dz;
-    float s = sx + sy;
-    s = s + sz;
-    return s;
-}
-
-// ND_PRED_SCALAR: s < fl(r*r) then d = sqrt(s) (BVHTraverse.jl:1027,1048).
-// ND_PRED_SIMD:   d = sqrt(s), d < r            (NaiveSIMD.jl:529,589).
-__device__ __forceinline__ bool accept(float s, float r, float r2, int32_t pred, float &d) {
-    if (pred == ND_PRED_SCALAR) {
-        if (s < r2) { d = sqrtf(s); return true; }
-        return false;
-    }
-    d = sqrtf(s);
-    return d < r;
-}
+using nd::accept;
+using nd::sqdist;
 
 struct Box { float mn[3], mx[3]; int32_t left, skip; };
 
@@ -245,6 +226,10 @@ k_bruteforce(const float *__restrict__ xyz, int32_t n, float r, int64_t *__restr
     if (!EMIT) counts[i] = c;
 }
 
+}  // namespace
+
+namespace nd {
+
 int64_t scan_counts(nd_ctx *ctx, int64_t n) {
     // counts in ctx->counts (n), offsets to ctx->offsets (n+1); returns total.
     k_scan_i64<<<1, 1024, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, ctx->offsets.as<int64_t>());
@@ -255,10 +240,6 @@ int64_t scan_counts(nd_ctx *ctx, int64_t n) {
     ND_HIP(hipStreamSynchronize(ctx->stream));
     return total;
 }
-
-}  // namespace
-
-namespace nd {
 
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
                       const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t pred,
@@ -427,7 +408,7 @@ int nd_unique_threshold_pairs(nd_ctx *ctx, const float *d_xyz, int32_t n, float 
     k_bruteforce<false><<<nd::grid_for(n), nd::kBlock, 0, ctx->stream>>>(
         d_xyz, n, threshold, ctx->counts.as<int64_t>(), nullptr, nullptr);
     ND_HIP(hipGetLastError());
-    const int64_t total = scan_counts(ctx, n);
+    const int64_t total = nd::scan_counts(ctx, n);
     *count = total;
     if (total > cap) throw nd::Error(ND_ERR_CAPACITY, "pair buffer too small; *count = required");
     k_bruteforce<true><<<nd::grid_for(n), nd::kBlock, 0, ctx->stream>>>(

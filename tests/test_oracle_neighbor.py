@@ -7,7 +7,7 @@ own tests do: test/BVHTraverse.jl:205-209, 222-289) and tree invariants.
 """
 import numpy as np
 import pytest
-from conftest import pairs_to_rows, rows_canonical
+from conftest import bruteforce_np, pairs_to_rows, rows_canonical
 
 POSITION8 = np.array([[0.1, 0.1, 0.1], [0.2, 0.2, 0.2], [0.346, 0.98, 0.12],
                       [0.01, 0.76, 0.99], [0.1111, 0.4, 0.31], [0.234, 0.29, 0.2],
@@ -287,3 +287,17 @@ def test_morton_hilbert_uses_the_interleaved_cells(O):
     t = (x - lo) * sc
     cells = np.where(t <= 0, 0, np.where(t >= 1023, 1023, np.floor(t))).astype(np.int64)
     assert np.array_equal(O.morton_hilbert(x, lo, sc), O.hilbert_cells(cells))
+
+
+def test_numpy_bruteforce_helper_matches_oracle(O):
+    """The tests' definition-level brute force (both predicates) against the
+    oracle's brute force (sqrt(s) < r, AllToAll.jl:65-88) and its scalar-test
+    traversal on the margin set, where the two predicates disagree."""
+    r, pts = margin_set()
+    simd = bruteforce_np(pts, r, 1)
+    assert np.array_equal(simd, pairs_to_rows(O.bruteforce(pts, r)))
+    ps, _, _ = O.leafbuild_traverse_bvh(pts, 2, r, predicate=O.PRED_SCALAR)
+    assert np.array_equal(bruteforce_np(pts, r, 0), rows_canonical(pairs_to_rows(ps)))
+    assert len(bruteforce_np(pts, r, 0)) != len(simd)
+    x = (np.random.default_rng(3).random((700, 3)) * 6.0).astype(np.float32)
+    assert np.array_equal(bruteforce_np(x, 1.0, 1), pairs_to_rows(O.bruteforce(x, 1.0)))
