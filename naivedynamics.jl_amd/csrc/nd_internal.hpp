@@ -71,6 +71,7 @@ struct nd_ctx {
     nd::DevBuf codes, codes_sorted, perm, xyz_sorted, nodes, store, counts, offsets, xyz_in,
         pairs_out, bbox;
     nd::DevBuf cell_keys, cell_vals, cell_pos, cell_start;  // nd_cell_neighborlist
+    nd::DevBuf trav_rec;  // four-wide records of the one-call traversal
     nd::SortBufs sort;
     // last tree built by nd_leafbuild_traverse_bvh
     int32_t tree_n = 0, tree_apl = 0;
@@ -133,9 +134,11 @@ void launch_list_walk(hipStream_t st, const nd_node *nodes, float4 *rec, const f
                       int32_t *ctr);
 
 // nd_traverse.hip
+// own_tree: the tree was built by launch_lbvh (well formed: left pointers and
+// ropes describe one depth-first order), which allows the wave-cooperative walk.
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
                       const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t predicate,
-                      nd_pair *out, int64_t cap);
+                      nd_pair *out, int64_t cap, bool own_tree = false);
 // Exclusive scan of ctx->counts (n int64) into ctx->offsets (n + 1); synchronises
 // and returns the total.
 int64_t scan_counts(nd_ctx *ctx, int64_t n);
@@ -252,6 +255,13 @@ __device__ __forceinline__ bool accept(float s, float r, float r2, int32_t pred,
     }
     d = sqrtf(s);
     return d < r;
+}
+
+// Order this wave's LDS writes before its later LDS reads (other lanes' data).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // q = round(Int32, v, RoundDown): defined for -2^31 <= v < 2^31 only.

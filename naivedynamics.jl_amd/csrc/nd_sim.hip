@@ -778,12 +778,7 @@ k_nlist2(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     }
 }
 
-// Order this wave's LDS writes before its later LDS reads (other lanes' data).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+using nd::wave_lds_sync;
 
 // Two-phase walk over four-wide records for leaves of A atoms (A = 4, 8, 16,
 // 32).  Every internal node has a record holding its grandchildren (a child

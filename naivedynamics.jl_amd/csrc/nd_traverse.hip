@@ -16,6 +16,7 @@ namespace {
 
 using nd::accept;
 using nd::sqdist;
+using nd::wave_lds_sync;
 
 struct Box { float mn[3], mx[3]; int32_t left, skip; };
 
@@ -176,6 +177,216 @@ k_leaf_traverse4(const nd_node *__restrict__ nodes, const float *__restrict__ xy
     if (!EMIT) counts[q - 1] = sink.n;
 }
 
+// ---- wave-cooperative form (trees built by this library) ----
+//
+// For a well-formed tree the rope walk of query leaf q visits, in increasing
+// leaf order, exactly the leaves t > q whose boxes strictly overlap q's: it
+// starts at q's rope, every later node is reached unless an ancestor after q
+// fails the test, and an ancestor's box (the exact union of its leaves')
+// overlaps whenever one of its leaves' does.  The root override never
+// matters (no rope leads to the root).  So one wave of 64 query atoms (64/A
+// leaves, lane = row a of leaf q) walks the tree once, depth first and left
+// to right over four-wide records, each stack entry carrying the lanes whose
+// leaf box overlaps it and whose leaf precedes the entry's last leaf.  Popped
+// leaves come in increasing order; a popped leaf t is tile-tested by the
+// lanes of every q < t in its mask, and each lane places its row of the tile
+// after the rows of the lanes before it in its leaf.  The output equals the
+// rope walk's pair for pair, in order.
+
+constexpr int kTravStack = 128;  // >= 3 * 32 record levels + 4 (depth <= 63)
+
+// rec[8k .. 8k+7] for internal node nL+1+k: the node's grandchildren in left
+// to right order (a leaf child stands for itself) as structure-of-arrays boxes
+// (min x, y, z, max x, y, z), their node numbers (0 = empty slot), and the
+// last leaf of each entry's subtree (the leaf before the first leaf of its
+// rope target; nL for a rope to 0).
+__global__ void __launch_bounds__(nd::kBlock)
+k_trav_records(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__ rec) {
+    const int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= nL - 1) return;
+    const int32_t nn = 2 * nL - 1;
+    int32_t id[4] = {0, 0, 0, 0}, last[4] = {0, 0, 0, 0};
+    int32_t m = 0;
+    const int32_t L = nodes[nL + k].left;
+    const int32_t R = (L >= 1 && L <= nn) ? nodes[L - 1].skip : 0;
+    const int32_t ch[2] = {L, R};
+    for (int c = 0; c < 2; ++c) {
+        const int32_t x = ch[c];
+        if (x < 1 || x > nn) continue;
+        if (x <= nL) {
+            id[m++] = x;
+        } else {
+            const int32_t g0 = nodes[x - 1].left;
+            const int32_t g1 = (g0 >= 1 && g0 <= nn) ? nodes[g0 - 1].skip : 0;
+            if (g0 >= 1 && g0 <= nn) id[m++] = g0;
+            if (g1 >= 1 && g1 <= nn) id[m++] = g1;
+        }
+    }
+    float b[6][4];
+    for (int e = 0; e < 4; ++e) {
+        const int32_t x = id[e];
+        if (x != 0) {
+            const nd_node &q = nodes[x - 1];
+            b[0][e] = q.min[0]; b[1][e] = q.min[1]; b[2][e] = q.min[2];
+            b[3][e] = q.max[0]; b[4][e] = q.max[1]; b[5][e] = q.max[2];
+            if (x <= nL) {
+                last[e] = x;
+            } else {
+                // leftmost leaf of the rope target, minus one (nL if the
+                // descent does not end in a leaf: no pruning, still exact)
+                int32_t y = q.skip, steps = 0;
+                while (y > nL && y <= nn && steps++ < 64) y = nodes[y - 1].left;
+                last[e] = q.skip == 0 ? nL : ((y >= 1 && y <= nL) ? y - 1 : nL);
+            }
+        } else {
+            b[0][e] = b[1][e] = b[2][e] = INFINITY;
+            b[3][e] = b[4][e] = b[5][e] = -INFINITY;
+        }
+    }
+    float4 *r = rec + 8 * (int64_t)k;
+    for (int d = 0; d < 6; ++d) r[d] = make_float4(b[d][0], b[d][1], b[d][2], b[d][3]);
+    r[6] = make_float4(__int_as_float(id[0]), __int_as_float(id[1]), __int_as_float(id[2]),
+                       __int_as_float(id[3]));
+    r[7] = make_float4(__int_as_float(last[0]), __int_as_float(last[1]), __int_as_float(last[2]),
+                       __int_as_float(last[3]));
+}
+
+template <bool EMIT, int A>
+__global__ void __launch_bounds__(nd::kBlock)
+k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ rec,
+                  const float *__restrict__ xyz, const int32_t *__restrict__ idx, int32_t nL,
+                  float r, float r2, int32_t pred, int64_t *__restrict__ counts,
+                  const int64_t *__restrict__ offsets, nd_pair *__restrict__ out,
+                  int32_t *__restrict__ status) {
+    static_assert(A >= 1 && A <= 64 && (64 % A) == 0, "leaf size must divide 64");
+    constexpr int kW = nd::kBlock / 64;
+    __shared__ int32_t s_node[kW][kTravStack];
+    __shared__ uint64_t s_mask[kW][kTravStack];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool active = s < nL * A;
+    const int32_t Q = s / A + 1;  // query leaf (1-based)
+    const int a = s % A;          // row of this lane in its leaf
+    const int gbase = lane - a;   // first lane of the leaf's group
+    Box ql;
+    if (active) {
+        ql = load_node(nodes, Q);
+    } else {
+        ql.mn[0] = ql.mn[1] = ql.mn[2] = INFINITY;
+        ql.mx[0] = ql.mx[1] = ql.mx[2] = -INFINITY;
+    }
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    int32_t oi = 0;
+    if (active) {
+        ax = xyz[3 * (int64_t)s];
+        ay = xyz[3 * (int64_t)s + 1];
+        az = xyz[3 * (int64_t)s + 2];
+        if (EMIT) oi = idx[s];
+    }
+    int64_t run = 0, limit = 0;  // pairs of leaf Q placed so far (the same in the group)
+    nd_pair *o = nullptr;
+    if (EMIT && active) {
+        o = out + offsets[Q - 1];
+        limit = offsets[Q] - offsets[Q - 1];
+    }
+    // Tile rows: bit b of hits = pair (a, b) of this lane's row against the
+    // atoms tlo .. tlo+A-1 accepted (b > a only on the own leaf).  Called
+    // by the whole wave (the group scan shuffles); lanes with part == false
+    // contribute nothing.
+    auto tile = [&](bool part, int64_t tlo, int b0) {
+        uint64_t hits = 0;
+        float d;
+        if (part) {
+            for (int b = 0; b < A; ++b) {
+                const float bx = xyz[3 * (tlo + b)], by = xyz[3 * (tlo + b) + 1], bz = xyz[3 * (tlo + b) + 2];
+                if (b >= b0 && accept(sqdist(ax, ay, az, bx, by, bz), r, r2, pred, d))
+                    hits |= (uint64_t)1 << b;
+            }
+        }
+        const int32_t c = __builtin_popcountll(hits);
+        int32_t x = c;
+#pragma unroll
+        for (int off = 1; off < A; off <<= 1) {
+            const int32_t v = __shfl_up(x, off, 64);
+            if (a >= off) x += v;
+        }
+        const int32_t total = __shfl(x, gbase + A - 1, 64);
+        if (EMIT && hits) {
+            int64_t pos = run + (x - c);
+            while (hits) {
+                const int b = __builtin_ctzll(hits);
+                hits &= hits - 1;
+                const float s2 = sqdist(ax, ay, az, xyz[3 * (tlo + b)], xyz[3 * (tlo + b) + 1],
+                                        xyz[3 * (tlo + b) + 2]);
+                accept(s2, r, r2, pred, d);
+                if (pos < limit) {
+                    nd_pair p;
+                    p.i = oi;
+                    p.j = idx[tlo + b];
+                    p.d = d;
+                    o[pos] = p;
+                }
+                ++pos;
+            }
+        }
+        run += total;
+    };
+    // same-leaf pairs (a, b > a), rows in order
+    tile(active, (int64_t)(Q - 1) * A, a + 1);
+
+    const uint64_t act = __builtin_amdgcn_ballot_w64(active);
+    if (lane == 0) {
+        s_node[w][0] = nL + 1;  // the root is internal (nL >= 2)
+        s_mask[w][0] = act;
+    }
+    int32_t sp = act ? 1 : 0, pops = 0;
+    bool bad = false;
+    const uint32_t nn = (uint32_t)(2 * nL - 1);
+    wave_lds_sync();
+    while (sp > 0) {
+        --sp;
+        const int32_t node = __builtin_amdgcn_readfirstlane(s_node[w][sp]);
+        const uint64_t mask = s_mask[w][sp];
+        wave_lds_sync();  // the popped slot is read before a push reuses it
+        if (++pops > 2 * nL) { bad = true; break; }
+        const bool mine = (mask >> lane) & 1u;
+        if (node > nL) {
+            const float4 *rc = rec + 8 * (int64_t)(node - nL - 1);
+            const float4 mnx = rc[0], mny = rc[1], mnz = rc[2], mxx = rc[3], mxy = rc[4], mxz = rc[5];
+            const float4 ids = rc[6], lst = rc[7];
+            auto push = [&](float mx0, float my0, float mz0, float mx1, float my1, float mz1, float idf,
+                            float lf) {
+                const int32_t c = __float_as_int(idf);
+                if (c == 0) return;
+                if ((uint32_t)(c - 1) >= nn || sp >= kTravStack) { bad = true; return; }
+                const bool ov = mine & (__float_as_int(lf) > Q) & (ql.mn[0] < mx1) & (ql.mx[0] > mx0) &
+                                (ql.mn[1] < my1) & (ql.mx[1] > my0) & (ql.mn[2] < mz1) & (ql.mx[2] > mz0);
+                const uint64_t mk = __builtin_amdgcn_ballot_w64(ov);
+                if (mk) {
+                    if (lane == 0) {
+                        s_node[w][sp] = c;
+                        s_mask[w][sp] = mk;
+                    }
+                    ++sp;
+                }
+            };
+            // an internal node has two children, so a record has >= 2 entries
+            if (__float_as_int(ids.y) == 0) { bad = true; break; }
+            // last entry first: entry 0 (leftmost) ends on top
+            push(mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, ids.w, lst.w);
+            push(mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, ids.z, lst.z);
+            push(mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, ids.y, lst.y);
+            push(mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, ids.x, lst.x);
+            if (bad) break;
+            wave_lds_sync();
+        } else {
+            tile(mine & (Q < node), (int64_t)(node - 1) * A, 0);
+        }
+    }
+    if (bad && lane == 0) atomicOr(status, nd::kStatusTree);
+    if (!EMIT && active && a == 0) counts[Q - 1] = run;
+}
+
 // Exclusive scan of n int64 counts into offsets[0..n]; offsets[n] = total.
 __global__ void __launch_bounds__(1024)
 k_scan_i64(const int64_t *__restrict__ in, int64_t n, int64_t *__restrict__ out) {
@@ -241,9 +452,30 @@ int64_t scan_counts(nd_ctx *ctx, int64_t n) {
     return total;
 }
 
+namespace {
+
+template <bool EMIT>
+void launch_traverse_w(hipStream_t st, int32_t apl, const nd_node *nodes, const float4 *rec,
+                       const float *xyz, const int32_t *idx, int32_t nL, float r, float r2, int32_t pred,
+                       int64_t *cnt, const int64_t *off, nd_pair *out, int32_t *status) {
+    const unsigned g = grid_for((int64_t)nL * apl);
+#define ND_TRAV_W(A)                                                                                 \
+    case A:                                                                                          \
+        k_leaf_traverse_w<EMIT, A><<<g, kBlock, 0, st>>>(nodes, rec, xyz, idx, nL, r, r2, pred, cnt, \
+                                                         off, out, status);                          \
+        break;
+    switch (apl) {
+        ND_TRAV_W(1) ND_TRAV_W(2) ND_TRAV_W(4) ND_TRAV_W(8) ND_TRAV_W(16) ND_TRAV_W(32) ND_TRAV_W(64)
+        default: throw Error(ND_ERR_ARG, "cooperative traversal: leaf size must divide 64");
+    }
+#undef ND_TRAV_W
+}
+
+}  // namespace
+
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
                       const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t pred,
-                      nd_pair *out, int64_t cap) {
+                      nd_pair *out, int64_t cap, bool own_tree) {
     const int32_t nL = n / apl;
     const float r2 = r * r;  // (neighbor_distance)^2 in Float32
     ctx->counts.reserve(sizeof(int64_t) * (size_t)(nL + 1));
@@ -251,6 +483,24 @@ int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted
     int64_t *cnt = ctx->counts.as<int64_t>();
     const int64_t *off = ctx->offsets.as<int64_t>();
     int32_t *st = ctx->status.as<int32_t>();
+    if (own_tree && apl <= 64 && 64 % apl == 0) {
+        // one depth-first walk per wave (same pairs, same order)
+        ctx->trav_rec.reserve(8 * sizeof(float4) * (size_t)(nL - 1));
+        float4 *rec = ctx->trav_rec.as<float4>();
+        k_trav_records<<<grid_for(nL - 1), kBlock, 0, ctx->stream>>>(nodes, nL, rec);
+        launch_traverse_w<false>(ctx->stream, apl, nodes, rec, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt,
+                                 off, nullptr, st);
+        ND_HIP(hipGetLastError());
+        const int64_t total = scan_counts(ctx, nL);
+        if (nd::read_status(ctx) & nd::kStatusTree)
+            throw nd::Error(ND_ERR_STATE, "leafneighbor_traverse: malformed tree");
+        if (total > cap || out == nullptr) return total;
+        launch_traverse_w<true>(ctx->stream, apl, nodes, rec, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt,
+                                off, out, st);
+        ND_HIP(hipGetLastError());
+        ND_HIP(hipStreamSynchronize(ctx->stream));
+        return total;
+    }
     if (apl == 4)
         k_leaf_traverse4<false><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
             nodes, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt, off, nullptr, st);
@@ -349,7 +599,7 @@ int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec
     ctx->tree_apl = apl;
     const int64_t total = nd::leaf_traverse(ctx, ctx->nodes.as<nd_node>(), ctx->xyz_sorted.as<float>(),
                                             ctx->perm.as<int32_t>(), n, apl, chk.neighbor_distance,
-                                            predicate, d_out, cap);
+                                            predicate, d_out, cap, true);
     *count = total;
     if (total > cap) throw nd::Error(ND_ERR_CAPACITY, "pair buffer too small; *count = required");
     ND_API_END(ctx)

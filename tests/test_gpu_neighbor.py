@@ -120,6 +120,16 @@ def test_apl_sweep_vs_oracle(pkg, O, apl):
         _compare_with_oracle(pkg, O, x, apl, 0.1, pkg.PRED_SIMD)
 
 
+@pytest.mark.parametrize("apl", [8, 16, 32, 64])
+def test_wave_walk_leaf_sizes_vs_oracle(pkg, O, apl):
+    """Leaf sizes dividing 64 take the one-walk-per-wave traversal (64 / apl
+    query leaves per wave); the pairs and their order are the rope walk's."""
+    L = (4096 / 0.8) ** (1 / 3)
+    x = _rand(4096, L, 19 + apl)
+    for pred in (pkg.PRED_SCALAR, pkg.PRED_SIMD):
+        _compare_with_oracle(pkg, O, x, apl, 1.5, pred)
+
+
 @pytest.mark.parametrize("shape", ["duplicates", "line", "plane", "two_clusters"])
 @pytest.mark.parametrize("morton", [0, 1])
 def test_degenerate_inputs_vs_oracle(pkg, O, shape, morton):
