@@ -108,14 +108,16 @@ def measured_traffic():
 
 def search_comparison(pkg, c):
     """The one-call LBVH neighbour list (interleaved key, 4 atoms per leaf)
-    against the cell list (nd_cell_neighborlist) on the same 262,144 positions,
-    r = r_list, unique pairs with s < r^2; median wall time of 5 calls each
-    (inputs resident on the device, results left there)."""
-    x = np.ascontiguousarray(c["x"], np.float32)
-    ctx = pkg.default_context()
-    d = pkg.DeviceArray.from_host(ctx, x)
+    against the cell list (nd_cell_neighborlist), N = 262,144, r = r_list,
+    unique pairs with s < r^2, on the config-2 initial positions (a jittered
+    lattice) and on uniform random positions at the same density (the
+    reference benchmarks its searches on uniform points); median wall time of
+    5 calls each, inputs resident on the device, results left there."""
     r = float(np.float32(c["rc"] + c["skin"]))
-    spec = pkg.SpheresBVHSpecs(neighbor_distance=r, atom_count=len(x), atomsperleaf=4)
+    n = len(c["x"])
+    L = (n / 0.8) ** (1 / 3)
+    ctx = pkg.default_context()
+    spec = pkg.SpheresBVHSpecs(neighbor_distance=r, atom_count=n, atomsperleaf=4)
 
     def timed(fn):
         count = fn()
@@ -126,11 +128,16 @@ def search_comparison(pkg, c):
             ts.append(time.perf_counter() - t0)
         return float(np.median(ts)) * 1e3, count
 
-    ms_l, n_l = timed(lambda: pkg.neighborlist_device(d, spec, pkg.PRED_SCALAR, pkg.MORTON_INTERLEAVE)[1])
-    ms_c, n_c = timed(lambda: pkg.cells.neighborlist_device(d, r, pkg.PRED_SCALAR)[1])
-    log("search: LBVH %.3f ms (%d pairs), cell list %.3f ms (%d pairs)" % (ms_l, n_l, ms_c, n_c))
-    return {"workload": "ljc-262144 initial positions", "r": r, "lbvh_ms": round(ms_l, 3),
-            "lbvh_pairs": int(n_l), "cell_list_ms": round(ms_c, 3), "cell_list_pairs": int(n_c)}
+    out = {"n": n, "r": r}
+    for name, x in (("lattice", np.ascontiguousarray(c["x"], np.float32)),
+                    ("uniform", (np.random.default_rng(7).random((n, 3)) * L).astype(np.float32))):
+        d = pkg.DeviceArray.from_host(ctx, x)
+        ms_l, n_l = timed(lambda: pkg.neighborlist_device(d, spec, pkg.PRED_SCALAR, pkg.MORTON_INTERLEAVE)[1])
+        ms_c, n_c = timed(lambda: pkg.cells.neighborlist_device(d, r, pkg.PRED_SCALAR)[1])
+        log("search %s: LBVH %.3f ms (%d pairs), cell list %.3f ms (%d pairs)" % (name, ms_l, n_l, ms_c, n_c))
+        out[name] = {"lbvh_ms": round(ms_l, 3), "lbvh_pairs": int(n_l), "cell_list_ms": round(ms_c, 3),
+                     "cell_list_pairs": int(n_c)}
+    return out
 
 
 def precision_form(pkg, steps=500):

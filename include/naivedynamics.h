@@ -164,6 +164,9 @@ int nd_unique_threshold_pairs(nd_ctx *ctx, const float *d_xyz, int32_t n, float 
  * Capacity protocol as above. */
 int nd_cell_neighborlist(nd_ctx *ctx, const float *d_xyz, int32_t n, float cutoff,
                          int32_t predicate, nd_pair *d_out, int64_t cap, int64_t *count);
+/* Host-buffer form of the same call (stages h_xyz in and pairs out). */
+int nd_cell_neighborlist_host(nd_ctx *ctx, const float *h_xyz, int32_t n, float cutoff,
+                              int32_t predicate, nd_pair *h_out, int64_t cap, int64_t *count);
 
 /* ---- forces over a pair list --------------------------------------------- */
 typedef struct {

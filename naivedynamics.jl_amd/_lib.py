@@ -115,6 +115,7 @@ SIGNATURES = {
     "nd_tree_export": [_P, _P, _P, _P],
     "nd_unique_threshold_pairs": [_P, _P, _i32, _f32, _P, _i64, ctypes.POINTER(_i64)],
     "nd_cell_neighborlist": [_P, _P, _i32, _f32, _i32, _P, _i64, ctypes.POINTER(_i64)],
+    "nd_cell_neighborlist_host": [_P, _P, _i32, _f32, _i32, _P, _i64, ctypes.POINTER(_i64)],
     "nd_force_lennardjones": [_P, _P, _P, _i64, _P, _i32, ctypes.POINTER(ForceParams),
                               ctypes.POINTER(ctypes.c_double)],
     "nd_force_coulomb": [_P, _P, _P, _i64, _P, _P, _i32, ctypes.POINTER(ForceParams),

@@ -261,4 +261,24 @@ int nd_cell_neighborlist(nd_ctx *ctx, const float *d_xyz, int32_t n, float cutof
     ND_API_END(ctx)
 }
 
+int nd_cell_neighborlist_host(nd_ctx *ctx, const float *h_xyz, int32_t n, float cutoff,
+                              int32_t predicate, nd_pair *h_out, int64_t cap, int64_t *count) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && count && n >= 0 && (h_xyz || n == 0), "nd_cell_neighborlist_host: bad argument");
+    ND_HIP(hipSetDevice(ctx->device));
+    ctx->xyz_in.reserve(sizeof(float) * 3 * (size_t)n);
+    if (n > 0)
+        ND_HIP(hipMemcpyAsync(ctx->xyz_in.p, h_xyz, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice,
+                              ctx->stream));
+    if (cap > 0) ctx->pairs_out.reserve(sizeof(nd_pair) * (size_t)cap);
+    const int rc = nd_cell_neighborlist(ctx, ctx->xyz_in.as<float>(), n, cutoff, predicate,
+                                        cap > 0 ? ctx->pairs_out.as<nd_pair>() : nullptr, cap, count);
+    if (rc != ND_OK) return rc;
+    if (*count > 0 && h_out)
+        ND_HIP(hipMemcpyAsync(h_out, ctx->pairs_out.p, sizeof(nd_pair) * (size_t)*count, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    ND_API_END(ctx)
+}
+
 }  // extern "C"

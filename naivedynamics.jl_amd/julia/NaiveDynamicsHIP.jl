@@ -14,7 +14,7 @@ module NaiveDynamicsHIP
 using NaiveDynamics
 using StaticArrays
 
-export HIPBackend
+export HIPBackend, cell_neighborlist
 
 const libnaivehip = get(ENV, "NAIVEHIP_LIB", "libnaivehip")
 
@@ -116,6 +116,28 @@ function NaiveDynamics.gpubvh_neighborlist(b::HIPBackend, position, spec)
     keys = [GridKey{Float32,Int32}(n.min, n.max, n.left, n.skip) for n in tree]
     prim = APointPrimitive{Float32,Int32}(perm .+ Int32(1), codes, [SVector{3,Float32}(position[p + 1]) for p in perm])
     return (pairlist = pairs, treedata = (tree = keys, mortonpositions = prim, quantizedposition = nothing, store = nothing))
+end
+
+# neighborlist(x, cutoff) of CellListMap (assets/BVHBenchSuite.jl:307, src/MDInput.jl:290):
+# the uniform cell-list search, (i, j, d) with i < j, 1-based; sqrt(s) < cutoff
+function cell_neighborlist(b::HIPBackend, position, cutoff::Real; predicate::Int32 = Int32(1))
+    xyz = packxyz(position)
+    n = Int32(size(xyz, 2))
+    cap = Int64(48) * n + 1024
+    while true
+        out = Vector{NTuple{3,UInt32}}(undef, cap)
+        count = Ref{Int64}(0)
+        rc = ccall((:nd_cell_neighborlist_host, libnaivehip), Cint,
+                   (Ptr{Cvoid}, Ptr{Float32}, Int32, Float32, Int32, Ptr{Cvoid}, Int64, Ref{Int64}),
+                   b.handle, xyz, n, Float32(cutoff), predicate, out, cap, count)
+        if rc == ND_ERR_CAPACITY
+            cap = count[] + count[] ÷ 8 + 1024
+            continue
+        end
+        check(b, rc, "neighborlist")
+        return [(reinterpret(Int32, t[1]) + Int32(1), reinterpret(Int32, t[2]) + Int32(1),
+                 reinterpret(Float32, t[3])) for t in view(out, 1:count[])]
+    end
 end
 
 # force_lennardjones!(force, pairslist, position) (Forces.jl:15-45), textbook LJ

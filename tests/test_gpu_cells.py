@@ -121,6 +121,20 @@ def test_capacity_protocol_and_deterministic_order(pkg, O):
     assert all(np.array_equal(rows[0], r) for r in rows[1:])
 
 
+def test_host_buffer_entry_point(pkg, O):
+    x = _rand(3000, 1.0, 48)
+    ref = pairs_to_rows(O.bruteforce(x, 0.1))
+    out = np.zeros(len(ref) + 5, pkg.PAIR_DTYPE)
+    cnt = ctypes.c_int64(0)
+    ctx = pkg.default_context()
+    rc = pkg.library().nd_cell_neighborlist_host(ctx.handle, x.ctypes.data_as(ctypes.c_void_p), len(x),
+                                                 np.float32(0.1), pkg.PRED_SIMD,
+                                                 out.ctypes.data_as(ctypes.c_void_p), len(out),
+                                                 ctypes.byref(cnt))
+    assert rc == 0 and cnt.value == len(ref)
+    assert np.array_equal(rows_canonical(pairs_to_rows(out[: cnt.value])), ref)
+
+
 def test_full_size_equals_gpu_bruteforce(pkg):
     """Property at N = 262,144 (density 0.8, r = 2.8): the cell list and the
     O(N^2) brute-force kernel give the same canonical set and distance bits."""
