@@ -61,7 +61,7 @@ int nd_ctx_destroy(nd_ctx *ctx) {
                           &ctx->offsets, &ctx->xyz_in, &ctx->pairs_out, &ctx->bbox,
                           &ctx->sort.keys_alt, &ctx->sort.vals_alt, &ctx->sort.hist,
                           &ctx->sort.flags, &ctx->cell_keys, &ctx->cell_vals, &ctx->cell_pos,
-                          &ctx->cell_start, &ctx->trav_rec};
+                          &ctx->cell_start, &ctx->trav_rec, &ctx->scan_tmp};
     for (auto *b : bufs) b->release();
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);

@@ -213,8 +213,15 @@ k_keybits(const int32_t *__restrict__ keys, int32_t n, uint32_t *__restrict__ ou
         o |= k;
         a &= k;
     }
-    atomicOr(&out[0], o);
-    atomicAnd(&out[1], a);
+    // one atomic pair per wave, not per lane (they all hit one address)
+    for (int off = 32; off > 0; off >>= 1) {
+        o |= (uint32_t)__shfl_xor((int)o, off, 64);
+        a &= (uint32_t)__shfl_xor((int)a, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicOr(&out[0], o);
+        atomicAnd(&out[1], a);
+    }
 }
 
 __global__ void k_keybits_init(uint32_t *out) {

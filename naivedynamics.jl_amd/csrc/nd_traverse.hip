@@ -251,20 +251,41 @@ k_trav_records(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict
                        __int_as_float(last[3]));
 }
 
-template <bool EMIT, int A>
+template <int PRED>
+__device__ __forceinline__ bool accept_t(float s, float r, float r2, float &d) {
+    if constexpr (PRED == ND_PRED_SCALAR) {
+        if (s < r2) { d = sqrtf(s); return true; }
+        return false;
+    } else {
+        d = sqrtf(s);
+        return d < r;
+    }
+}
+
+template <bool EMIT, int A, int PRED>
 __global__ void __launch_bounds__(nd::kBlock)
 k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ rec,
                   const float *__restrict__ xyz, const int32_t *__restrict__ idx, int32_t nL,
-                  float r, float r2, int32_t pred, int64_t *__restrict__ counts,
+                  float r, float r2, int32_t nslice, int64_t *__restrict__ counts,
                   const int64_t *__restrict__ offsets, nd_pair *__restrict__ out,
                   int32_t *__restrict__ status) {
     static_assert(A >= 1 && A <= 64 && (64 % A) == 0, "leaf size must divide 64");
+    // query atoms per wave: 32 (the upper half-wave idles) keeps twice the
+    // walks in flight and a smaller union of leaf boxes per walk
+    constexpr int kApw = A <= 32 ? 32 : 64;
     constexpr int kW = nd::kBlock / 64;
-    __shared__ int32_t s_node[kW][kTravStack];
+    __shared__ int32_t s_node[kW][kTravStack], s_first[kW][kTravStack];
     __shared__ uint64_t s_mask[kW][kTravStack];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    const bool active = s < nL * A;
+    // wave g walks slice g % nslice (the leaves t in [tlo, thi]) for the
+    // query atoms of wave group g / nslice: a leaf whose box overlaps many
+    // leaves spreads its walk over nslice waves
+    const int32_t gw = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int32_t slice = gw % nslice;
+    const int32_t tlo = 1 + (int32_t)((int64_t)slice * nL / nslice);
+    const int32_t thi = (int32_t)((int64_t)(slice + 1) * nL / nslice);
+    const int32_t s = (gw / nslice) * kApw + lane;
+    const bool active = lane < kApw && s < nL * A;
     const int32_t Q = s / A + 1;  // query leaf (1-based)
     const int a = s % A;          // row of this lane in its leaf
     const int gbase = lane - a;   // first lane of the leaf's group
@@ -283,26 +304,22 @@ k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ 
         az = xyz[3 * (int64_t)s + 2];
         if (EMIT) oi = idx[s];
     }
-    int64_t run = 0, limit = 0;  // pairs of leaf Q placed so far (the same in the group)
+    // pairs of (leaf Q, this slice) placed so far (the same in the group)
+    int64_t run = 0, limit = 0;
+    const int64_t cslot = (int64_t)(Q - 1) * nslice + slice;
     nd_pair *o = nullptr;
     if (EMIT && active) {
-        o = out + offsets[Q - 1];
-        limit = offsets[Q] - offsets[Q - 1];
+        o = out + offsets[cslot];
+        limit = offsets[cslot + 1] - offsets[cslot];
     }
     // Tile rows: bit b of hits = pair (a, b) of this lane's row against the
     // atoms tlo .. tlo+A-1 accepted (b > a only on the own leaf).  Called
     // by the whole wave (the group scan shuffles); lanes with part == false
     // contribute nothing.
-    auto tile = [&](bool part, int64_t tlo, int b0) {
-        uint64_t hits = 0;
-        float d;
-        if (part) {
-            for (int b = 0; b < A; ++b) {
-                const float bx = xyz[3 * (tlo + b)], by = xyz[3 * (tlo + b) + 1], bz = xyz[3 * (tlo + b) + 2];
-                if (b >= b0 && accept(sqdist(ax, ay, az, bx, by, bz), r, r2, pred, d))
-                    hits |= (uint64_t)1 << b;
-            }
-        }
+    // Places this lane's row of a tile (bit b of hits = pair (a, b) against
+    // atoms tlo .. tlo+A-1) after the rows of the lanes before it in its
+    // leaf, and emits it.  Called by the whole wave (the group scan shuffles).
+    auto place = [&](uint64_t hits, int64_t tlo) {
         const int32_t c = __builtin_popcountll(hits);
         int32_t x = c;
 #pragma unroll
@@ -316,9 +333,10 @@ k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ 
             while (hits) {
                 const int b = __builtin_ctzll(hits);
                 hits &= hits - 1;
-                const float s2 = sqdist(ax, ay, az, xyz[3 * (tlo + b)], xyz[3 * (tlo + b) + 1],
-                                        xyz[3 * (tlo + b) + 2]);
-                accept(s2, r, r2, pred, d);
+                float d;
+                accept_t<PRED>(sqdist(ax, ay, az, xyz[3 * (tlo + b)], xyz[3 * (tlo + b) + 1],
+                                      xyz[3 * (tlo + b) + 2]),
+                               r, r2, d);
                 if (pos < limit) {
                     nd_pair p;
                     p.i = oi;
@@ -331,21 +349,63 @@ k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ 
         }
         run += total;
     };
-    // same-leaf pairs (a, b > a), rows in order
-    tile(active, (int64_t)(Q - 1) * A, a + 1);
+    // same-leaf pairs (a, b > a), rows in order, in the first slice
+    if (slice == 0) {
+        uint64_t hits = 0;
+        const int64_t qlo = (int64_t)(Q - 1) * A;
+        if (active)
+            for (int b = a + 1; b < A; ++b) {
+                float d;
+                if (accept_t<PRED>(sqdist(ax, ay, az, xyz[3 * (qlo + b)], xyz[3 * (qlo + b) + 1],
+                                          xyz[3 * (qlo + b) + 2]),
+                                   r, r2, d))
+                    hits |= (uint64_t)1 << b;
+            }
+        place(hits, qlo);
+    }
+    // a popped leaf t (wave-uniform): its atoms are fetched together, in
+    // chunks of 4, then tested by the participating lanes
+    auto tile = [&](bool part, int32_t t) {
+        const int64_t t0 = (int64_t)(t - 1) * A;
+        constexpr int CH = A < 4 ? A : 4;
+        uint64_t hits = 0;
+#pragma unroll
+        for (int c0 = 0; c0 < A; c0 += CH) {
+            float tx[CH], ty[CH], tz[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                tx[k] = xyz[3 * (t0 + c0 + k)];
+                ty[k] = xyz[3 * (t0 + c0 + k) + 1];
+                tz[k] = xyz[3 * (t0 + c0 + k) + 2];
+            }
+            if (part) {
+#pragma unroll
+                for (int k = 0; k < CH; ++k) {
+                    float d;
+                    if (accept_t<PRED>(sqdist(ax, ay, az, tx[k], ty[k], tz[k]), r, r2, d))
+                        hits |= (uint64_t)1 << (c0 + k);
+                }
+            }
+        }
+        place(hits, t0);
+    };
 
     const uint64_t act = __builtin_amdgcn_ballot_w64(active);
     if (lane == 0) {
         s_node[w][0] = nL + 1;  // the root is internal (nL >= 2)
+        s_first[w][0] = 1;
         s_mask[w][0] = act;
     }
     int32_t sp = act ? 1 : 0, pops = 0;
     bool bad = false;
     const uint32_t nn = (uint32_t)(2 * nL - 1);
     wave_lds_sync();
-    while (sp > 0) {
+    while (true) {
+        sp = __builtin_amdgcn_readfirstlane(sp);  // wave-uniform: scalar loop control
+        if (sp <= 0) break;
         --sp;
         const int32_t node = __builtin_amdgcn_readfirstlane(s_node[w][sp]);
+        int32_t first = __builtin_amdgcn_readfirstlane(s_first[w][sp]);  // first leaf of node
         const uint64_t mask = s_mask[w][sp];
         wave_lds_sync();  // the popped slot is read before a push reuses it
         if (++pops > 2 * nL) { bad = true; break; }
@@ -354,17 +414,27 @@ k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ 
             const float4 *rc = rec + 8 * (int64_t)(node - nL - 1);
             const float4 mnx = rc[0], mny = rc[1], mnz = rc[2], mxx = rc[3], mxy = rc[4], mxz = rc[5];
             const float4 ids = rc[6], lst = rc[7];
+            // entries cover consecutive leaf ranges: entry e spans
+            // [first(e), last(e)], first(e+1) = last(e) + 1
+            int32_t efirst[4];
+            efirst[0] = first;
+            efirst[1] = __float_as_int(lst.x) + 1;
+            efirst[2] = __float_as_int(lst.y) + 1;
+            efirst[3] = __float_as_int(lst.z) + 1;
             auto push = [&](float mx0, float my0, float mz0, float mx1, float my1, float mz1, float idf,
-                            float lf) {
+                            float lf, int32_t ef) {
                 const int32_t c = __float_as_int(idf);
+                const int32_t el = __float_as_int(lf);
                 if (c == 0) return;
                 if ((uint32_t)(c - 1) >= nn || sp >= kTravStack) { bad = true; return; }
-                const bool ov = mine & (__float_as_int(lf) > Q) & (ql.mn[0] < mx1) & (ql.mx[0] > mx0) &
+                if (el < tlo || ef > thi) return;  // no leaf of this slice below it
+                const bool ov = mine & (el > Q) & (ql.mn[0] < mx1) & (ql.mx[0] > mx0) &
                                 (ql.mn[1] < my1) & (ql.mx[1] > my0) & (ql.mn[2] < mz1) & (ql.mx[2] > mz0);
                 const uint64_t mk = __builtin_amdgcn_ballot_w64(ov);
                 if (mk) {
                     if (lane == 0) {
                         s_node[w][sp] = c;
+                        s_first[w][sp] = ef;
                         s_mask[w][sp] = mk;
                     }
                     ++sp;
@@ -373,18 +443,72 @@ k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ 
             // an internal node has two children, so a record has >= 2 entries
             if (__float_as_int(ids.y) == 0) { bad = true; break; }
             // last entry first: entry 0 (leftmost) ends on top
-            push(mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, ids.w, lst.w);
-            push(mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, ids.z, lst.z);
-            push(mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, ids.y, lst.y);
-            push(mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, ids.x, lst.x);
+            push(mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, ids.w, lst.w, efirst[3]);
+            push(mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, ids.z, lst.z, efirst[2]);
+            push(mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, ids.y, lst.y, efirst[1]);
+            push(mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, ids.x, lst.x, efirst[0]);
             if (bad) break;
             wave_lds_sync();
         } else {
-            tile(mine & (Q < node), (int64_t)(node - 1) * A, 0);
+            tile(mine & (Q < node), node);
         }
     }
     if (bad && lane == 0) atomicOr(status, nd::kStatusTree);
-    if (!EMIT && active && a == 0) counts[Q - 1] = run;
+    if (!EMIT && active && a == 0) counts[cslot] = run;
+}
+
+// Exclusive scan of n int64 counts in tiles of kScanTile: tile sums, a scan
+// of the sums (k_scan_i64), then each tile's scan plus its offset.
+constexpr int kScanPer = 8, kScanTile = nd::kBlock * kScanPer;
+
+__global__ void __launch_bounds__(nd::kBlock)
+k_scan_tile_sums(const int64_t *__restrict__ in, int64_t n, int64_t *__restrict__ sums) {
+    __shared__ int64_t part[nd::kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int64_t i = base + k * nd::kBlock + threadIdx.x;  // coalesced
+        if (i < n) s += in[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < nd::kBlock / 64; ++w) t += part[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(nd::kBlock)
+k_scan_tile_apply(const int64_t *__restrict__ in, int64_t n, const int64_t *__restrict__ tile_off,
+                  int64_t ntiles, int64_t *__restrict__ out) {
+    __shared__ int64_t part[nd::kBlock];
+    const int t = threadIdx.x;
+    const int64_t lo = (int64_t)blockIdx.x * kScanTile + (int64_t)t * kScanPer;
+    int64_t v[kScanPer];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = lo + k < n ? in[lo + k] : 0;
+        s += v[k];
+    }
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < nd::kBlock; off <<= 1) {
+        const int64_t x = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    int64_t run = tile_off[blockIdx.x] + part[t] - s;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        if (lo + k < n) out[lo + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == ntiles - 1 && t == 0) out[n] = tile_off[ntiles];
 }
 
 // Exclusive scan of n int64 counts into offsets[0..n]; offsets[n] = total.
@@ -443,7 +567,17 @@ namespace nd {
 
 int64_t scan_counts(nd_ctx *ctx, int64_t n) {
     // counts in ctx->counts (n), offsets to ctx->offsets (n+1); returns total.
-    k_scan_i64<<<1, 1024, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, ctx->offsets.as<int64_t>());
+    if (n <= 4 * kScanTile) {
+        k_scan_i64<<<1, 1024, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, ctx->offsets.as<int64_t>());
+    } else {
+        const int64_t nt = (n + kScanTile - 1) / kScanTile;
+        ctx->scan_tmp.reserve(sizeof(int64_t) * (size_t)(2 * nt + 1));
+        int64_t *sums = ctx->scan_tmp.as<int64_t>(), *off = sums + nt;
+        k_scan_tile_sums<<<(unsigned)nt, kBlock, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, sums);
+        k_scan_i64<<<1, 1024, 0, ctx->stream>>>(sums, nt, off);
+        k_scan_tile_apply<<<(unsigned)nt, kBlock, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, off, nt,
+                                                                     ctx->offsets.as<int64_t>());
+    }
     ND_HIP(hipGetLastError());
     int64_t total = 0;
     ND_HIP(hipMemcpyAsync(&total, ctx->offsets.as<int64_t>() + n, sizeof(int64_t),
@@ -454,15 +588,24 @@ int64_t scan_counts(nd_ctx *ctx, int64_t n) {
 
 namespace {
 
+// leaf-range slices per wave group of the cooperative walk
+inline int32_t trav_slices(int32_t nL) { return nL >= 64 ? 8 : 1; }
+
 template <bool EMIT>
 void launch_traverse_w(hipStream_t st, int32_t apl, const nd_node *nodes, const float4 *rec,
                        const float *xyz, const int32_t *idx, int32_t nL, float r, float r2, int32_t pred,
                        int64_t *cnt, const int64_t *off, nd_pair *out, int32_t *status) {
-    const unsigned g = grid_for((int64_t)nL * apl);
-#define ND_TRAV_W(A)                                                                                 \
-    case A:                                                                                          \
-        k_leaf_traverse_w<EMIT, A><<<g, kBlock, 0, st>>>(nodes, rec, xyz, idx, nL, r, r2, pred, cnt, \
-                                                         off, out, status);                          \
+    const int32_t ns = trav_slices(nL);
+    const int64_t waves = ((int64_t)nL * apl + (apl <= 32 ? 31 : 63)) / (apl <= 32 ? 32 : 64);
+    const unsigned g = grid_for(waves * ns * 64);
+#define ND_TRAV_W(A)                                                                              \
+    case A:                                                                                       \
+        if (pred == ND_PRED_SCALAR)                                                               \
+            k_leaf_traverse_w<EMIT, A, ND_PRED_SCALAR><<<g, kBlock, 0, st>>>(nodes, rec, xyz, idx, nL, \
+                                                                            r, r2, ns, cnt, off, out, status); \
+        else                                                                                      \
+            k_leaf_traverse_w<EMIT, A, ND_PRED_SIMD><<<g, kBlock, 0, st>>>(nodes, rec, xyz, idx, nL,   \
+                                                                          r, r2, ns, cnt, off, out, status);   \
         break;
     switch (apl) {
         ND_TRAV_W(1) ND_TRAV_W(2) ND_TRAV_W(4) ND_TRAV_W(8) ND_TRAV_W(16) ND_TRAV_W(32) ND_TRAV_W(64)
@@ -484,14 +627,20 @@ int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted
     const int64_t *off = ctx->offsets.as<int64_t>();
     int32_t *st = ctx->status.as<int32_t>();
     if (own_tree && apl <= 64 && 64 % apl == 0) {
-        // one depth-first walk per wave (same pairs, same order)
+        // one depth-first walk per wave and leaf-range slice (same pairs, same
+        // order: counts per (leaf, slice), scanned leaf-major)
+        const int64_t nc = (int64_t)nL * trav_slices(nL);
+        ctx->counts.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+        ctx->offsets.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+        cnt = ctx->counts.as<int64_t>();
+        off = ctx->offsets.as<int64_t>();
         ctx->trav_rec.reserve(8 * sizeof(float4) * (size_t)(nL - 1));
         float4 *rec = ctx->trav_rec.as<float4>();
         k_trav_records<<<grid_for(nL - 1), kBlock, 0, ctx->stream>>>(nodes, nL, rec);
         launch_traverse_w<false>(ctx->stream, apl, nodes, rec, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt,
                                  off, nullptr, st);
         ND_HIP(hipGetLastError());
-        const int64_t total = scan_counts(ctx, nL);
+        const int64_t total = scan_counts(ctx, nc);
         if (nd::read_status(ctx) & nd::kStatusTree)
             throw nd::Error(ND_ERR_STATE, "leafneighbor_traverse: malformed tree");
         if (total > cap || out == nullptr) return total;
