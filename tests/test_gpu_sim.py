@@ -208,6 +208,32 @@ def test_thermostat_moves_temperature(pkg):
     sim.close()
 
 
+@pytest.mark.parametrize("precision,xtol", [("fp32", 2e-4), ("fp64", 1e-10)])
+@pytest.mark.parametrize("key", [0, 2])
+def test_config1_plain_small_vs_oracle(pkg, O, precision, xtol, key):
+    """The small plain configuration in full: 4,096 atoms at density 0.8, LJ
+    only (eps = sigma = 1, rc = 2.5), dt = 0.001, 100 steps with the list
+    rebuilt every step at r_list = rc (no skin), 4 atoms per leaf; the
+    reference's masked key (0) and the Hilbert key (2)."""
+    c = pkg.synth.plain_small(seed=11)
+    sim = pkg.Simulation(c["n"], pkg.SimSpec(duration=c["steps"], stepwidth=c["dt"], rc=c["rc"],
+                                             rc_coul=c["rc_coul"], skin=c["skin"],
+                                             rebuild_every=c["rebuild_every"], lo=0.0, hi=c["L"],
+                                             precision=precision, energy_every=1, atomsperleaf=4,
+                                             morton=key))
+    sim.set_state(c["x"], c["v"], None)
+    sim.run(c["steps"])
+    xg, vg, fg = sim.get_state()
+    xr, vr, fr, E = O.vv_run64(c["x"], c["v"], None, None, O.o64_params(rc_lj=c["rc"]), c["dt"],
+                               c["steps"], walls=True, lo=0.0, hi=c["L"])
+    assert np.abs(xg - xr).max() < xtol
+    assert _rel(fg, fr) < (5e-4 if precision == "fp32" else 1e-10)
+    assert sim.stats()["rebuilds"] == c["steps"] + 1
+    e = sim.energies()
+    assert np.allclose(e[:, 1], E[:, 0], rtol=1e-4 if precision == "fp32" else 1e-10)
+    sim.close()
+
+
 def test_full_size_forces_vs_oracle(pkg, O):
     """N = 262,144 LJ + Coulomb after 20 steps: FP32 and MIXED forces vs FP64."""
     c = pkg.synth.plain_scale(seed=7)
