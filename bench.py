@@ -294,7 +294,7 @@ def main():
                     help="only the FP32 timed region (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 5 x 256 MiB device-to-device (counter calibration run)")
-    ap.add_argument("--form", choices=("precision", "batched", "adjoint"),
+    ap.add_argument("--form", choices=("precision", "batched", "adjoint", "search"),
                     help="only this form, with --form-steps steps (for profiler runs)")
     ap.add_argument("--form-steps", type=int, default=0)
     args = ap.parse_args()
@@ -309,7 +309,8 @@ def main():
         copy_calibration(pkg)
         return
     if args.form:
-        fn = {"precision": precision_form, "batched": batched_form, "adjoint": adjoint_form}[args.form]
+        fn = {"precision": precision_form, "batched": batched_form, "adjoint": adjoint_form,
+              "search": lambda pkg: search_comparison(pkg, pkg.synth.plain_scale(seed=2))}[args.form]
         out = fn(pkg, args.form_steps) if args.form_steps > 0 else fn(pkg)
         print(json.dumps({"form": args.form, "result": out}, default=float), flush=True)
         return
