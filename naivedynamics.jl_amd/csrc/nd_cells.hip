@@ -222,7 +222,8 @@ int nd_cell_neighborlist(nd_ctx *ctx, const float *d_xyz, int32_t n, float cutof
     ctx->bbox.reserve(64);
     int32_t *bb = ctx->bbox.as<int32_t>();
     k_cell_bbox_init<<<1, 64, 0, st>>>(bb);
-    k_cell_bbox<<<(unsigned)std::min<int64_t>(nd::grid_for(n), 1024), nd::kBlock, 0, st>>>(d_xyz, n, bb);
+    // few workgroups: their atomics on the six words serialise in L2
+    k_cell_bbox<<<(unsigned)std::min<int64_t>(nd::grid_for(n), 64), nd::kBlock, 0, st>>>(d_xyz, n, bb);
     ND_HIP(hipGetLastError());
     int32_t hb[6];
     ND_HIP(hipMemcpyAsync(hb, bb, sizeof hb, hipMemcpyDeviceToHost, st));

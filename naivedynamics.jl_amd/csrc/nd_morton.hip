@@ -138,7 +138,8 @@ void launch_gather_xyz(hipStream_t s, const float *xyz, const int32_t *perm, int
 void launch_bbox(nd_ctx *ctx, const float *xyz, int32_t n, float *bbox6) {
     int32_t *bb = reinterpret_cast<int32_t *>(bbox6);
     k_bbox_init<<<1, 64, 0, ctx->stream>>>(bb);
-    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(n), 1024));
+    // few workgroups: their atomics on the six words serialise in L2
+    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(n), 64));
     k_bbox<<<g, kBlock, 0, ctx->stream>>>(xyz, n, bb);
     k_bbox_finalize<<<1, 64, 0, ctx->stream>>>(bb);
     ND_HIP(hipGetLastError());

@@ -676,7 +676,10 @@ k_nlist(const float4 *__restrict__ pos, const nd_node *__restrict__ nodes, int32
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0 && m > 0) {
-        atomicMax(&misc->maxcnt, m);
+        // the counter only grows: skip the (serialising) atomic when it is
+        // already at least m
+        if (m > __hip_atomic_load(&misc->maxcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&misc->maxcnt, m);
         if (m > cap) atomicOr(&misc->overflow, 1);
     }
 }
@@ -773,7 +776,10 @@ k_nlist2(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0 && m > 0) {
-        atomicMax(&misc->maxcnt, m);
+        // the counter only grows: skip the (serialising) atomic when it is
+        // already at least m
+        if (m > __hip_atomic_load(&misc->maxcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&misc->maxcnt, m);
         if (m > cap) atomicOr(&misc->overflow, 1);
     }
 }
@@ -966,7 +972,9 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
     if (lane == 0 && m > 0) {
-        atomicMax(&ctr[0], m);
+        // the counter only grows: skip the (serialising) atomic when it is
+        // already at least m
+        if (m > __hip_atomic_load(&ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&ctr[0], m);
         if (m > cap) atomicOr(&ctr[1], 1);
     }
 }
@@ -1017,7 +1025,9 @@ k_cell_rows(const float4 *__restrict__ pos, const float4 *__restrict__ cpos,
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
     if (lane == 0 && m > 0) {
-        atomicMax(&ctr[0], m);
+        // the counter only grows: skip the (serialising) atomic when it is
+        // already at least m
+        if (m > __hip_atomic_load(&ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&ctr[0], m);
         if (m > cap) atomicOr(&ctr[1], 1);
     }
 }

@@ -213,14 +213,17 @@ k_keybits(const int32_t *__restrict__ keys, int32_t n, uint32_t *__restrict__ ou
         o |= k;
         a &= k;
     }
-    // one atomic pair per wave, not per lane (they all hit one address)
-    for (int off = 32; off > 0; off >>= 1) {
-        o |= (uint32_t)__shfl_xor((int)o, off, 64);
-        a &= (uint32_t)__shfl_xor((int)a, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicOr(&out[0], o);
-        atomicAnd(&out[1], a);
+    // one atomic pair per workgroup: atomics on one address serialise in
+    // one L2 channel, so the grid is kept small (see the launch)
+    __shared__ uint32_t so, sa;
+    if (threadIdx.x == 0) { so = 0u; sa = 0xFFFFFFFFu; }
+    __syncthreads();
+    atomicOr(&so, o);
+    atomicAnd(&sa, a);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicOr(&out[0], so);
+        atomicAnd(&out[1], sa);
     }
 }
 
@@ -297,7 +300,7 @@ void radix_sort_pairs_auto(nd_ctx *ctx, int32_t *keys, int32_t *vals, int32_t n)
     ctx->sort.flags.reserve(64);
     uint32_t *fl = ctx->sort.flags.as<uint32_t>();
     k_keybits_init<<<1, 64, 0, ctx->stream>>>(fl);
-    unsigned g = (unsigned)std::min<int64_t>(grid_for(n), 1024);
+    unsigned g = (unsigned)std::min<int64_t>(grid_for(n), 64);
     k_keybits<<<g, kBlock, 0, ctx->stream>>>(keys, n, fl);
     uint32_t h[2];
     ND_HIP(hipMemcpyAsync(h, fl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
