@@ -183,8 +183,8 @@ struct PairAcc {
     int4 pfx;     // MIXED: fixed-point self position
     float inv;    // MIXED: fixed-point unit
     AccS fx = 0, fy = 0, fz = 0, e = 0, w = 0;
-    // MIXED: FP32 force sums over one batch of neighbours, added to the FP64
-    // sums by flush() (one conversion and FP64 add per batch, not per pair)
+    // MIXED: FP32 force sums over two batches of neighbours (8 pairs), added
+    // to the FP64 sums by flush() (one conversion and FP64 add per 8 pairs)
     static constexpr bool kBatchSums = sizeof(AccS) != sizeof(CalcS);
     CalcS bx = 0, by = 0, bz = 0;
     int32_t np = 0;
@@ -358,7 +358,6 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 for (int u = 0; u < B; ++u) ja[u] = idx(k + 2 * B + u);
 #pragma unroll
                 for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], k + u < cnt);
-                acc.flush();
                 if (k + B >= cnt) break;
 #pragma unroll
                 for (int u = 0; u < B; ++u) qa[u] = gather(ja[u]);
@@ -366,9 +365,10 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 for (int u = 0; u < B; ++u) jb[u] = idx(k + 3 * B + u);
 #pragma unroll
                 for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], k + B + u < cnt);
-                acc.flush();
+                acc.flush();  // MIXED: FP32 sums over 2 batches, then FP64
                 if (k + 2 * B >= cnt) break;
             }
+            acc.flush();  // after an exit at the first batch (a repeat adds zeros)
         };
         // equal cutoffs (uniform over the launch): one mask per pair
         if (acc.rc2 == acc.rcc2) pairs(std::true_type{});
