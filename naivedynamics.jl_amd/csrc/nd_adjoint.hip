@@ -449,6 +449,8 @@ void build_list(nd_adj *s, const T *xp) {
         if (h[1] || h[6]) throw nd::Error(ND_ERR_STATE, "adjoint: malformed neighbour tree");
         if (h[4] <= s->cap) return;
         s->cap = h[4] + h[4] / 4 + 8;
+        // k_adj_map_ids takes one grid row per list row
+        if (s->cap > 65535) throw nd::Error(ND_ERR_CAPACITY, "adjoint: more than 65,535 neighbours per atom");
         reserve_rows(s);
         nd::launch_fill_i32(st, flags + 4, 3, 0);
     }

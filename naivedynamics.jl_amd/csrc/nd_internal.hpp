@@ -123,6 +123,10 @@ void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
                  nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared = false);
 
+// Largest per-atom list capacity: a wave's [cap][64] block of int32 rows is
+// addressed through a buffer resource whose byte range (cap * 256) is int32.
+constexpr int32_t kMaxListRows = (1 << 23) - 1;
+
 // nd_sim.hip: the two-phase list walk over an LBVH of float4 positions
 // (sorted; leaf boxes +- rlist/2) with 4, 8, 16 or 32 atoms per leaf.  It
 // writes the four-wide records into rec (8 float4 per internal node), one

@@ -1067,6 +1067,7 @@ void launch_list_walk(hipStream_t st, const nd_node *nodes, float4 *rec, const f
                       int32_t nL, int32_t apl, float rlist, int32_t cap, int32_t *nbr, int32_t *ncount,
                       int32_t *ctr) {
     if (nL >= kMaxLeaves3) throw Error(ND_ERR_ARG, "list walk: too many leaves");
+    if (cap < 1 || cap > kMaxListRows) throw Error(ND_ERR_CAPACITY, "list walk: capacity out of range");
     launch_lbvh_wide4(st, nodes, nL, rec);
     const float hl = rlist / 2.0f, rl2 = rlist * rlist;
     const unsigned g = grid_for(n);
@@ -1137,8 +1138,8 @@ void drop_graphs(nd_sim *s) {
 void alloc_lists(nd_sim *s, int32_t cap) {
     // a wave's block of the list (cap rows of 256 B) is addressed through a
     // buffer resource with a 31-bit byte range
-    if (cap > (1 << 23))
-        throw nd::Error(ND_ERR_CAPACITY, "simulation: more than 2^23 neighbours per atom");
+    if (cap > nd::kMaxListRows)
+        throw nd::Error(ND_ERR_CAPACITY, "simulation: 2^23 or more neighbours per atom");
     s->cap = cap;
     s->sp.cap = cap;
     // one block of cap rows x 64 slots per 64 atoms (n rounded up to 64)
