@@ -148,9 +148,11 @@ class BatchedSimulation:
 def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None):
     """Pick the leaf size (the traversal / force tile) and list walk of the
     packed run by timing `windows` rebuild windows of each candidate after one
-    warm-up window.  Returns (best spec, [(leaf, walk, ms_per_step)])."""
+    warm-up window: LBVH leaves of 8, 16 or 32 atoms with the two-phase (0)
+    or rope (1) walk, and the cell list (3, no tree).  Returns (best spec,
+    [(leaf, walk, ms_per_step)])."""
     if candidates is None:
-        candidates = [(a, w) for a in (8, 16, 32) for w in (0, 1)]
+        candidates = [(a, w) for a in (8, 16, 32) for w in (0, 1)] + [(16, 3)]
     table = []
     best = None
     for apl, walk in candidates:

@@ -19,13 +19,15 @@ def _spec(pkg, precision="fp64", **kw):
     return pkg.SimSpec(**d)
 
 
-def test_packed_replicas_follow_the_oracle(pkg, O):
+@pytest.mark.parametrize("walk", [0, 3])
+def test_packed_replicas_follow_the_oracle(pkg, O, walk):
+    # walk 0: two-phase LBVH walk; 3: cell list over the packed box
     reps = pkg.batched.make_replicas(SIZES, seed=5)
     # the first layer in x of every replica moves outwards fast: walls are hit
     for R in reps:
         R["v"] = R["v"].copy()
         R["v"][R["x"][:, 0] < 0.8, 0] = -20.0
-    b = pkg.BatchedSimulation(reps, _spec(pkg))
+    b = pkg.BatchedSimulation(reps, _spec(pkg, list_walk=walk))
     b.set_state()
     b.run(40)
     states = b.replica_states()
@@ -69,8 +71,8 @@ def test_set_replicas_rejects_bad_input(pkg):
 
 def test_autotune_picks_a_measured_candidate(pkg):
     reps = pkg.batched.make_replicas([512, 1024, 700], seed=8)
-    best, table = pkg.autotune(reps, _spec(pkg, "fp32"), candidates=[(8, 0), (16, 0), (16, 1)],
+    best, table = pkg.autotune(reps, _spec(pkg, "fp32"), candidates=[(8, 0), (16, 0), (16, 1), (16, 3)],
                                windows=1)
-    assert len(table) == 3
+    assert len(table) == 4
     fastest = min(table, key=lambda t: t[2])
     assert (best.atomsperleaf, best.list_walk) == fastest[:2]
