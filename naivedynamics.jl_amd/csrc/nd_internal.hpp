@@ -118,10 +118,12 @@ void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_strid
 void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *wide);
 // Four-wide records of the internal nodes (8 float4 per internal node).
 void launch_lbvh_wide4(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *rec);
-// slots_cleared: the caller has already zeroed store[0 .. nL-2] (the CAS slots).
+// slots_cleared: the caller has already zeroed store[0 .. nL-2] (the CAS slots);
+// leaves_done: the caller has already written the leaf boxes (+- r/2).
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
-                 nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared = false);
+                 nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared = false,
+                 bool leaves_done = false);
 
 // Largest per-atom list capacity: a wave's [cap][64] block of int32 rows is
 // addressed through a buffer resource whose byte range (cap * 256) is int32.

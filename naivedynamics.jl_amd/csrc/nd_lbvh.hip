@@ -382,7 +382,8 @@ void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *w
 
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
                  const int32_t *codes_sorted, int32_t nL, int32_t apl, float r, int32_t override_root,
-                 nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared) {
+                 nd_node *nodes, int32_t *store, int32_t *status, bool slots_cleared,
+                 bool leaves_done) {
     // store (lbvh_scratch_ints(nL) ints): nL-1 CAS slots | nL-1 int2 leaf
     // ranges | one 32-byte box per 64-leaf chunk
     if (!slots_cleared) launch_fill_i32(s, store, nL - 1, 0);
@@ -391,7 +392,7 @@ void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
     int2 *ranges = reinterpret_cast<int2 *>(store + off_r);
     float4 *chunks = reinterpret_cast<float4 *>(store + off_c);
     const float h = r / 2.0f;  // neighbor_distance/2 in Float32
-    launch_leaf_boxes(s, xyz_sorted, xyz_stride, nL, apl, h, nodes);
+    if (!leaves_done) launch_leaf_boxes(s, xyz_sorted, xyz_stride, nL, apl, h, nodes);
     k_lbvh_topology<<<grid_for(nL), kBlock, 0, s>>>(codes_sorted, nL, apl, nodes, store, ranges, status);
     const int64_t nchunks = (nL + kChunk - 1) / kChunk;
     k_lbvh_chunks<<<grid_for(nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);

@@ -569,24 +569,50 @@ k_sim_keys(const typename V4<PosS>::T *__restrict__ pos, int32_t n, int32_t mode
 // Gather the state into the new order; also writes the FP32 copy of the
 // positions that the tree build and traversal read, and (MIXED) the
 // fixed-point copy that the pair kernel gathers.
-template <class PosS>
+// SEG > 0 (a power of two <= 64, the leaf size): also the leaf boxes of the
+// new order, the FP32 positions +- h reduced over each leaf's SEG lanes with
+// Julia's min / max, as the separate leaf-box kernel would compute them.
+template <class PosS, int SEG>
 __global__ void __launch_bounds__(nd::kBlock)
 k_permute(const int32_t *__restrict__ perm, int32_t n, const typename V4<PosS>::T *__restrict__ pos_in,
           typename V4<PosS>::T *__restrict__ pos_out, const typename V4<PosS>::T *__restrict__ vel_in,
           typename V4<PosS>::T *__restrict__ vel_out, const int32_t *__restrict__ orig_in,
           int32_t *__restrict__ orig_out, float4 *__restrict__ pos32, int4 *__restrict__ fix_out,
-          StepParams P, SimMisc *__restrict__ misc) {
+          StepParams P, SimMisc *__restrict__ misc, float h, nd_node *__restrict__ nodes) {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int32_t o = perm[s];
-    const auto p = pos_in[o];
-    pos_out[s] = p;
-    vel_out[s] = vel_in[o];
-    orig_out[s] = orig_in[o];
-    if (pos32) pos32[s] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
-    if (fix_out)
-        fix_out[s] = make_int4(to_fix((double)p.x, P, misc), to_fix((double)p.y, P, misc),
-                               to_fix((double)p.z, P, misc), __float_as_int((float)p.w));
+    const bool act = s < n;
+    const float inf = __int_as_float(0x7F800000);
+    float mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;
+    if (act) {
+        const int32_t o = perm[s];
+        const auto p = pos_in[o];
+        pos_out[s] = p;
+        vel_out[s] = vel_in[o];
+        orig_out[s] = orig_in[o];
+        const float px = (float)p.x, py = (float)p.y, pz = (float)p.z;
+        if (pos32) pos32[s] = make_float4(px, py, pz, (float)p.w);
+        if (fix_out)
+            fix_out[s] = make_int4(to_fix((double)p.x, P, misc), to_fix((double)p.y, P, misc),
+                                   to_fix((double)p.z, P, misc), __float_as_int((float)p.w));
+        mn0 = px - h; mn1 = py - h; mn2 = pz - h;
+        mx0 = px + h; mx1 = py + h; mx2 = pz + h;
+    }
+    if constexpr (SEG > 0) {
+#pragma unroll
+        for (int off = 1; off < SEG; off <<= 1) {
+            mn0 = jl_min(mn0, __shfl_xor(mn0, off, 64));
+            mn1 = jl_min(mn1, __shfl_xor(mn1, off, 64));
+            mn2 = jl_min(mn2, __shfl_xor(mn2, off, 64));
+            mx0 = jl_max(mx0, __shfl_xor(mx0, off, 64));
+            mx1 = jl_max(mx1, __shfl_xor(mx1, off, 64));
+            mx2 = jl_max(mx2, __shfl_xor(mx2, off, 64));
+        }
+        if (act && (s & (SEG - 1)) == 0) {
+            float *q = reinterpret_cast<float *>(nodes + s / SEG);
+            *reinterpret_cast<float4 *>(q) = make_float4(mn0, mn1, mn2, mx0);
+            *reinterpret_cast<float2 *>(q + 4) = make_float2(mx1, mx2);
+        }
+    }
 }
 
 // One wave per 64 consecutive atoms (Morton-adjacent, so spatially compact):
@@ -1270,16 +1296,34 @@ void launch_sort_and_build(nd_sim *s) {
     // keys are < 2^key_bits, so the sign-flipped key's higher bits are constant
     nd::radix_sort_keys10(st, s->sort, codes, perm, n, key_bits, &codes, &perm);
     const int a = s->cur ^ 1, va = s->vcur ^ 1;
-    if (s->wide)
-        k_permute<double><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
-                                                    s->vel[s->vcur].as<double4>(), s->vel[va].as<double4>(),
-                                                    s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),
-                                                    s->pos32.as<float4>(), fix_ptr(s, a), s->sp, misc);
-    else
-        k_permute<float><<<g, nd::kBlock, 0, st>>>(perm, n, s->pos[s->cur].as<float4>(), s->pos[a].as<float4>(),
-                                                   s->vel[s->vcur].as<float4>(), s->vel[va].as<float4>(),
-                                                   s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),
-                                                   nullptr, nullptr, s->sp, misc);
+    // leaf boxes in the permutation for power-of-two leaves up to 64 (the
+    // cell walk needs no tree, so none there)
+    const bool boxes = s->p.list_walk != 3 && s->apl <= 64 && (s->apl & (s->apl - 1)) == 0;
+    const float h = s->rlist / 2.0f;  // as launch_lbvh computes it
+    nd_node *nodes = s->nodes.as<nd_node>();
+#define ND_PERMUTE(PT, SEG)                                                                              \
+    k_permute<PT, SEG><<<g, nd::kBlock, 0, st>>>(                                                        \
+        perm, n, s->pos[s->cur].as<V4<PT>::T>(), s->pos[a].as<V4<PT>::T>(), s->vel[s->vcur].as<V4<PT>::T>(), \
+        s->vel[va].as<V4<PT>::T>(), s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),           \
+        s->wide ? s->pos32.as<float4>() : nullptr, s->wide ? fix_ptr(s, a) : nullptr, s->sp, misc, h, nodes)
+#define ND_PERMUTE_SEG(PT)                                        \
+    switch (boxes ? s->apl : 0) {                                \
+        case 1: ND_PERMUTE(PT, 1); break;                         \
+        case 2: ND_PERMUTE(PT, 2); break;                         \
+        case 4: ND_PERMUTE(PT, 4); break;                         \
+        case 8: ND_PERMUTE(PT, 8); break;                         \
+        case 16: ND_PERMUTE(PT, 16); break;                       \
+        case 32: ND_PERMUTE(PT, 32); break;                       \
+        case 64: ND_PERMUTE(PT, 64); break;                       \
+        default: ND_PERMUTE(PT, 0);                               \
+    }
+    if (s->wide) {
+        ND_PERMUTE_SEG(double)
+    } else {
+        ND_PERMUTE_SEG(float)
+    }
+#undef ND_PERMUTE_SEG
+#undef ND_PERMUTE
     s->cur = a;
     s->vcur = va;
     if (s->p.list_walk == 3) {  // the cell list needs no tree
@@ -1289,7 +1333,7 @@ void launch_sort_and_build(nd_sim *s) {
     const float *p32 = s->wide ? reinterpret_cast<const float *>(s->pos32.p)
                                : reinterpret_cast<const float *>(s->pos[s->cur].p);
     nd::launch_lbvh(st, p32, 4, codes, s->nL, s->apl, s->rlist, 0, s->nodes.as<nd_node>(),
-                    s->store.as<int32_t>(), &misc->status, true);
+                    s->store.as<int32_t>(), &misc->status, true, boxes);
     ND_HIP(hipGetLastError());
 }
 
