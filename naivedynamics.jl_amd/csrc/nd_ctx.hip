@@ -54,8 +54,9 @@ int nd_ctx_create(int32_t device, nd_ctx **out) {
 
 int nd_ctx_destroy(nd_ctx *ctx) {
     if (!ctx) return ND_OK;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    // teardown: errors here have no caller to report to
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     nd::DevBuf *bufs[] = {&ctx->status, &ctx->codes, &ctx->codes_sorted, &ctx->perm,
                           &ctx->xyz_sorted, &ctx->nodes, &ctx->store, &ctx->counts,
                           &ctx->offsets, &ctx->xyz_in, &ctx->pairs_out, &ctx->bbox,
@@ -63,9 +64,9 @@ int nd_ctx_destroy(nd_ctx *ctx) {
                           &ctx->sort.flags, &ctx->cell_keys, &ctx->cell_vals, &ctx->cell_pos,
                           &ctx->cell_start, &ctx->trav_rec, &ctx->scan_tmp};
     for (auto *b : bufs) b->release();
-    if (ctx->ev0) hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) hipEventDestroy(ctx->ev1);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return ND_OK;
 }

@@ -45,7 +45,7 @@ struct DevBuf {
         bytes = n;
     }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);  // teardown: nothing to report to
         p = nullptr;
         bytes = 0;
     }

@@ -1158,7 +1158,7 @@ int4 *fix_ptr(nd_sim *s, int b) { return s->p.precision == ND_PREC_MIXED ? s->fi
 
 void drop_graphs(nd_sim *s) {
     for (auto &g : s->gexec)
-        if (g) { hipGraphExecDestroy(g); g = nullptr; }
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
 void alloc_lists(nd_sim *s, int32_t cap) {
@@ -1502,7 +1502,7 @@ void run_window_graph(nd_sim *s, int64_t k0) {
         try {
             launch_window(s, k0, full);
         } catch (...) {
-            hipStreamEndCapture(st, &graph);
+            (void)hipStreamEndCapture(st, &graph);
             throw;
         }
         ND_HIP(hipStreamEndCapture(st, &graph));
@@ -1715,11 +1715,11 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
 int nd_sim_destroy(nd_sim *s) {
     if (!s) return ND_OK;
     if (s->ctx) {
-        hipSetDevice(s->ctx->device);
-        hipStreamSynchronize(s->ctx->stream);
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
     }
     drop_graphs(s);
-    for (auto e : s->ev_pool) hipEventDestroy(e);
+    for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
                           &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids, &s->rep, &s->boxes,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
