@@ -817,7 +817,7 @@ using nd::wave_lds_sync;
 // that is a leaf stands for itself): up to 4 entries with boxes and node
 // numbers.  Phase 1 pops up to kFront records from the wave's LDS stack and
 // fetches them together (lane l loads float4 l & 7 of records (l >> 3) + 8 u),
-// so one memory latency covers up to 16 records (two tree levels each); every lane
+// so one memory latency covers up to kFront records (two tree levels each); every lane
 // tests its box against each entry, overlapped internal entries are pushed,
 // and an overlapped leaf is queued for each group of 8 lanes (8 consecutive
 // atoms) with an overlapping lane, together with the group's 8 overlap bits.
@@ -834,7 +834,7 @@ using nd::wave_lds_sync;
 // remain the frontier shrinks to one record (plain depth-first order), which
 // adds at most 3 entries per record level.  Queue entries pack
 // (leaf << 8) | bits, so leaves must number below 2^23.
-constexpr int kFront = 16, kStack2 = 256, kStackDfs = 128, kGroupQ = 128;
+constexpr int kFront = 8, kStack2 = 256, kStackDfs = 128, kGroupQ = 128;
 constexpr int32_t kMaxLeaves3 = 1 << 23;
 
 template <int A>
