@@ -138,6 +138,18 @@ int nd_leafneighbor_traverse(nd_ctx *ctx, const nd_node *d_nodes, const float *d
 int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
                               int32_t predicate, int32_t morton_mode, nd_pair *d_out,
                               int64_t cap, int64_t *count);
+/* Stream-ordered form of nd_leafbuild_traverse_bvh (no host synchronisation,
+ * for hipGraph capture and pipelines): *d_count (device int64) receives the
+ * required pair count and the pairs are written only if it is <= cap
+ * (otherwise nothing is written, as in the synchronous form); *d_status
+ * (device int32) receives status bits, 1 = a coordinate outside the key
+ * domain (the synchronous form's ND_ERR_DOMAIN), 4 = a malformed tree.  Same
+ * pairs in the same order as the synchronous form; the key sort covers all
+ * 32 bits instead of probing which digits vary.  Work buffers grow on the
+ * first call for a size, so call once before capturing a graph. */
+int nd_leafbuild_traverse_bvh_async(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
+                                    int32_t predicate, int32_t morton_mode, nd_pair *d_out,
+                                    int64_t cap, int64_t *d_count, int32_t *d_status);
 /* Host-buffer form of the same call (stages h_xyz in and pairs out). */
 int nd_leafbuild_traverse_bvh_host(nd_ctx *ctx, const float *h_xyz, const nd_bvh_spec *spec,
                                    int32_t predicate, int32_t morton_mode, nd_pair *h_out,

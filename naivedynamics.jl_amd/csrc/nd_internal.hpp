@@ -143,11 +143,16 @@ void launch_list_walk(hipStream_t st, const nd_node *nodes, float4 *rec, const f
 // nd_traverse.hip
 // own_tree: the tree was built by launch_lbvh (well formed: left pointers and
 // ropes describe one depth-first order), which allows the wave-cooperative walk.
+// count_dev != null: stream-ordered (no host synchronisation): the required
+// count goes to *count_dev, status bits to status_dev, pairs are written only
+// if the count is <= cap; returns -1.
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
                       const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t predicate,
-                      nd_pair *out, int64_t cap, bool own_tree = false);
-// Exclusive scan of ctx->counts (n int64) into ctx->offsets (n + 1); synchronises
-// and returns the total.
+                      nd_pair *out, int64_t cap, bool own_tree = false, int32_t *status_dev = nullptr,
+                      int64_t *count_dev = nullptr);
+// Exclusive scan of ctx->counts (n int64) into ctx->offsets (n + 1): launch only,
+// or launch, synchronise and return the total.
+void launch_scan(nd_ctx *ctx, int64_t n);
 int64_t scan_counts(nd_ctx *ctx, int64_t n);
 
 // Uniform cell grid: cell d-coordinate of x is clamp(floor((x - lo[d]) * inv[d]),

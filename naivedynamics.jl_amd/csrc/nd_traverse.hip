@@ -56,9 +56,10 @@ __global__ void __launch_bounds__(nd::kBlock)
 k_leaf_traverse(const nd_node *__restrict__ nodes, const float *__restrict__ xyz,
                 const int32_t *__restrict__ idx, int32_t nL, int32_t apl, float r, float r2,
                 int32_t pred, int64_t *__restrict__ counts, const int64_t *__restrict__ offsets,
-                nd_pair *__restrict__ out, int32_t *__restrict__ status) {
+                nd_pair *__restrict__ out, int32_t *__restrict__ status, int64_t cap) {
     const int32_t q = 1 + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (q > nL) return;
+    if (EMIT && offsets[nL] > cap) return;  // too small a buffer: write nothing
     Sink<EMIT> sink;
     if (EMIT) {
         sink.out = out + offsets[q - 1];
@@ -114,9 +115,10 @@ __global__ void __launch_bounds__(nd::kBlock)
 k_leaf_traverse4(const nd_node *__restrict__ nodes, const float *__restrict__ xyz,
                  const int32_t *__restrict__ idx, int32_t nL, float r, float r2, int32_t pred,
                  int64_t *__restrict__ counts, const int64_t *__restrict__ offsets,
-                 nd_pair *__restrict__ out, int32_t *__restrict__ status) {
+                 nd_pair *__restrict__ out, int32_t *__restrict__ status, int64_t cap) {
     const int32_t q = 1 + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (q > nL) return;
+    if (EMIT && offsets[nL] > cap) return;  // too small a buffer: write nothing
     Sink<EMIT> sink;
     if (EMIT) {
         sink.out = out + offsets[q - 1];
@@ -268,8 +270,10 @@ k_leaf_traverse_w(const nd_node *__restrict__ nodes, const float4 *__restrict__ 
                   const float *__restrict__ xyz, const int32_t *__restrict__ idx, int32_t nL,
                   float r, float r2, int32_t nslice, int64_t *__restrict__ counts,
                   const int64_t *__restrict__ offsets, nd_pair *__restrict__ out,
-                  int32_t *__restrict__ status) {
+                  int32_t *__restrict__ status, int64_t cap) {
     static_assert(A >= 1 && A <= 64 && (64 % A) == 0, "leaf size must divide 64");
+    // too small a buffer: write nothing (uniform over the launch)
+    if (EMIT && offsets[(int64_t)nL * nslice] > cap) return;
     // query atoms per wave: 32 (the upper half-wave idles) keeps twice the
     // walks in flight and a smaller union of leaf boxes per walk
     constexpr int kApw = A <= 32 ? 32 : 64;
@@ -511,6 +515,10 @@ k_scan_tile_apply(const int64_t *__restrict__ in, int64_t n, const int64_t *__re
     if (blockIdx.x == ntiles - 1 && t == 0) out[n] = tile_off[ntiles];
 }
 
+__global__ void k_copy_total(const int64_t *__restrict__ src, int64_t *__restrict__ dst) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 // Exclusive scan of n int64 counts into offsets[0..n]; offsets[n] = total.
 __global__ void __launch_bounds__(1024)
 k_scan_i64(const int64_t *__restrict__ in, int64_t n, int64_t *__restrict__ out) {
@@ -565,8 +573,8 @@ k_bruteforce(const float *__restrict__ xyz, int32_t n, float r, int64_t *__restr
 
 namespace nd {
 
-int64_t scan_counts(nd_ctx *ctx, int64_t n) {
-    // counts in ctx->counts (n), offsets to ctx->offsets (n+1); returns total.
+void launch_scan(nd_ctx *ctx, int64_t n) {
+    // counts in ctx->counts (n), offsets to ctx->offsets (n+1), no host sync
     if (n <= 4 * kScanTile) {
         k_scan_i64<<<1, 1024, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, ctx->offsets.as<int64_t>());
     } else {
@@ -579,6 +587,10 @@ int64_t scan_counts(nd_ctx *ctx, int64_t n) {
                                                                      ctx->offsets.as<int64_t>());
     }
     ND_HIP(hipGetLastError());
+}
+
+int64_t scan_counts(nd_ctx *ctx, int64_t n) {
+    launch_scan(ctx, n);
     int64_t total = 0;
     ND_HIP(hipMemcpyAsync(&total, ctx->offsets.as<int64_t>() + n, sizeof(int64_t),
                           hipMemcpyDeviceToHost, ctx->stream));
@@ -594,7 +606,7 @@ inline int32_t trav_slices(int32_t nL) { return nL >= 64 ? 8 : 1; }
 template <bool EMIT>
 void launch_traverse_w(hipStream_t st, int32_t apl, const nd_node *nodes, const float4 *rec,
                        const float *xyz, const int32_t *idx, int32_t nL, float r, float r2, int32_t pred,
-                       int64_t *cnt, const int64_t *off, nd_pair *out, int32_t *status) {
+                       int64_t *cnt, const int64_t *off, nd_pair *out, int32_t *status, int64_t cap) {
     const int32_t ns = trav_slices(nL);
     const int64_t waves = ((int64_t)nL * apl + (apl <= 32 ? 31 : 63)) / (apl <= 32 ? 32 : 64);
     const unsigned g = grid_for(waves * ns * 64);
@@ -602,10 +614,10 @@ void launch_traverse_w(hipStream_t st, int32_t apl, const nd_node *nodes, const 
     case A:                                                                                       \
         if (pred == ND_PRED_SCALAR)                                                               \
             k_leaf_traverse_w<EMIT, A, ND_PRED_SCALAR><<<g, kBlock, 0, st>>>(nodes, rec, xyz, idx, nL, \
-                                                                            r, r2, ns, cnt, off, out, status); \
+                                                                            r, r2, ns, cnt, off, out, status, cap); \
         else                                                                                      \
             k_leaf_traverse_w<EMIT, A, ND_PRED_SIMD><<<g, kBlock, 0, st>>>(nodes, rec, xyz, idx, nL,   \
-                                                                          r, r2, ns, cnt, off, out, status);   \
+                                                                          r, r2, ns, cnt, off, out, status, cap);   \
         break;
     switch (apl) {
         ND_TRAV_W(1) ND_TRAV_W(2) ND_TRAV_W(4) ND_TRAV_W(8) ND_TRAV_W(16) ND_TRAV_W(32) ND_TRAV_W(64)
@@ -618,56 +630,65 @@ void launch_traverse_w(hipStream_t st, int32_t apl, const nd_node *nodes, const 
 
 int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted,
                       const int32_t *idx_sorted, int32_t n, int32_t apl, float r, int32_t pred,
-                      nd_pair *out, int64_t cap, bool own_tree) {
+                      nd_pair *out, int64_t cap, bool own_tree, int32_t *status_dev, int64_t *count_dev) {
     const int32_t nL = n / apl;
     const float r2 = r * r;  // (neighbor_distance)^2 in Float32
-    ctx->counts.reserve(sizeof(int64_t) * (size_t)(nL + 1));
-    ctx->offsets.reserve(sizeof(int64_t) * (size_t)(nL + 1));
+    const bool async = count_dev != nullptr;
+    int32_t *st = async ? status_dev : ctx->status.as<int32_t>();
+    const bool wave_walk = own_tree && apl <= 64 && 64 % apl == 0;
+    // one depth-first walk per wave and leaf-range slice (same pairs, same
+    // order: counts per (leaf, slice), scanned leaf-major), or the rope walk
+    // per query leaf
+    const int64_t nc = wave_walk ? (int64_t)nL * trav_slices(nL) : (int64_t)nL;
+    ctx->counts.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+    ctx->offsets.reserve(sizeof(int64_t) * (size_t)(nc + 1));
     int64_t *cnt = ctx->counts.as<int64_t>();
     const int64_t *off = ctx->offsets.as<int64_t>();
-    int32_t *st = ctx->status.as<int32_t>();
-    if (own_tree && apl <= 64 && 64 % apl == 0) {
-        // one depth-first walk per wave and leaf-range slice (same pairs, same
-        // order: counts per (leaf, slice), scanned leaf-major)
-        const int64_t nc = (int64_t)nL * trav_slices(nL);
-        ctx->counts.reserve(sizeof(int64_t) * (size_t)(nc + 1));
-        ctx->offsets.reserve(sizeof(int64_t) * (size_t)(nc + 1));
-        cnt = ctx->counts.as<int64_t>();
-        off = ctx->offsets.as<int64_t>();
+    float4 *rec = nullptr;
+    auto pass = [&](bool emit) {
+        if (wave_walk) {
+            if (emit)
+                launch_traverse_w<true>(ctx->stream, apl, nodes, rec, xyz_sorted, idx_sorted, nL, r, r2, pred,
+                                        cnt, off, out, st, cap);
+            else
+                launch_traverse_w<false>(ctx->stream, apl, nodes, rec, xyz_sorted, idx_sorted, nL, r, r2, pred,
+                                         cnt, off, nullptr, st, cap);
+        } else if (apl == 4) {
+            if (emit)
+                k_leaf_traverse4<true><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
+                    nodes, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt, off, out, st, cap);
+            else
+                k_leaf_traverse4<false><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
+                    nodes, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt, off, nullptr, st, cap);
+        } else {
+            if (emit)
+                k_leaf_traverse<true><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
+                    nodes, xyz_sorted, idx_sorted, nL, apl, r, r2, pred, cnt, off, out, st, cap);
+            else
+                k_leaf_traverse<false><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
+                    nodes, xyz_sorted, idx_sorted, nL, apl, r, r2, pred, cnt, off, nullptr, st, cap);
+        }
+        ND_HIP(hipGetLastError());
+    };
+    if (wave_walk) {
         ctx->trav_rec.reserve(8 * sizeof(float4) * (size_t)(nL - 1));
-        float4 *rec = ctx->trav_rec.as<float4>();
+        rec = ctx->trav_rec.as<float4>();
         k_trav_records<<<grid_for(nL - 1), kBlock, 0, ctx->stream>>>(nodes, nL, rec);
-        launch_traverse_w<false>(ctx->stream, apl, nodes, rec, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt,
-                                 off, nullptr, st);
-        ND_HIP(hipGetLastError());
-        const int64_t total = scan_counts(ctx, nc);
-        if (nd::read_status(ctx) & nd::kStatusTree)
-            throw nd::Error(ND_ERR_STATE, "leafneighbor_traverse: malformed tree");
-        if (total > cap || out == nullptr) return total;
-        launch_traverse_w<true>(ctx->stream, apl, nodes, rec, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt,
-                                off, out, st);
-        ND_HIP(hipGetLastError());
-        ND_HIP(hipStreamSynchronize(ctx->stream));
-        return total;
     }
-    if (apl == 4)
-        k_leaf_traverse4<false><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
-            nodes, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt, off, nullptr, st);
-    else
-        k_leaf_traverse<false><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
-            nodes, xyz_sorted, idx_sorted, nL, apl, r, r2, pred, cnt, off, nullptr, st);
-    ND_HIP(hipGetLastError());
-    const int64_t total = scan_counts(ctx, nL);
+    pass(false);
+    if (async) {
+        // stream-ordered: the emitting pass checks the total against cap itself
+        launch_scan(ctx, nc);
+        if (out) pass(true);
+        k_copy_total<<<1, 64, 0, ctx->stream>>>(off + nc, count_dev);
+        ND_HIP(hipGetLastError());
+        return -1;
+    }
+    const int64_t total = scan_counts(ctx, nc);
     if (nd::read_status(ctx) & nd::kStatusTree)
         throw nd::Error(ND_ERR_STATE, "leafneighbor_traverse: malformed tree (rope cycle or bad index)");
     if (total > cap || out == nullptr) return total;
-    if (apl == 4)
-        k_leaf_traverse4<true><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
-            nodes, xyz_sorted, idx_sorted, nL, r, r2, pred, cnt, off, out, st);
-    else
-        k_leaf_traverse<true><<<grid_for(nL), kBlock, 0, ctx->stream>>>(
-            nodes, xyz_sorted, idx_sorted, nL, apl, r, r2, pred, cnt, off, out, st);
-    ND_HIP(hipGetLastError());
+    pass(true);
     ND_HIP(hipStreamSynchronize(ctx->stream));
     return total;
 }
@@ -751,6 +772,41 @@ int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec
                                             predicate, d_out, cap, true);
     *count = total;
     if (total > cap) throw nd::Error(ND_ERR_CAPACITY, "pair buffer too small; *count = required");
+    ND_API_END(ctx)
+}
+
+int nd_leafbuild_traverse_bvh_async(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
+                                    int32_t predicate, int32_t morton_mode, nd_pair *d_out, int64_t cap,
+                                    int64_t *d_count, int32_t *d_status) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && d_xyz && d_count && d_status, "nd_leafbuild_traverse_bvh_async: null argument");
+    nd_bvh_spec chk;
+    check_spec(spec, chk);
+    check_pred(predicate);
+    ND_REQUIRE(morton_mode == ND_MORTON_MASKED || morton_mode == ND_MORTON_INTERLEAVE,
+               "unknown key mode");
+    ND_HIP(hipSetDevice(ctx->device));
+    const int32_t n = chk.atom_count, apl = chk.atomsperleaf, nL = chk.leaves_count;
+    hipStream_t st = ctx->stream;
+    ctx->has_tree = false;
+    ctx->codes_sorted.reserve(sizeof(int32_t) * (size_t)n);
+    ctx->perm.reserve(sizeof(int32_t) * (size_t)n);
+    ctx->xyz_sorted.reserve(sizeof(float) * 3 * (size_t)n);
+    ctx->nodes.reserve(sizeof(nd_node) * (size_t)(2 * nL - 1));
+    ctx->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(nL));
+    nd::launch_fill_i32(st, d_status, 1, 0);
+    nd::launch_morton(ctx, d_xyz, n, morton_mode, nullptr, nullptr, ctx->codes_sorted.as<int32_t>(),
+                      ctx->perm.as<int32_t>(), d_status);
+    // every key bit: the synchronous form's varying-digit probe reads back
+    nd::radix_sort_pairs(st, ctx->sort, ctx->codes_sorted.as<int32_t>(), ctx->perm.as<int32_t>(), n, 0, 32);
+    nd::launch_gather_xyz(st, d_xyz, ctx->perm.as<int32_t>(), n, ctx->xyz_sorted.as<float>());
+    nd::launch_lbvh(st, ctx->xyz_sorted.as<float>(), 3, ctx->codes_sorted.as<int32_t>(), nL, apl,
+                    chk.neighbor_distance, 1, ctx->nodes.as<nd_node>(), ctx->store.as<int32_t>(), d_status);
+    ctx->has_tree = true;
+    ctx->tree_n = n;
+    ctx->tree_apl = apl;
+    nd::leaf_traverse(ctx, ctx->nodes.as<nd_node>(), ctx->xyz_sorted.as<float>(), ctx->perm.as<int32_t>(), n,
+                      apl, chk.neighbor_distance, predicate, d_out, cap, true, d_status, d_count);
     ND_API_END(ctx)
 }
 

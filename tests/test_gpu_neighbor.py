@@ -264,3 +264,41 @@ def test_full_size_pair_set_independent_of_key(pkg):
     b, nb = pkg.neighborlist_device(x, spec, pkg.PRED_SCALAR, pkg.MORTON_MASKED)
     rb = rows_canonical(pairs_to_rows(b.to_host(nb)))
     assert na == nb and np.array_equal(ra, rb)
+
+
+@pytest.mark.parametrize("apl,pred", [(4, 0), (4, 1), (5, 0), (32, 0)])
+def test_async_entry_point_matches_sync(pkg, apl, pred):
+    """The stream-ordered one-call list: same pairs in the same order as the
+    synchronous call, count and status in device memory, nothing written when
+    the buffer is too small, and a domain error reported as status bit 1."""
+    import ctypes
+    n = 4000
+    x = _rand(n, (n / 0.8) ** (1 / 3), 60 + apl)
+    spec = pkg.SpheresBVHSpecs(neighbor_distance=2.5, atom_count=n, atomsperleaf=apl)
+    ref_out, nref = pkg.neighborlist_device(x, spec, pred, pkg.MORTON_INTERLEAVE)
+    ref = pairs_to_rows(ref_out.to_host(nref))
+    ctx = pkg.default_context()
+    lib = pkg.library()
+    d = pkg.DeviceArray.from_host(ctx, x)
+    cnt = pkg.DeviceArray.from_host(ctx, np.zeros(1, np.int64))
+    st = pkg.DeviceArray.from_host(ctx, np.full(1, 99, np.int32))
+
+    def call(out, cap, xs=d, mode=pkg.MORTON_INTERLEAVE):
+        rc = lib.nd_leafbuild_traverse_bvh_async(ctx.handle, xs.ptr, ctypes.byref(spec.c), pred, mode,
+                                                 out.ptr, cap, cnt.ptr, st.ptr)
+        assert rc == 0
+        ctx.synchronize()
+        return int(cnt.to_host()[0]), int(st.to_host()[0])
+
+    out = pkg.DeviceArray(ctx, (nref + 16,), pkg.PAIR_DTYPE)
+    assert call(out, nref + 16) == (nref, 0)
+    assert np.array_equal(pairs_to_rows(out.to_host(nref)), ref)
+    sentinel = np.zeros(10, pkg.PAIR_DTYPE)
+    sentinel["i"] = -7
+    small = pkg.DeviceArray.from_host(ctx, sentinel)
+    assert call(small, 10) == (nref, 0)
+    assert np.array_equal(small.to_host()["i"], sentinel["i"])
+    bad = x.copy()
+    bad[3, 1] = 3.0e9  # beyond round(Int32, x / binwidth) in the reference key
+    _, status = call(out, nref + 16, pkg.DeviceArray.from_host(ctx, bad), pkg.MORTON_MASKED)
+    assert status & 1
