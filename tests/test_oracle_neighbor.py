@@ -1,7 +1,7 @@
 """CPU: pins the neighbour-search oracle (oracle/refcpu.c).
 
 Checks against the golden fixtures (an independent top-down restatement,
-tests/golden/make_golden.py), the survey's hand-derived position8 keys,
+tests/golden/make_golden.py), hand-derived keys of the reference test's position8 points,
 numpy's stable sort, brute force (AllToAll.jl semantics, as the reference's
 own tests do: test/BVHTraverse.jl:205-209, 222-289) and tree invariants.
 """
