@@ -73,6 +73,7 @@ struct nd_ctx {
     nd::DevBuf cell_keys, cell_vals, cell_pos, cell_start;  // nd_cell_neighborlist
     nd::DevBuf trav_rec;  // four-wide records of the one-call traversal
     nd::DevBuf scan_tmp;  // tile sums of scan_counts
+    nd::DevBuf force_acc, force_part;  // pair-list forces: fixed-point sums, energy partials
     nd::SortBufs sort;
     // last tree built by nd_leafbuild_traverse_bvh
     int32_t tree_n = 0, tree_apl = 0;

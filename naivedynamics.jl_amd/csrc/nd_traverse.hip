@@ -866,6 +866,7 @@ int nd_unique_threshold_pairs(nd_ctx *ctx, const float *d_xyz, int32_t n, float 
     const int64_t total = nd::scan_counts(ctx, n);
     *count = total;
     if (total > cap) throw nd::Error(ND_ERR_CAPACITY, "pair buffer too small; *count = required");
+    if (total > 0 && !d_out) throw nd::Error(ND_ERR_ARG, "nd_unique_threshold_pairs: null output buffer");
     k_bruteforce<true><<<nd::grid_for(n), nd::kBlock, 0, ctx->stream>>>(
         d_xyz, n, threshold, nullptr, ctx->offsets.as<int64_t>(), d_out);
     ND_HIP(hipGetLastError());

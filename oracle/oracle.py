@@ -92,6 +92,9 @@ def lib():
         L.o64_forces_pairs.restype = None
         L.o64_vv_run.argtypes = [P, P, P, P, i32, P, ctypes.c_double, i32, i32,
                                  ctypes.c_double, ctypes.c_double, P, P]
+        L.o64_vv_run_thermo.argtypes = [P, P, P, P, i32, P, ctypes.c_double, i32, i32,
+                                        ctypes.c_double, ctypes.c_double, i32, ctypes.c_double,
+                                        ctypes.c_double, P, P]
         L.base_run.argtypes = [P, P, P, i32, P, i32, P, P]
         L.adj64_run.argtypes = [P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P]
         _lib = L
@@ -274,7 +277,12 @@ def forces64_pairs(x, q, pi, pj, params):
     return F, U.value, W.value
 
 
-def vv_run64(x, v, q, m, params, dt, nsteps, walls=False, lo=0.0, hi=0.0):
+def vv_run64(x, v, q, m, params, dt, nsteps, walls=False, lo=0.0, hi=0.0, thermostat_every=0,
+             temperature=1.0, gamma=1.0):
+    """FP64 kick-drift-force-kick velocity Verlet (o64_vv_run_thermo): returns
+    the final (x, v, F) and the (nsteps + 1) x 3 rows (kinetic, potential,
+    virial) of every step; thermostat_every > 0 rescales the velocities as
+    rescale_velocity! does (Simulator.jl:119-144, T from v^2)."""
     x = np.array(x, np.float64).reshape(-1, 3)
     v = np.array(v, np.float64).reshape(-1, 3)
     N = len(x)
@@ -282,8 +290,9 @@ def vv_run64(x, v, q, m, params, dt, nsteps, walls=False, lo=0.0, hi=0.0):
     m = np.ascontiguousarray(m if m is not None else np.ones(N), np.float64)
     F = np.zeros((N, 3), np.float64)
     E = np.zeros((nsteps + 1, 3), np.float64)
-    lib().o64_vv_run(_p(x), _p(v), _p(qq), _p(m), N, ctypes.byref(params), dt, nsteps,
-                     int(walls), lo, hi, _p(F), _p(E))
+    lib().o64_vv_run_thermo(_p(x), _p(v), _p(qq), _p(m), N, ctypes.byref(params), dt, nsteps,
+                            int(walls), lo, hi, int(thermostat_every), temperature, gamma, _p(F),
+                            _p(E))
     return x, v, F, E
 
 

@@ -209,18 +209,37 @@ int64_t o64_forces(const double *x, const double *q, int32_t N, const o64_params
     return np;
 }
 
-int o64_vv_run(double *x, double *v, const double *q, const double *m, int32_t N,
-               const o64_params *p, double dt, int32_t nsteps, int32_t walls, double lo,
-               double hi, double *F, double *energy) {
+/* Velocity rescaling of rescale_velocity! (src/Simulator.jl:119-144), called
+ * every thermostat_every steps (src/Simulator.jl:241-243), with the
+ * temperature from the kinetic energy, T = 2 K / (3 N) (k_B = 1; the
+ * reference sums |v| instead of v^2, Simulator.jl:129-132):
+ *   beta = sqrt(1 + gamma (T0 / T - 1)),  v <- beta v.
+ * It acts on the full-step velocity v_s after step s's force, before the
+ * kick-drift that starts step s + 1 (s > 0 and s a multiple of the period),
+ * so a rescale at the last step of a run is not applied within that run. */
+static void o64_rescale(double *v, int32_t N, double K, double t0, double gamma) {
+    const double T = 2.0 * K / (3.0 * (double)N);
+    double beta = 1.0;
+    if (T > 0.0) {
+        const double b2 = 1.0 + gamma * (t0 / T - 1.0);
+        beta = sqrt(b2 > 0.0 ? b2 : 0.0);
+    }
+    for (int64_t k = 0; k < 3 * (int64_t)N; ++k) v[k] *= beta;
+}
+
+int o64_vv_run_thermo(double *x, double *v, const double *q, const double *m, int32_t N,
+                      const o64_params *p, double dt, int32_t nsteps, int32_t walls, double lo,
+                      double hi, int32_t thermostat_every, double t0, double gamma, double *F,
+                      double *energy) {
     double U, W;
     o64_forces(x, q, N, p, F, &U, &W);
-    if (energy) {
-        double K = 0.0;
-        for (int32_t i = 0; i < N; ++i)
-            K += 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
-        energy[0] = K; energy[1] = U; energy[2] = W;
-    }
+    double K = 0.0;
+    for (int32_t i = 0; i < N; ++i)
+        K += 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    if (energy) { energy[0] = K; energy[1] = U; energy[2] = W; }
     for (int32_t s = 1; s <= nsteps; ++s) {
+        if (thermostat_every > 0 && s - 1 > 0 && (s - 1) % thermostat_every == 0)
+            o64_rescale(v, N, K, t0, gamma);
         for (int32_t i = 0; i < N; ++i) {
             double h = 0.5 * dt / m[i];
             for (int d = 0; d < 3; ++d) {
@@ -233,7 +252,7 @@ int o64_vv_run(double *x, double *v, const double *q, const double *m, int32_t N
             }
         }
         o64_forces(x, q, N, p, F, &U, &W);
-        double K = 0.0;
+        K = 0.0;
         for (int32_t i = 0; i < N; ++i) {
             double h = 0.5 * dt / m[i];
             for (int d = 0; d < 3; ++d) v[3 * i + d] += h * F[3 * i + d];
@@ -242,6 +261,12 @@ int o64_vv_run(double *x, double *v, const double *q, const double *m, int32_t N
         if (energy) { energy[3 * s] = K; energy[3 * s + 1] = U; energy[3 * s + 2] = W; }
     }
     return REF_OK;
+}
+
+int o64_vv_run(double *x, double *v, const double *q, const double *m, int32_t N,
+               const o64_params *p, double dt, int32_t nsteps, int32_t walls, double lo,
+               double hi, double *F, double *energy) {
+    return o64_vv_run_thermo(x, v, q, m, N, p, dt, nsteps, walls, lo, hi, 0, 0.0, 0.0, F, energy);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -85,6 +85,13 @@ void o64_forces_pairs(const double *x, const double *q, int32_t N, const int32_t
 int o64_vv_run(double *x, double *v, const double *q, const double *m, int32_t N,
                const o64_params *p, double dt, int32_t nsteps, int32_t walls, double lo,
                double hi, double *F, double *energy);
+/* The same with the velocity rescaling of rescale_velocity!
+ * (src/Simulator.jl:119-144) every thermostat_every steps (0 = off): target
+ * temperature t0, damping gamma, T from the kinetic energy (k_B = 1). */
+int o64_vv_run_thermo(double *x, double *v, const double *q, const double *m, int32_t N,
+                      const o64_params *p, double dt, int32_t nsteps, int32_t walls, double lo,
+                      double hi, int32_t thermostat_every, double t0, double gamma, double *F,
+                      double *energy);
 
 /* ---- threaded FP32 CPU baseline (reference path + textbook forces) ------ */
 typedef struct {

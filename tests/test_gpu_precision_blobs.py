@@ -1,0 +1,148 @@
+"""GPU: the precision workload (atoms in Gaussian blobs, deep and unbalanced
+LBVH) against the FP64 oracle, in all three precision forms.
+
+Reduced size: 32,768 atoms in 8 blobs (4,096 atoms per blob), the same
+generator, blob std (2 sigma), lattice spacing and mean density (0.8) as the
+1,048,576-atom, 64-blob workload.  Full size: the 1,048,576-atom workload
+itself for the initial forces (the oracle's cell-list FP64 forces finish in
+seconds) and a 100-step run checked through properties.
+
+Tolerances, as for the uniform liquid (relative force error =
+||F_gpu - F_64|| / ||F_64||):
+  FP32 2e-4, MIXED 1e-5, FP64 1e-10 for forces;
+  energies and virial FP32 1e-4, MIXED 1e-6, FP64 1e-12 relative;
+  pairs inside the cutoff: exact in FP64, within 2 per 32,768 atoms in FP32 and
+  MIXED (r^2 < rc^2 is tested in single precision there);
+  50-step trajectory: max |dx| FP32 2e-4, MIXED 1e-7, FP64 1e-10.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BLOB_LEAF = 16
+TOL = {"fp32": (2e-4, 1e-4, 2e-4), "mixed": (1e-5, 1e-6, 1e-7), "fp64": (1e-10, 1e-12, 1e-10)}
+
+
+def _workload(pkg, n, nblobs, seed=3):
+    L = pkg.synth.box_side(n)
+    x = pkg.synth.blob_positions(n, L, seed, nblobs=nblobs)
+    v = pkg.synth.gaussian_velocities(n, 1.0, seed)
+    q = pkg.synth.balanced_charges(n, seed)
+    return x, v, q, L
+
+
+def _spec(pkg, L, precision, **kw):
+    d = dict(duration=0, stepwidth=0.001, rc=2.5, rc_coul=2.5, skin=0.3, rebuild_every=10, lo=0.0,
+             hi=L, precision=precision, energy_every=1, atomsperleaf=BLOB_LEAF)
+    d.update(kw)
+    return pkg.SimSpec(**d)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def small(pkg):
+    return _workload(pkg, 32768, 8)
+
+
+@pytest.fixture(scope="module")
+def full(pkg):
+    return _workload(pkg, 1048576, 64)
+
+
+def test_blob_workload_is_clustered(pkg, small):
+    """The generator gives the depth imbalance the workload is for: the
+    occupancy of 4^3 sub-boxes ranges from empty to lattice density."""
+    x, _, _, L = small
+    h, _ = np.histogramdd(x, bins=8, range=[(0, L)] * 3)
+    cell = (L / 8) ** 3
+    assert h.min() == 0 and h.max() > 1.3 * 0.8 * cell
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "fp64"])
+def test_blob_forces_energies_vs_oracle(pkg, O, small, precision):
+    x, v, q, L = small
+    ftol, etol, _ = TOL[precision]
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision))
+    sim.set_state(x, v, q)
+    sim.run(0)
+    _, _, f = sim.get_state()
+    F, U, W, npairs = O.forces64(x, q, O.o64_params(rc_lj=2.5, rc_coul=2.5))
+    assert _rel(f, F) < ftol, _rel(f, F)
+    e = sim.energies()
+    assert abs(e[0, 2] - U) < etol * abs(U)
+    assert abs(e[0, 3] - W) < etol * abs(W)
+    assert abs(int(e[0, 4]) - npairs) <= (0 if precision == "fp64" else 2)
+    sim.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "fp64"])
+def test_blob_trajectory_vs_oracle(pkg, O, small, precision):
+    x, v, q, L = small
+    ftol, etol, xtol = TOL[precision]
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision))
+    sim.set_state(x, v, q)
+    sim.run(50)
+    xg, vg, fg = sim.get_state()
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 50,
+                               walls=True, lo=0.0, hi=L)
+    assert np.abs(xg - xr).max() < xtol, np.abs(xg - xr).max()
+    assert _rel(fg, fr) < (5 * ftol if precision == "fp32" else max(ftol, 1e-10))
+    e = sim.energies()
+    assert len(e) == 51
+    assert np.allclose(e[:, 1], E[:, 0], rtol=etol)        # kinetic energy per step
+    assert np.allclose(e[:, 2], E[:, 1], rtol=etol, atol=etol * abs(E[0, 1]))
+    assert sim.stats()["rebuilds"] == 6
+    sim.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "fp64"])
+def test_full_size_initial_forces_vs_oracle(pkg, O, full, precision):
+    """1,048,576 atoms in 64 blobs: initial forces, energy, virial and pair
+    count against the FP64 oracle at the workload's own size."""
+    x, v, q, L = full
+    ftol, etol, _ = TOL[precision]
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision))
+    sim.set_state(x, v, q)
+    sim.run(0)
+    _, _, f = sim.get_state()
+    F, U, W, npairs = O.forces64(x, q, O.o64_params(rc_lj=2.5, rc_coul=2.5))
+    assert _rel(f, F) < ftol, _rel(f, F)
+    e = sim.energies()
+    assert abs(e[0, 2] - U) < etol * abs(U)
+    assert abs(e[0, 3] - W) < etol * abs(W)
+    assert abs(int(e[0, 4]) - npairs) <= (0 if precision == "fp64" else 64)
+    sim.close()
+
+
+def test_full_size_run_properties(pkg, full):
+    """100 MIXED steps of the full workload: the LBVH list holds exactly the
+    cell list's pairs within the list radius at the last rebuild, the total
+    force vanishes (Newton's third law, walls exert no force) and the energy
+    drift stays at the plain cutoff's level."""
+    x, v, q, L = full
+    n = len(x)
+    sim = pkg.Simulation(n, _spec(pkg, L, "mixed", energy_every=10))
+    sim.set_state(x, v, q)
+    sim.run(100)
+    xg, vg, fg = sim.get_state()
+    st = sim.stats()
+    e = sim.energies()
+    assert np.all(np.isfinite(xg)) and xg.min() >= 0.0 and xg.max() <= L
+    # sum of forces over all atoms vs the scale of one atom's force
+    assert np.abs(fg.sum(axis=0)).max() < 1e-6 * np.abs(fg).max() * np.sqrt(n)
+    E = e[:, 1] + e[:, 2]
+    assert abs(E[-1] - E[0]) < 0.05 * abs(E[0])
+    # the list radius as the simulation pads it (FP32 rounding of the box)
+    rl = 2.5 + 0.3
+    rlist = float(np.float32(rl + (L + rl) * 9.5367431640625e-07))
+    # the step-100 list was built from the positions reached at step 100
+    xs = xg.astype(np.float32)
+    count = pkg.cells.neighborlist_device(pkg.DeviceArray.from_host(pkg.default_context(), xs),
+                                          rlist, pkg.PRED_SCALAR)[1]
+    assert st["rebuilds"] == 11
+    assert abs(st["list_entries"] // 2 - count) <= 1e-6 * count, (st["list_entries"] // 2, count)
+    sim.close()

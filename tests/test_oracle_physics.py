@@ -130,3 +130,65 @@ def test_cpu_baseline_runs_and_conserves(O):
     xr, vr, _, _ = O.vv_run64(x, v, q, None, p, 0.001, 20, walls=True, lo=0.0, hi=L)
     assert np.abs(xb - xr).max() < 1e-3
     assert pe > 0 and np.all(times >= 0)
+
+
+def _free_gas(n, seed, L=1000.0):
+    """n atoms far apart (no pair inside the cutoff): forces vanish, so the
+    only change of velocity is the thermostat's rescaling."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(n ** (1 / 3)))
+    g = np.stack(np.unravel_index(np.arange(n), (side, side, side)), 1).astype(float)
+    x = (g + 0.5) * (L / side)
+    v = rng.normal(size=(n, 3))
+    return x, v
+
+
+@pytest.mark.parametrize("gamma", [1.0, 0.5, 0.0])
+def test_thermostat_relaxes_temperature_in_closed_form(O, gamma):
+    """rescale_velocity! (Simulator.jl:119-144) with T = 2K/(3N): with no
+    forces the velocities change only by the rescale, v <- beta v with
+    beta^2 = 1 + gamma (T0/T - 1), so after a rescale T' = T + gamma (T0 - T).
+    It acts on v_s at every step s > 0 that is a multiple of the period,
+    before the kick-drift of step s + 1 (never at s = 0 or within the last
+    step of a run)."""
+    n, T0 = 64, 2.5
+    x, v = _free_gas(n, 6)
+    p = O.o64_params(rc_lj=2.5)
+    x1, v1, F, E = O.vv_run64(x, v, None, None, p, 0.001, 25, thermostat_every=10,
+                              temperature=T0, gamma=gamma)
+    assert np.abs(F).max() == 0.0
+    T = 2.0 * E[:, 0] / (3 * n)
+    T_init = T[0]
+    # steps 0..10 unchanged (no rescale at step 0), 11..20 after one rescale
+    assert np.allclose(T[:11], T_init, rtol=1e-13)
+    T1 = T_init + gamma * (T0 - T_init)
+    assert np.allclose(T[11:21], T1, rtol=1e-12)
+    T2 = T1 + gamma * (T0 - T1)
+    assert np.allclose(T[21:], T2, rtol=1e-12)
+    # the direction of every velocity is kept (a uniform scale)
+    beta = np.sqrt(T2 / T_init)
+    assert np.allclose(v1, beta * v, rtol=1e-12)
+    # the rescale at the run's last step (25 is no multiple; 20 is) was applied
+    # before step 21; a run of exactly 20 steps ends before it
+    _, v20, _, E20 = O.vv_run64(x, v, None, None, p, 0.001, 20, thermostat_every=10,
+                                temperature=T0, gamma=gamma)
+    assert np.allclose(2.0 * E20[-1, 0] / (3 * n), T1, rtol=1e-12)
+
+
+def test_thermostat_floor_at_zero(O):
+    """beta = sqrt(max(0, 1 + gamma (T0/T - 1))): gamma > T/(T - T0) stops the atoms."""
+    x, v = _free_gas(27, 7)
+    p = O.o64_params(rc_lj=2.5)
+    _, v1, _, E = O.vv_run64(x, v, None, None, p, 0.001, 12, thermostat_every=10,
+                             temperature=0.0, gamma=2.0)
+    assert np.all(v1 == 0.0) and E[-1, 0] == 0.0
+
+
+def test_thermostat_period_zero_is_plain_vv(O):
+    x, q, L = _liquid(343, 8)
+    v = np.random.default_rng(8).normal(size=x.shape)
+    p = O.o64_params(rc_lj=2.5, rc_coul=2.5)
+    a = O.vv_run64(x, v, q, None, p, 0.001, 30)
+    b = O.vv_run64(x, v, q, None, p, 0.001, 30, thermostat_every=0, temperature=5.0, gamma=1.0)
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)
