@@ -239,6 +239,12 @@ typedef struct {
     int32_t max_neighbors;    /* largest per-atom count at the last build */
     int32_t capacity;         /* current per-atom capacity */
     int64_t pairs_in_cutoff;  /* unique pairs with r < cutoff at the last energy step */
+    int64_t stale_reruns;     /* nd_sim_run calls redone because an atom had moved half the
+                                 skin (by its path length since the list build) before a
+                                 scheduled rebuild */
+    int64_t forced_rebuilds;  /* list rebuilds outside the rebuild_every schedule in those
+                                 reruns (made before the first force the stale list would
+                                 have served) */
 } nd_sim_stats;
 
 typedef struct nd_sim nd_sim;
@@ -251,8 +257,21 @@ int nd_sim_set_state(nd_sim *sim, const double *h_x, const double *h_v, const do
                      const double *h_m);
 int nd_sim_get_state(nd_sim *sim, double *h_x, double *h_v, double *h_f);
 /* Advance nsteps velocity-Verlet steps (kick-drift-force-kick), rebuilding
- * the neighbour list every rebuild_every steps.  Returns after completion. */
+ * the neighbour list every rebuild_every steps.  The list holds the pairs
+ * within max(rc, rc_coul) + skin; each atom's path length since the build
+ * is tracked, and if an atom reaches skin / 2 before a scheduled rebuild the
+ * steps are run again from the call's starting state with a rebuild before
+ * every force that list could no longer serve (stats: stale_reruns,
+ * forced_rebuilds).  Returns after completion. */
 int nd_sim_run(nd_sim *sim, int32_t nsteps);
+/* No reference counterpart (the reference has no captured run loop): sizes
+ * the neighbour list and computes the current force if that is not done yet
+ * (as nd_sim_run(sim, 0) does), then captures and instantiates every rebuild-
+ * window hipGraph that nd_sim_run can replay (every buffer parity and window
+ * kind), so that later runs spend no time on capture.  Advances no step.
+ * Graphs are dropped again when the list capacity grows or the state or
+ * replicas are replaced. */
+int nd_sim_prepare(nd_sim *sim);
 /* Energy rows (step, kinetic, potential, virial, pairs_in_cutoff) recorded
  * since the last call; *rows receives the number written. */
 int nd_sim_energies(nd_sim *sim, double *h_rows, int32_t max_rows, int32_t *rows);

@@ -81,7 +81,8 @@ class AdjParams(ctypes.Structure):
 class SimStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("rebuilds", ctypes.c_int64),
                 ("list_entries", ctypes.c_int64), ("max_neighbors", ctypes.c_int32),
-                ("capacity", ctypes.c_int32), ("pairs_in_cutoff", ctypes.c_int64)]
+                ("capacity", ctypes.c_int32), ("pairs_in_cutoff", ctypes.c_int64),
+                ("stale_reruns", ctypes.c_int64), ("forced_rebuilds", ctypes.c_int64)]
 
 
 _P = ctypes.c_void_p
@@ -127,6 +128,7 @@ SIGNATURES = {
     "nd_sim_set_state": [_P, _P, _P, _P, _P],
     "nd_sim_get_state": [_P, _P, _P, _P],
     "nd_sim_run": [_P, _i32],
+    "nd_sim_prepare": [_P],
     "nd_sim_energies": [_P, _P, _i32, ctypes.POINTER(_i32)],
     "nd_sim_stats_get": [_P, ctypes.POINTER(SimStats)],
     "nd_sim_kernel_timing": [_P, _i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)],

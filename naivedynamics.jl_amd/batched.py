@@ -125,6 +125,9 @@ class BatchedSimulation:
     def run(self, nsteps):
         self.sim.run(nsteps)
 
+    def prepare(self):
+        self.sim.prepare()
+
     def energies(self):
         return self.sim.energies()
 
@@ -161,6 +164,7 @@ def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None):
         b.set_state()
         W = s2.rebuild_every
         b.run(W)
+        b.prepare()                         # every window graph captured before timing
         b.ctx.synchronize()
         b.ctx.timer_start()
         b.run(windows * W)
@@ -182,7 +186,8 @@ def run_one_per_replica(reps, spec, nsteps, ctx=None):
     for R in reps:
         b = BatchedSimulation([R], spec, ctx=ctx)
         b.set_state()
-        b.run(spec.rebuild_every)           # warm-up window (graph capture)
+        b.run(spec.rebuild_every)           # warm-up window
+        b.prepare()                         # every window graph captured before timing
         b.ctx.synchronize()
         b.ctx.timer_start()
         b.run(nsteps)

@@ -47,7 +47,11 @@ struct SimMisc {
     int64_t step;
     int64_t partial_step;
     double ke;
-    double pad[3];
+    // set by a drift when an atom's path length since the last list build
+    // reaches half the skin while the list is still to be used (see path_add)
+    int32_t stale;
+    int32_t pad0;
+    double pad[2];
 };
 static_assert(offsetof(SimMisc, maxcnt) == 0 && offsetof(SimMisc, overflow) == 4 &&
                   offsetof(SimMisc, status) == 8,
@@ -58,6 +62,7 @@ struct StepParams {
     float eps24, eps4, sigma2, rc2, rcc2, rcmax2, kc, elj_shift, ecoul_inv_rc;
     float dtf, lof, hif;
     float fx_inv;  // fixed-point unit in coordinate units (MIXED)
+    float path_scale;  // path-length units per coordinate unit (2^14 units = skin / 2)
     double deps24, deps4, dsigma2, drc2, drcc2, drcmax2, dkc, delj_shift, decoul_inv_rc;
     double dt, lo, hi;
     double fx_c, fx_s;  // fixed-point frame: X = round((x - fx_c) * fx_s) (MIXED)
@@ -104,6 +109,37 @@ __device__ __forceinline__ void reflect(P4 &x, P4 &v, const StepParams &P, const
     if (x.y > hi[1]) { x.y = hi[1]; v.y = -v.y; }
     if (x.z < lo[2]) { x.z = lo[2]; v.z = -v.z; }
     if (x.z > hi[2]) { x.z = hi[2]; v.z = -v.z; }
+}
+
+// List validity.  Each drift adds the atom's displacement |x_new - x_old|
+// (after the walls) to its path length since the last list build, kept as a
+// 16-bit count of units of (skin / 2) / 2^14, rounded up and saturating; the
+// path length bounds how far the atom can be from where it was when the list
+// was built.  While every atom has moved less than skin / 2, every pair
+// within the cutoff was within cutoff + skin at the build, so it is in the
+// list.  A drift whose positions will be used with the current list (check;
+// false right before a rebuild) flags misc->stale when the atom reaches
+// 2^14 units; the host then runs the steps again, rebuilding where needed.
+// first: the first drift after a build (the path starts at 0).
+__device__ __forceinline__ void path_add(uint16_t *__restrict__ path, int32_t i, float d, float scale,
+                                         int32_t first, int32_t check, SimMisc *misc) {
+    const float t = fminf(ceilf(d * scale), 65535.0f);  // NaN -> 65535
+    const uint32_t q = first ? 0u : (uint32_t)path[i];
+    const uint32_t s = min(q + (uint32_t)t, 65535u);
+    path[i] = (uint16_t)s;
+    if (check && s >= 16384u) atomicOr(&misc->stale, 1);
+}
+
+template <class P4>
+__device__ __forceinline__ float displacement(const P4 &a, const P4 &b) {
+    const double dx = (double)a.x - (double)b.x, dy = (double)a.y - (double)b.y,
+                 dz = (double)a.z - (double)b.z;
+    return (float)sqrt(dx * dx + dy * dy + dz * dz);
+}
+template <>
+__device__ __forceinline__ float displacement<float4>(const float4 &a, const float4 &b) {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
 template <class S> struct V4;
@@ -281,7 +317,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
        typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
        const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, StepParams P,
        double *__restrict__ partials, SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in,
-       int4 *__restrict__ fix_out, Walls W) {
+       int4 *__restrict__ fix_out, Walls W, uint16_t *__restrict__ path, int32_t first, int32_t check) {
     using P4 = typename V4<PosS>::T;
     // MIXED gathers the fixed-point copy (16 B per neighbour)
     constexpr bool FX = sizeof(PosS) == 8 && sizeof(CalcS) == 4;
@@ -407,6 +443,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 x.z += dt * v.z;
                 if (P.walls) reflect<P4, PosS>(x, v, P, W, i);
                 pos_out[i] = x;
+                path_add(path, i, displacement(x, pi), P.path_scale, first, check, misc);
                 if constexpr (FX)
                     fix_out[i] = make_int4(to_fix(x.x, P, misc), to_fix(x.y, P, misc),
                                            to_fix(x.z, P, misc), __float_as_int((float)x.w));
@@ -439,7 +476,8 @@ __global__ void __launch_bounds__(nd::kBlock)
 k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
              typename V4<PosS>::T *__restrict__ vel, const typename V4<PosS>::T *__restrict__ frc,
              StepParams P, SimMisc *__restrict__ misc, int32_t rescale, double t0, double gamma,
-             int4 *__restrict__ fix_out, Walls W) {
+             int4 *__restrict__ fix_out, Walls W, uint16_t *__restrict__ path, int32_t first,
+             int32_t check) {
     using P4 = typename V4<PosS>::T;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P.n) {
@@ -458,12 +496,14 @@ k_kick_drift(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>:
         v.x += h * f.x;
         v.y += h * f.y;
         v.z += h * f.z;
-        P4 x = pos_in[i];
+        const P4 x0 = pos_in[i];
+        P4 x = x0;
         x.x += dt * v.x;
         x.y += dt * v.y;
         x.z += dt * v.z;
         if (P.walls) reflect<P4, PosS>(x, v, P, W, i);
         pos_out[i] = x;
+        path_add(path, i, displacement(x, x0), P.path_scale, first, check, misc);
         if (fix_out)
             fix_out[i] = make_int4(to_fix((double)x.x, P, misc), to_fix((double)x.y, P, misc),
                                    to_fix((double)x.z, P, misc), __float_as_int((float)x.w));
@@ -1117,6 +1157,7 @@ struct nd_sim {
     nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, fix[2], codes, perm, nodes, store, kids, nbr,
         ncount, misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig, rep, boxes;
     nd::DevBuf cpos, cstart;  // cell-list walk: cell-sorted positions, cell starts
+    nd::DevBuf path;          // uint16 path length of each atom since the last build
     nd::CellGrid cg{};
     int32_t nrep = 0;  // packed replicas (0: one system)
     nd::SortBufs sort;
@@ -1125,6 +1166,8 @@ struct nd_sim {
     int32_t cap = 0;
     int32_t max_rows = 1 << 16;
     int64_t step = 0, rebuilds = 0, last_build_step = -1;
+    int32_t since = 0;  // drifts launched since the last list build
+    int64_t stale_reruns = 0, forced_rebuilds = 0;
     bool have_force = false, list_valid = false, have_state = false;
     // the tree in nodes/store matches the current atom order (a refit may reuse it)
     bool tree_valid = false;
@@ -1197,6 +1240,7 @@ void launch_sort_and_build(nd_sim *s);
 // Neighbour-list rebuild: a full re-sort + tree build, or a box refit of the
 // current tree, followed by the list walk.
 void launch_rebuild(nd_sim *s, bool full) {
+    s->since = 0;  // the next drift starts every path length at 0
     hipStream_t st = s->ctx->stream;
     const int32_t n = s->n;
     const unsigned g = nd::grid_for(n);
@@ -1337,8 +1381,10 @@ void launch_sort_and_build(nd_sim *s) {
     ND_HIP(hipGetLastError());
 }
 
+// check: the positions this launch drifts to will be used with the current
+// list (the next step is no rebuild step); a fused launch counts as a drift.
 template <class PosS, class CalcS, class AccS, int MODE, bool E>
-void launch_step_t(nd_sim *s) {
+void launch_step_t(nd_sim *s, bool check) {
     using P4 = typename V4<PosS>::T;
     hipStream_t st = s->ctx->stream;
     const unsigned g = nd::grid_for(s->n);
@@ -1346,17 +1392,23 @@ void launch_step_t(nd_sim *s) {
     k_step<PosS, CalcS, AccS, MODE, E><<<g, nd::kBlock, 0, st>>>(
         s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
-        s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s));
+        s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s), s->path.as<uint16_t>(),
+        s->since == 0 ? 1 : 0, check ? 1 : 0);
+    if (MODE == kModeFused) s->since++;
 }
 
 template <int MODE, bool E>
-void launch_step_prec(nd_sim *s) {
+void launch_step_prec(nd_sim *s, bool check) {
     switch (s->p.precision) {
-        case ND_PREC_FP32: launch_step_t<float, float, float, MODE, E>(s); break;
-        case ND_PREC_MIXED: launch_step_t<double, float, double, MODE, E>(s); break;
-        default: launch_step_t<double, double, double, MODE, E>(s); break;
+        case ND_PREC_FP32: launch_step_t<float, float, float, MODE, E>(s, check); break;
+        case ND_PREC_MIXED: launch_step_t<double, float, double, MODE, E>(s, check); break;
+        default: launch_step_t<double, double, double, MODE, E>(s, check); break;
     }
 }
+
+// Whether the positions reached at step k are used with the current list
+// (false when step k rebuilds it).
+bool uses_list(const nd_sim *s, int64_t k) { return k % s->p.rebuild_every != 0; }
 
 // One force-evaluating kernel for global step k, plus its energy row.
 void launch_force_step(nd_sim *s, int mode, int64_t k) {
@@ -1367,9 +1419,10 @@ void launch_force_step(nd_sim *s, int mode, int64_t k) {
         e1 = next_event(s);
         ND_HIP(hipEventRecord(e0, s->ctx->stream));
     }
-    if (mode == kModeForce) e ? launch_step_prec<kModeForce, true>(s) : launch_step_prec<kModeForce, false>(s);
-    else if (mode == kModeFused) e ? launch_step_prec<kModeFused, true>(s) : launch_step_prec<kModeFused, false>(s);
-    else e ? launch_step_prec<kModeFinal, true>(s) : launch_step_prec<kModeFinal, false>(s);
+    const bool c = uses_list(s, k + 1);  // a fused launch drifts to step k + 1
+    if (mode == kModeForce) e ? launch_step_prec<kModeForce, true>(s, c) : launch_step_prec<kModeForce, false>(s, c);
+    else if (mode == kModeFused) e ? launch_step_prec<kModeFused, true>(s, c) : launch_step_prec<kModeFused, false>(s, c);
+    else e ? launch_step_prec<kModeFinal, true>(s, c) : launch_step_prec<kModeFinal, false>(s, c);
     ND_HIP(hipGetLastError());
     if (s->timing) ND_HIP(hipEventRecord(e1, s->ctx->stream));
     if (mode == kModeFused) s->cur ^= 1;
@@ -1382,22 +1435,26 @@ void launch_force_step(nd_sim *s, int mode, int64_t k) {
     }
 }
 
-void launch_kick_drift(nd_sim *s, bool rescale) {
+// Kick-drift from step k to k + 1 (after the thermostat's rescale if asked).
+void launch_kick_drift(nd_sim *s, bool rescale, int64_t k) {
     hipStream_t st = s->ctx->stream;
     const unsigned g = nd::grid_for(s->n);
     const int a = s->cur ^ 1;
+    const int32_t first = s->since == 0 ? 1 : 0, check = uses_list(s, k + 1) ? 1 : 0;
+    uint16_t *path = s->path.as<uint16_t>();
     if (s->wide)
         k_kick_drift<double><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), s->pos[a].as<double4>(),
                                                        s->vel[s->vcur].as<double4>(), s->frc.as<double4>(), s->sp,
                                                        s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp,
-                                                       fix_ptr(s, a), walls_of(s));
+                                                       fix_ptr(s, a), walls_of(s), path, first, check);
     else
         k_kick_drift<float><<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<float4>(), s->pos[a].as<float4>(),
                                                       s->vel[s->vcur].as<float4>(), s->frc.as<float4>(), s->sp,
                                                       s->misc.as<SimMisc>(), rescale, s->p.temperature, s->p.vdamp,
-                                                      nullptr, walls_of(s));
+                                                      nullptr, walls_of(s), path, first, check);
     ND_HIP(hipGetLastError());
     s->cur ^= 1;
+    s->since++;
 }
 
 bool thermo_step(const nd_sim *s, int64_t k) {
@@ -1412,7 +1469,7 @@ void launch_middle_step(nd_sim *s, int64_t k) {
         s->p.energy_every = 1;
         launch_force_step(s, kModeFinal, k);
         s->p.energy_every = ee;
-        launch_kick_drift(s, true);
+        launch_kick_drift(s, true, k);
     } else {
         launch_force_step(s, kModeFused, k);
     }
@@ -1489,36 +1546,56 @@ void launch_window(nd_sim *s, int64_t k0, bool full) {
     for (int32_t o = 0; o < s->p.rebuild_every; ++o) launch_middle_step(s, k0 + o);
 }
 
+// Graph of the window that starts at k0 with position buffer cur, velocity
+// buffer vcur and the given kind (re-sort or refit), captured and
+// instantiated once; capture executes nothing and leaves the host-side state
+// as it found it.
+hipGraphExec_t window_graph(nd_sim *s, int64_t k0, int cur, int vcur, bool full) {
+    const int key = cur | (vcur << 1) | (full ? 4 : 0);
+    if (s->gexec[key]) return s->gexec[key];
+    hipStream_t st = s->ctx->stream;
+    hipGraph_t graph;
+    const int cur0 = s->cur, vcur0 = s->vcur, since0 = s->since;
+    const int64_t rb0 = s->rebuilds, lb0 = s->last_build_step;
+    const bool tv0 = s->tree_valid;
+    s->cur = cur;
+    s->vcur = vcur;
+    s->tree_valid = true;  // a refit window is captured only for a valid tree
+    ND_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        launch_window(s, k0, full);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        s->cur = cur0;
+        s->vcur = vcur0;
+        s->since = since0;
+        s->rebuilds = rb0;
+        s->last_build_step = lb0;
+        s->tree_valid = tv0;
+        throw;
+    }
+    s->cur = cur0;
+    s->vcur = vcur0;
+    s->since = since0;
+    s->rebuilds = rb0;
+    s->last_build_step = lb0;
+    s->tree_valid = tv0;
+    ND_HIP(hipStreamEndCapture(st, &graph));
+    const hipError_t e = hipGraphInstantiate(&s->gexec[key], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    ND_HIP(e);
+    return s->gexec[key];
+}
+
 void run_window_graph(nd_sim *s, int64_t k0) {
     const bool full = resort_due(s, k0);
-    const int key = s->cur | (s->vcur << 1) | (full ? 4 : 0);
-    if (!s->gexec[key]) {
-        hipStream_t st = s->ctx->stream;
-        hipGraph_t graph;
-        const int cur0 = s->cur, vcur0 = s->vcur;
-        const int64_t rb0 = s->rebuilds;
-        const bool tv0 = s->tree_valid;
-        ND_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        try {
-            launch_window(s, k0, full);
-        } catch (...) {
-            (void)hipStreamEndCapture(st, &graph);
-            throw;
-        }
-        ND_HIP(hipStreamEndCapture(st, &graph));
-        ND_HIP(hipGraphInstantiate(&s->gexec[key], graph, nullptr, nullptr, 0));
-        ND_HIP(hipGraphDestroy(graph));
-        s->cur = cur0;  // capture did not execute anything
-        s->vcur = vcur0;
-        s->rebuilds = rb0;
-        s->tree_valid = tv0;
-    }
-    ND_HIP(hipGraphLaunch(s->gexec[key], s->ctx->stream));
+    ND_HIP(hipGraphLaunch(window_graph(s, k0, s->cur, s->vcur, full), s->ctx->stream));
     // Host-side bookkeeping of launch_window: a re-sort swaps the state
     // buffers once and every middle step swaps the positions once more (a
     // fused step, or a final step followed by a kick-drift).
     s->rebuilds++;
     s->last_build_step = k0;
+    s->since = s->p.rebuild_every;  // one drift per step of the window
     if (full) {
         s->cur ^= (1 + s->p.rebuild_every) & 1;
         s->vcur ^= 1;
@@ -1528,9 +1605,25 @@ void run_window_graph(nd_sim *s, int64_t k0) {
     }
 }
 
-void run_once(nd_sim *s, int32_t nsteps) {
+// Reads and clears the stale flag (synchronises).
+bool take_stale(nd_sim *s) {
+    SimMisc m;
+    read_misc(s, m);
+    if (!m.stale) return false;
+    m.stale = 0;
+    ND_HIP(hipMemcpyAsync(&reinterpret_cast<SimMisc *>(s->misc.p)->stale, &m.stale, sizeof(int32_t),
+                          hipMemcpyHostToDevice, s->ctx->stream));
+    ND_HIP(hipStreamSynchronize(s->ctx->stream));
+    return true;
+}
+
+// checked: eager launches with the stale flag read after every drift; a
+// flagged drift makes the next step rebuild the list first (a forced
+// rebuild), so no force is evaluated with a list that may miss a pair.
+void run_once(nd_sim *s, int32_t nsteps, bool checked) {
     const int32_t W = s->p.rebuild_every;
     int64_t k = s->step;
+    const bool had_force = s->have_force;
     if (!s->list_valid) {
         // the stored force is in the order before this permutation: recompute it
         rebuild_checked(s);
@@ -1538,35 +1631,45 @@ void run_once(nd_sim *s, int32_t nsteps) {
         s->have_force = false;
     }
     if (!s->have_force) {
+        // a recomputed force (after a rerun's re-sort) records no second
+        // energy row for step k
+        const int32_t ee = s->p.energy_every;
+        if (had_force) s->p.energy_every = 0;
         launch_force_step(s, kModeForce, k);
+        s->p.energy_every = ee;
         s->have_force = true;
     }
     if (nsteps == 0) return;
     // step k -> k+1 starts with a kick-drift from the stored force (after the
     // velocity rescale if step k is a thermostat step; the previous run's
     // final step recorded KE(v_k) for it)
-    launch_kick_drift(s, thermo_step(s, k));
+    launch_kick_drift(s, thermo_step(s, k), k);
+    bool forced = checked && take_stale(s);
     const int64_t kend = k + nsteps;  // integer step reached at the end
     int64_t j = k + 1;                // next force evaluation index
-    const bool use_graph = graph_eligible(s);
+    const bool use_graph = !checked && graph_eligible(s);
     while (j < kend) {
-        if (j % W == 0) {
-            if (use_graph && j + W <= kend) {
+        if (j % W == 0 || forced) {
+            if (use_graph && j % W == 0 && j + W <= kend) {
                 run_window_graph(s, j);
                 j += W;
                 continue;
             }
             launch_rebuild(s, resort_due(s, j));
             s->rebuilds++;
+            if (j % W != 0) s->forced_rebuilds++;
             s->last_build_step = j;
+            forced = false;
         }
         launch_middle_step(s, j);
+        if (checked) forced = take_stale(s);
         ++j;
     }
     // final integer step: rebuild if due, force + finishing half kick
-    if (kend % W == 0 && s->last_build_step != kend) {
+    if ((kend % W == 0 && s->last_build_step != kend) || forced) {
         launch_rebuild(s, resort_due(s, kend));
         s->rebuilds++;
+        if (kend % W != 0) s->forced_rebuilds++;
         s->last_build_step = kend;
     }
     if (thermo_step(s, kend)) {
@@ -1614,6 +1717,9 @@ void set_step_params(nd_sim *s) {
     q.fx_c = 0.5 * (p.lo + p.hi);
     q.fx_s = std::ldexp(1.0, (int)std::floor(std::log2(0.999 * 2147483648.0 / half)));
     q.fx_inv = (float)(1.0 / q.fx_s);
+    // 2^14 path-length units make half the skin; with no skin any motion
+    // saturates the count (the list is valid only at its own build)
+    q.path_scale = p.skin > 0.0 ? (float)(16384.0 / (0.5 * p.skin)) : 3.0e38f;
 }
 
 }  // namespace
@@ -1677,6 +1783,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(s->nL));
         s->kids.reserve(8 * sizeof(float4) * (size_t)(s->nL - 1));  // four-wide records
         s->ncount.reserve(sizeof(int32_t) * n);
+        s->path.reserve(sizeof(uint16_t) * n);
         s->misc.reserve(sizeof(SimMisc));
         s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
         s->rows.reserve(sizeof(double) * 5 * (size_t)s->max_rows);
@@ -1724,7 +1831,7 @@ int nd_sim_destroy(nd_sim *s) {
                           &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids, &s->rep, &s->boxes,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
                           &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
-                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos, &s->cstart};
+                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos, &s->cstart, &s->path};
     for (auto *b : bufs) b->release();
     delete s;
     return ND_OK;
@@ -1823,15 +1930,17 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
     ND_REQUIRE(s->have_state, "nd_sim_run: call nd_sim_set_state first");
     ND_HIP(hipSetDevice(ctx->device));
     s->ev_used = 0;
-    for (int attempt = 0; attempt < 6; ++attempt) {
-        // snapshot for a rerun if the neighbour list overflows mid-run
+    bool checked = false;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        // snapshot for a rerun if the neighbour list overflows or goes stale
         SimMisc m0;
         read_misc(s, m0);
-        const int cur0 = s->cur, vcur0 = s->vcur;
-        const int64_t step0 = s->step, rb0 = s->rebuilds, lb0 = s->last_build_step;
-        const bool hf0 = s->have_force, lv0 = s->list_valid;
+        const int cur0 = s->cur, vcur0 = s->vcur, since0 = s->since;
+        const int64_t step0 = s->step, rb0 = s->rebuilds, lb0 = s->last_build_step,
+                      fr0 = s->forced_rebuilds;
+        const bool hf0 = s->have_force;
         snapshot(s);
-        run_once(s, nsteps);
+        run_once(s, nsteps, checked);
         SimMisc m;
         read_misc(s, m);
         if (m.status & nd::kStatusDomain)
@@ -1839,7 +1948,7 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
         if (m.status & nd::kStatusTree)
             throw nd::Error(ND_ERR_STATE, "simulation: malformed neighbour tree");
         s->last_maxcnt = m.maxcnt;
-        if (!m.overflow) {
+        if (!m.overflow && !m.stale) {
             if (s->timing && s->ev_used >= 2) {
                 // every force launch recorded an event pair
                 for (size_t e = 0; e + 1 < s->ev_used; e += 2) {
@@ -1851,26 +1960,60 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
             }
             return ND_OK;
         }
-        // overflow: grow, restore the snapshot and run again
+        // restore the snapshot and run again: with a larger list after an
+        // overflow, in checked mode after a stale list (an atom moved half
+        // the skin before a scheduled rebuild)
         s->cur = cur0;
         s->vcur = vcur0;
+        s->since = since0;
         restore(s);
-        alloc_lists(s, m.maxcnt + m.maxcnt / 4 + 8);
-        set_step_params(s);
+        if (m.overflow) {
+            alloc_lists(s, m.maxcnt + m.maxcnt / 4 + 8);
+            set_step_params(s);
+            s->list_valid = false;
+            s->tree_valid = false;
+        } else {
+            // later rebuilds of the failed attempt re-sorted the atoms: the
+            // rerun builds a list for the restored state first
+            s->list_valid = false;
+            s->tree_valid = false;
+            checked = true;
+            s->stale_reruns++;
+        }
         m0.overflow = 0;
         m0.maxcnt = 0;
+        m0.stale = 0;
         ND_HIP(hipMemcpyAsync(s->misc.p, &m0, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
         ND_HIP(hipStreamSynchronize(ctx->stream));
         s->step = step0;
         s->rebuilds = rb0;
+        s->forced_rebuilds = fr0;
         s->last_build_step = lb0;
         s->have_force = hf0;
-        s->list_valid = false;
-        s->tree_valid = false;
-        (void)lv0;
         s->ev_used = 0;
     }
     throw nd::Error(ND_ERR_CAPACITY, "nd_sim_run: neighbour list kept overflowing");
+    ND_API_END(ctx)
+}
+
+int nd_sim_prepare(nd_sim *s) {
+    nd_ctx *ctx = s ? s->ctx : nullptr;
+    ND_API_BEGIN
+    ND_REQUIRE(s, "nd_sim_prepare: null simulation");
+    ND_REQUIRE(s->have_state, "nd_sim_prepare: call nd_sim_set_state first");
+    ND_HIP(hipSetDevice(ctx->device));
+    // size the list (and compute the first force) as a run of zero steps does
+    if (!s->list_valid || !s->have_force) {
+        const int rc = nd_sim_run(s, 0);
+        if (rc != ND_OK) return rc;
+    }
+    if (!graph_eligible(s)) return ND_OK;
+    const int32_t W = s->p.rebuild_every;
+    const bool refits = resort_period(s) > W;
+    for (int kind = 0; kind < (refits ? 2 : 1); ++kind)
+        for (int cur = 0; cur < 2; ++cur)
+            for (int vcur = 0; vcur < 2; ++vcur) window_graph(s, W, cur, vcur, kind == 0);
+    ND_HIP(hipStreamSynchronize(ctx->stream));
     ND_API_END(ctx)
 }
 
@@ -1909,6 +2052,8 @@ int nd_sim_stats_get(nd_sim *s, nd_sim_stats *out) {
     out->max_neighbors = s->last_maxcnt;
     out->capacity = s->cap;
     out->pairs_in_cutoff = s->pairs_last;
+    out->stale_reruns = s->stale_reruns;
+    out->forced_rebuilds = s->forced_rebuilds;
     ND_API_END(ctx)
 }
 
@@ -1925,9 +2070,9 @@ int nd_sim_time_fused_window(nd_sim *s, int32_t nsteps, double *avg_fused_ms) {
     rebuild_checked(s);  // fresh list for this window (checks capacity)
     s->last_build_step = s->step;
     launch_force_step(s, kModeForce, s->step);
-    launch_kick_drift(s, false);
+    launch_kick_drift(s, false, s->step);
     ND_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    for (int32_t o = 1; o < nsteps; ++o) launch_step_prec<kModeFused, false>(s), s->cur ^= 1;
+    for (int32_t o = 1; o < nsteps; ++o) launch_step_prec<kModeFused, false>(s, true), s->cur ^= 1;
     ND_HIP(hipGetLastError());
     ND_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     launch_force_step(s, kModeFinal, s->step + nsteps);

@@ -123,6 +123,11 @@ class Simulation:
     def run(self, nsteps):
         check(load().nd_sim_run(self.handle, int(nsteps)), self.ctx, "simulate!")
 
+    def prepare(self):
+        """Size the list and capture every rebuild-window graph now, so that
+        later runs spend no time on capture (nd_sim_prepare)."""
+        check(load().nd_sim_prepare(self.handle), self.ctx, "nd_sim_prepare")
+
     def energies(self, max_rows=1 << 16):
         rows = np.empty((max_rows, 5), np.float64)
         r = ctypes.c_int32(0)

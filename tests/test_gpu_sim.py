@@ -196,15 +196,34 @@ def test_size_limit_is_an_argument_error(pkg):
         pkg.Simulation(n, _spec(pkg, 1000.0, "fp32"))
 
 
-def test_thermostat_moves_temperature(pkg):
-    x, v, q, L = _cfg(pkg, 4096, 6, coul=False)
-    sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", coul=False, thermostat_every=10,
-                                       temperature=2.0, vDamp=1.0, energy_every=10))
-    sim.set_state(x, v * 0.5, None)
-    sim.run(100)
+@pytest.mark.parametrize("precision,xtol", [("fp32", 2e-4), ("fp64", 1e-10)])
+@pytest.mark.parametrize("splits", [(100,), (25, 75), (20, 30, 50)])
+def test_thermostat_trajectory_vs_oracle(pkg, O, precision, xtol, splits):
+    """rescale_velocity! every 10 steps (Simulator.jl:119-144, 241-243; T from
+    v^2) towards T0 = 1.5 with damping 0.5, LJ + Coulomb, against the FP64
+    oracle's thermostatted velocity Verlet; runs split at and between
+    thermostat steps (a rescale due at a run's last step is applied by the
+    next run)."""
+    x, v, q, L = _cfg(pkg, 4096, 6)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision, thermostat_every=10, temperature=1.5,
+                                       vDamp=0.5, energy_every=10))
+    sim.set_state(x, v, q)
+    for k in splits:
+        sim.run(k)
+    xg, vg, fg = sim.get_state()
     e = sim.energies()
-    T = 2 * e[:, 1] / (3 * len(x))
-    assert T[0] < 0.5 and abs(T[-1] - 2.0) < 0.5
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 100,
+                               walls=True, lo=0.0, hi=L, thermostat_every=10, temperature=1.5,
+                               gamma=0.5)
+    assert np.abs(xg - xr).max() < xtol
+    assert np.abs(vg - vr).max() < (2e-3 if precision == "fp32" else 1e-9)
+    # the kinetic energy rows at steps 0, 10, ..., 100 (before each rescale)
+    ktol = {"fp32": 1e-4, "fp64": 1e-10}[precision]
+    assert np.array_equal(e[:, 0], np.arange(0, 101, 10))
+    assert np.allclose(e[:, 1], E[::10, 0], rtol=ktol)
+    # the thermostat moved the temperature (initial T ~ 1)
+    T = 2 * E[:, 0] / (3 * len(x))
+    assert abs(T[-1] - 1.5) < abs(T[0] - 1.5)
     sim.close()
 
 
@@ -268,6 +287,95 @@ def test_pair_list_force_api(pkg, O):
     assert _rel(f_lj, Flj) < 1e-4 and abs(e_lj - Ulj) < 1e-4 * abs(Ulj)
 
 
+def test_pair_list_force_api_is_deterministic_and_checks_indices(pkg):
+    x, v, q, L = _cfg(pkg, 4096, 8)
+    xf = x.astype(np.float32)
+    spec = pkg.SpheresBVHSpecs(neighbor_distance=2.5, atom_count=len(x), atomsperleaf=4)
+    pairs = pkg.leafbuild_traverse_bvh(xf, spec, morton=pkg.MORTON_INTERLEAVE)
+    runs = []
+    for _ in range(3):
+        f = np.zeros((len(x), 3), np.float32)
+        e = pkg.force_coulomb_(f, pairs, q, xf, rc=2.5)
+        runs.append((f, e))
+    # integer (fixed-point) sums: the same bits whatever order the lanes add in
+    for f, e in runs[1:]:
+        assert np.array_equal(f.view(np.uint32), runs[0][0].view(np.uint32)) and e == runs[0][1]
+    bad = np.array(pairs, copy=True)
+    bad["j"][17] = len(x)          # one index past the end
+    with pytest.raises(pkg.SpecError):
+        pkg.force_lennardjones_(np.zeros((len(x), 3), np.float32), bad, xf, rc=2.5)
+    bad["j"][17] = -1
+    with pytest.raises(pkg.SpecError):
+        pkg.force_coulomb_(np.zeros((len(x), 3), np.float32), bad, q, xf, rc=2.5)
+    # the context stays usable after the rejected call
+    f = np.zeros((len(x), 3), np.float32)
+    pkg.force_coulomb_(f, pairs, q, xf, rc=2.5)
+    assert np.array_equal(f.view(np.uint32), runs[0][0].view(np.uint32))
+
+
+@pytest.mark.parametrize("resort_every", [0, 50])
+def test_prepared_graphs_give_the_same_trajectory(pkg, resort_every):
+    """nd_sim_prepare captures every window graph up front; the run that
+    follows replays them and must advance exactly as a run that captures
+    lazily (and as an eager run without graphs)."""
+    x, v, q, L = _cfg(pkg, 4096, 12)
+    out = []
+    for mode in ("prepared", "lazy", "eager"):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", resort_every=resort_every, energy_every=10,
+                                           use_graph=mode != "eager"))
+        sim.set_state(x, v, q)
+        sim.run(5)
+        if mode == "prepared":
+            sim.prepare()
+            sim.prepare()          # idempotent
+        sim.run(95)
+        out.append(sim.get_state())
+        assert sim.stats()["rebuilds"] == 11   # steps 0, 10, ..., 100
+        sim.close()
+    for k in (1, 2):
+        for a, b in zip(out[0], out[k]):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision,xtol", [("fp32", 2e-3), ("fp64", 1e-9)])
+@pytest.mark.parametrize("splits", [(60,), (25, 35)])
+def test_fast_atoms_force_rebuilds(pkg, O, precision, xtol, splits):
+    """Two atoms fast enough to move more than skin/2 = 0.15 between scheduled
+    rebuilds (|v| = 30 and 27: ~0.03 per step): the run is redone with the
+    list rebuilt before every force it could no longer serve, and the
+    trajectory follows the oracle, which evaluates every pair every step (as
+    the reference rebuilds its tree every step, Simulator.jl:351-376)."""
+    x, v, q, L = _cfg(pkg, 4096, 13)
+    v = v.copy()
+    v[100] = (30.0, 0.0, 0.0)
+    v[2000] = (0.0, -25.0, 10.0)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision, energy_every=10))
+    sim.set_state(x, v, q)
+    for k in splits:
+        sim.run(k)
+    xg, vg, fg = sim.get_state()
+    st = sim.stats()
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 60,
+                               walls=True, lo=0.0, hi=L)
+    assert np.abs(xg - xr).max() < xtol, np.abs(xg - xr).max()
+    assert st["stale_reruns"] >= 1 and st["forced_rebuilds"] >= 1, st
+    assert st["steps"] == 60
+    e = sim.energies()
+    assert np.array_equal(e[:, 0], np.arange(0, 61, 10))   # one row per energy step
+    sim.close()
+
+
+def test_ordinary_runs_never_rerun(pkg):
+    """At kT = 1 no atom comes near skin/2 within 10 steps: no rerun."""
+    x, v, q, L = _cfg(pkg, 32768, 14)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", energy_every=10))
+    sim.set_state(x, v, q)
+    sim.run(200)
+    st = sim.stats()
+    assert st["stale_reruns"] == 0 and st["forced_rebuilds"] == 0 and st["rebuilds"] == 21
+    sim.close()
+
+
 def test_simulate_poslog_frames_match_the_state(pkg):
     """poslog frames recorded on the device (ring of 2, drained twice) equal
     the positions get_state returns after the same number of steps."""
@@ -285,3 +393,23 @@ def test_simulate_poslog_frames_match_the_state(pkg):
         sim.close()
         assert np.array_equal(poslog[k], xs)
     assert np.array_equal(poslog[-1], sysobj.position)
+
+
+@pytest.mark.parametrize("precision,xtol", [("fp32", 2e-4), ("fp64", 1e-10)])
+def test_simulate_poslog_frames_vs_oracle(pkg, O, precision, xtol):
+    """poslog of simulate! (Simulator.jl:167, 245) every 7 steps over 50 steps
+    (frames at 0, 7, ..., 49 and 50; a ring of 3 drained three times) against
+    the FP64 oracle's trajectory at the same steps."""
+    x, v, q, L = _cfg(pkg, 4096, 10)
+    spec = _spec(pkg, L, precision, steps=50, record_every=7)
+    sysobj = pkg.ObjectCollection(position=x.copy(), velocity=v.copy(), charge=q)
+    poslog = pkg.simulate_(sysobj, spec, ring=3)
+    steps = list(range(0, 50, 7)) + [50]
+    assert len(poslog) == len(steps)
+    p = O.o64_params(rc_lj=2.5, rc_coul=2.5)
+    xr, vr = x.copy(), v.copy()
+    for k, (a, b) in enumerate(zip(steps[:-1], steps[1:])):
+        if k == 0:
+            assert np.array_equal(poslog[0], x)
+        xr, vr, _, _ = O.vv_run64(xr, vr, q, None, p, 0.001, b - a, walls=True, lo=0.0, hi=L)
+        assert np.abs(poslog[k + 1] - xr).max() < xtol, (b, np.abs(poslog[k + 1] - xr).max())
