@@ -14,6 +14,11 @@
 //           atoms (one contiguous 256-byte read per k for a wave), and rows of
 //           different k are adjacent rather than a power of two apart
 //   ncount  int32 [n]
+//   tables  per workgroup of 256 consecutive atoms: the distinct neighbours of
+//           its atoms (sorted indices, at most lcap of them) and their count
+//           (tcount; -1: too many, the workgroup reads nbr directly)
+//   nbr16   the same lists as 16-bit positions in the workgroup's table, two
+//           per int32: one block of [ceil(cap/2)][64] per 64 atoms
 // Rebuild (every rebuild_every steps): key -> radix sort -> permute state ->
 // LBVH (leaf boxes +-r_list/2, nd_lbvh.hip) -> one lane per atom walks the
 // tree from the root and appends every slot j != i with |x_i - x_j|^2 <
@@ -317,11 +322,15 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
        typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
        const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, StepParams P,
        double *__restrict__ partials, SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in,
-       int4 *__restrict__ fix_out, Walls W, uint16_t *__restrict__ path, int32_t first, int32_t check) {
+       int4 *__restrict__ fix_out, Walls W, uint16_t *__restrict__ path, int32_t first, int32_t check,
+       const uint32_t *__restrict__ nbr16, const int32_t *__restrict__ tables,
+       const int32_t *__restrict__ tcount, int32_t lcap) {
     using P4 = typename V4<PosS>::T;
     // MIXED gathers the fixed-point copy (16 B per neighbour)
     constexpr bool FX = sizeof(PosS) == 8 && sizeof(CalcS) == 4;
+    using G = std::conditional_t<FX, int4, P4>;
     __shared__ double sh[nd::kBlock / 64][4];
+    extern __shared__ __align__(16) char s_table[];  // lcap neighbour records (G)
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t i = blk * blockDim.x + threadIdx.x;
     const int32_t n = P.n;
@@ -334,6 +343,34 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     int32_t wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
     const int32_t last = max(__builtin_amdgcn_readfirstlane(wmax) - 1, 0);
+    const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
+        FX ? (void *)fix_in : (void *)pos_in, 0, n * (int32_t)sizeof(G), kBufWord3);
+    // neighbour record j (a sorted index; -1 gives zeros)
+    auto gather = [&](int32_t j) -> G {
+        const uint32_t off = (uint32_t)j * (uint32_t)sizeof(G);
+        G g;
+        if constexpr (sizeof(G) == 16) {
+            const auto a = __builtin_amdgcn_raw_buffer_load_b128(prs, off, 0, 0);
+            __builtin_memcpy(&g, &a, 16);
+        } else {
+            const auto a = __builtin_amdgcn_raw_buffer_load_b128(prs, off, 0, 0);
+            const auto b = __builtin_amdgcn_raw_buffer_load_b128(prs, off + 16u, 0, 0);
+            __builtin_memcpy(&g, &a, 16);
+            __builtin_memcpy(reinterpret_cast<char *>(&g) + 16, &b, 16);
+        }
+        return g;
+    };
+    // this workgroup's neighbourhood table (k_table) staged in LDS; -1: none
+    // (more distinct neighbours than lcap), the lists hold sorted indices
+    const int32_t U = __builtin_amdgcn_readfirstlane(tcount[blk]);
+    G *lt = reinterpret_cast<G *>(s_table);
+    if (U >= 0) {
+        const int32_t *tab = tables + (int64_t)blk * lcap;
+        for (int32_t t = threadIdx.x; t < U; t += blockDim.x) lt[t] = gather(tab[t]);
+        // padding entries read record 0: keep it finite for an empty table
+        if (U == 0 && threadIdx.x == 0) lt[0] = gather(-1);
+        __syncthreads();
+    }
     if (i < n) {
         PairAcc<PosS, CalcS, AccS, ENERGY> acc;
         acc.pi = pos_in[i];
@@ -343,62 +380,36 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         }
         acc.init(P);
         // Software pipeline over batches of kPairBatch neighbours: while
-        // batch b's pairs are computed, the positions of batch b + 1 and the
-        // indices of batch b + 2 are in flight, so one memory latency (not a
-        // list load followed by a gather) separates consecutive batches.
-        // Both go through buffer loads: the row offset is a scalar operand and
-        // the gather offset is j times the record size.  Slots past the
-        // atom's count gather index -1, past the end of the buffer, which the
-        // hardware range check answers with zeros and no memory access; the
-        // pair maths masks them.
-        using G = std::conditional_t<FX, int4, P4>;
+        // batch b's pairs are computed, the records of batch b + 1 and the
+        // list entries of batch b + 2 are in flight, so one memory latency
+        // (not a list load followed by a gather) separates consecutive
+        // batches.  List rows go through buffer loads (the row offset is a
+        // scalar operand).  Direct lists: slots past the atom's count gather
+        // index -1, past the end of the buffer, which the hardware range check
+        // answers with zeros and no memory access; table lists: they read
+        // table record 0.  The pair maths masks them either way.
         const int32_t wave = __builtin_amdgcn_readfirstlane(i >> 6);
-        const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(nbr + (int64_t)wave * P.cap * 64), 0, P.cap * 256, kBufWord3);
         const uint32_t lane4 = (uint32_t)(i & 63) * 4u;
-        auto idx = [&](int32_t r) {
-            const uint32_t row = (uint32_t)(r < last ? r : last);
-            const int32_t v = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(lrs, lane4, row * 256u, 0);
-            return r < cnt ? v : -1;
-        };
-        const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
-            FX ? (void *)fix_in : (void *)pos_in, 0, n * (int32_t)sizeof(G), kBufWord3);
-        auto gather = [&](int32_t j) -> G {
-            const uint32_t off = (uint32_t)j * (uint32_t)sizeof(G);
-            G g;
-            if constexpr (sizeof(G) == 16) {
-                const auto a = __builtin_amdgcn_raw_buffer_load_b128(prs, off, 0, 0);
-                __builtin_memcpy(&g, &a, 16);
-            } else {
-                const auto a = __builtin_amdgcn_raw_buffer_load_b128(prs, off, 0, 0);
-                const auto b = __builtin_amdgcn_raw_buffer_load_b128(prs, off + 16u, 0, 0);
-                __builtin_memcpy(&g, &a, 16);
-                __builtin_memcpy(reinterpret_cast<char *>(&g) + 16, &b, 16);
-            }
-            return g;
-        };
         // Two register sets (a, b) take turns, so no batch is copied.
-        auto pairs = [&](auto one) {
+        auto pipeline = [&](auto one, auto entries, auto record) {
             constexpr bool ONE = decltype(one)::value;
             constexpr int B = kPairBatch;
             int32_t ja[B], jb[B];
             G qa[B], qb[B];
+            entries(0, ja);
 #pragma unroll
-            for (int u = 0; u < B; ++u) qa[u] = gather(idx(u));
-#pragma unroll
-            for (int u = 0; u < B; ++u) jb[u] = idx(B + u);
+            for (int u = 0; u < B; ++u) qa[u] = record(ja[u]);
+            entries(B, jb);
             for (int32_t k = 0;; k += 2 * B) {
 #pragma unroll
-                for (int u = 0; u < B; ++u) qb[u] = gather(jb[u]);
-#pragma unroll
-                for (int u = 0; u < B; ++u) ja[u] = idx(k + 2 * B + u);
+                for (int u = 0; u < B; ++u) qb[u] = record(jb[u]);
+                entries(k + 2 * B, ja);
 #pragma unroll
                 for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], k + u < cnt);
                 if (k + B >= cnt) break;
 #pragma unroll
-                for (int u = 0; u < B; ++u) qa[u] = gather(ja[u]);
-#pragma unroll
-                for (int u = 0; u < B; ++u) jb[u] = idx(k + 3 * B + u);
+                for (int u = 0; u < B; ++u) qa[u] = record(ja[u]);
+                entries(k + 3 * B, jb);
 #pragma unroll
                 for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], k + B + u < cnt);
                 acc.flush();  // MIXED: FP32 sums over 2 batches, then FP64
@@ -406,9 +417,42 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
             }
             acc.flush();  // after an exit at the first batch (a repeat adds zeros)
         };
+        auto run = [&](auto one) {
+            if (U >= 0) {
+                // 16-bit table positions, two per int32 row entry
+                const int32_t cap2 = (P.cap + 1) >> 1;
+                const __amdgpu_buffer_rsrc_t l16 = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(nbr16 + (int64_t)wave * cap2 * 64), 0, cap2 * 256, kBufWord3);
+                const int32_t last2 = last >> 1;
+                auto entries = [&](int32_t r, int32_t *j) {
+                    static_assert(kPairBatch % 2 == 0, "two entries per word");
+#pragma unroll
+                    for (int u = 0; u < kPairBatch; u += 2) {
+                        const int32_t q = (r + u) >> 1;
+                        const uint32_t row = (uint32_t)(q < last2 ? q : last2);
+                        const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(l16, lane4, row * 256u, 0);
+                        j[u] = (int32_t)(v & 0xFFFFu);
+                        j[u + 1] = (int32_t)(v >> 16);
+                    }
+                };
+                pipeline(one, entries, [&](int32_t j) -> G { return lt[j]; });
+            } else {
+                const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(nbr + (int64_t)wave * P.cap * 64), 0, P.cap * 256, kBufWord3);
+                auto entries = [&](int32_t r, int32_t *j) {
+#pragma unroll
+                    for (int u = 0; u < kPairBatch; ++u) {
+                        const uint32_t row = (uint32_t)(r + u < last ? r + u : last);
+                        const int32_t v = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(lrs, lane4, row * 256u, 0);
+                        j[u] = r + u < cnt ? v : -1;
+                    }
+                };
+                pipeline(one, entries, gather);
+            }
+        };
         // equal cutoffs (uniform over the launch): one mask per pair
-        if (acc.rc2 == acc.rcc2) pairs(std::true_type{});
-        else pairs(std::false_type{});
+        if (acc.rc2 == acc.rcc2) run(std::true_type{});
+        else run(std::false_type{});
         fx = acc.fx;
         fy = acc.fy;
         fz = acc.fz;
@@ -877,11 +921,21 @@ using nd::wave_lds_sync;
 constexpr int kFront = 8, kStack2 = 256, kStackDfs = 128, kGroupQ = 128;
 constexpr int32_t kMaxLeaves3 = 1 << 23;
 
-template <int A>
+// records per atom in the walk's record buffer: cap rounded up to 8
+__host__ __device__ __forceinline__ int32_t rec_stride(int32_t cap) { return (cap + 7) & ~7; }
+
+// REC: instead of list rows, each atom gets one record per leaf it has
+// neighbours in, (group g, mask): its neighbours there are the sorted indices
+// 32 g + b for the set bits b of the mask (leaves of A <= 32 atoms start at
+// multiples of A, so a leaf lies in one 32-atom group).  Records of an atom
+// are contiguous (recs[i * rec_stride(cap) + k], k < nrecs[i]; blocks of 64 B
+// multiples, so a reader fetches two records per 16-byte load); each holds
+// at least one neighbour, so there are never more records than list entries.
+template <int A, bool REC = false>
 __global__ void __launch_bounds__(nd::kBlock)
 k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
          float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
-         int32_t *__restrict__ ctr) {
+         int32_t *__restrict__ ctr, uint2 *__restrict__ recs = nullptr, int32_t *__restrict__ nrecs = nullptr) {
     static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
     constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
     constexpr int kSlot = 4 * A + 4;     // floats per group slot (16 B pad: no bank conflicts)
@@ -903,6 +957,8 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(nbr + (int64_t)__builtin_amdgcn_readfirstlane(i >> 6) * cap * 64), 0, cap * 256, kBufWord3);
     uint32_t voff = (uint32_t)lane * 4u;  // byte offset of row cnt in this lane's column
+    int32_t rcnt = 0, nrec = 0;           // REC: list entries and records so far
+    uint2 *myrec = recs + (REC ? (int64_t)i * rec_stride(cap) : 0);
     const float4 p = active ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     // lanes without an atom get an empty query box (it overlaps nothing)
     const float qmn0 = active ? p.x - h : INFINITY, qmn1 = active ? p.y - h : INFINITY,
@@ -968,11 +1024,20 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
                     }
                     const uint32_t self = (uint32_t)(i - base);
                     if (self < (uint32_t)A) hits &= ~(1u << self);
-                    while (hits) {
-                        const int32_t m = __builtin_ctz(hits);
-                        hits &= hits - 1u;
-                        __builtin_amdgcn_raw_buffer_store_b32(base + m, lrs, voff, 0, 0);
-                        voff += 256u;
+                    if constexpr (REC) {
+                        if (hits) {
+                            if (nrec < cap)
+                                myrec[nrec] = make_uint2((uint32_t)(base >> 5), hits << (base & 31));
+                            ++nrec;
+                            rcnt += __popc(hits);
+                        }
+                    } else {
+                        while (hits) {
+                            const int32_t m = __builtin_ctz(hits);
+                            hits &= hits - 1u;
+                            __builtin_amdgcn_raw_buffer_store_b32(base + m, lrs, voff, 0, 0);
+                            voff += 256u;
+                        }
                     }
                 }
             }
@@ -1033,8 +1098,11 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         wave_lds_sync();
     }
     if (bad && lane == 0) atomicOr(&ctr[2], nd::kStatusTree);
-    const int32_t cnt = (int32_t)((voff - (uint32_t)lane * 4u) >> 8);
-    if (active) ncount[i] = cnt;
+    const int32_t cnt = REC ? rcnt : (int32_t)((voff - (uint32_t)lane * 4u) >> 8);
+    if (active) {
+        ncount[i] = cnt;
+        if constexpr (REC) nrecs[i] = nrec;
+    }
     int32_t m = cnt;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off, 64));
     if (lane == 0 && m > 0) {
@@ -1043,6 +1111,321 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         if (m > __hip_atomic_load(&ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&ctr[0], m);
         if (m > cap) atomicOr(&ctr[1], 1);
     }
+}
+
+// ---- neighbourhood tables ----
+// The step kernel stages, per workgroup, the positions of every distinct
+// neighbour of its 256 atoms in LDS once and then reads each pair's
+// neighbour from there: about 930 positions for 256 atoms at density 0.8
+// and r_list = 2.8 (19,400 list entries), so a workgroup gathers each
+// neighbour once instead of once per list entry, and a list entry shrinks to
+// a 16-bit position in the table.  The table is built from a hash over
+// 32-atom groups of sorted indices (key: index >> 5, value: a 32-bit mask of
+// the group's atoms in use), a scan of the masks' population counts, and the
+// position of sorted index j is its group's base plus the number of set mask
+// bits below j & 31.  Table order is hash-slot order; each group's atoms are
+// consecutive in it.
+constexpr int kHashSlots = 1024;
+
+__device__ __forceinline__ uint32_t group_hash(int32_t g) {
+    return ((uint32_t)g * 2654435761u) >> (32 - 10);  // kHashSlots = 2^10
+}
+
+// A workgroup's table under construction: per hash slot the group (-1:
+// empty), the mask of its atoms in use and (after finish()) the table
+// position of the group's first atom in use.
+struct TableLds {
+    int32_t key[kHashSlots];
+    uint32_t mask[kHashSlots];
+    int32_t base[kHashSlots];
+    int32_t part[nd::kBlock / 64];
+    int32_t fail;
+
+    __device__ void init() {
+        for (int32_t t = threadIdx.x; t < kHashSlots; t += nd::kBlock) { key[t] = -1; mask[t] = 0u; }
+        if (threadIdx.x == 0) fail = 0;
+    }
+    // returns the group's slot (-1: the hash is full)
+    __device__ int32_t insert(int32_t g, uint32_t bits) {
+        uint32_t h = group_hash(g);
+        for (int probe = 0; probe < kHashSlots; ++probe) {
+            const int32_t old = atomicCAS(&key[h], -1, g);
+            if (old == -1 || old == g) {
+                atomicOr(&mask[h], bits);
+                return (int32_t)h;
+            }
+            h = (h + 1u) & (kHashSlots - 1);
+        }
+        fail = 1;  // more than kHashSlots groups
+        return -1;
+    }
+    // After every insert (and a barrier): group bases by an exclusive scan of
+    // the masks' population counts; returns the table size, or -1 when the
+    // table does not fit (lcap) or the hash overflowed.  Ends with a barrier.
+    __device__ int32_t finish(int32_t lcap) {
+        constexpr int kPer = kHashSlots / nd::kBlock;
+        const int32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+        int32_t c[kPer], tot = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) { c[k] = __popc(mask[kPer * tid + k]); tot += c[k]; }
+        int32_t incl = tot;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) part[w] = incl;
+        __syncthreads();
+        int32_t wbase = 0, U = 0;
+#pragma unroll
+        for (int k = 0; k < nd::kBlock / 64; ++k) {
+            wbase += k < w ? part[k] : 0;
+            U += part[k];
+        }
+        int32_t b = wbase + incl - tot;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) { base[kPer * tid + k] = b; b += c[k]; }
+        const bool bad = fail != 0 || U > lcap;
+        __syncthreads();
+        return bad ? -1 : U;
+    }
+    // the table: sorted indices of the groups' atoms in use, in slot order
+    __device__ void write(int32_t *tab) const {
+        constexpr int kPer = kHashSlots / nd::kBlock;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int32_t sl = kPer * threadIdx.x + k;
+            uint32_t m = mask[sl];
+            int32_t b = base[sl];
+            const int32_t g32 = key[sl] << 5;
+            while (m) {
+                tab[b++] = g32 + __builtin_ctz(m);
+                m &= m - 1u;
+            }
+        }
+    }
+    __device__ int32_t slot(int32_t g) const {
+        uint32_t h = group_hash(g);
+        while (key[h] != g) h = (h + 1u) & (kHashSlots - 1);  // present: inserted
+        return (int32_t)h;
+    }
+    // table position of sorted index j (its group was inserted)
+    __device__ uint32_t local(int32_t j) const {
+        const int32_t h = slot(j >> 5);
+        return (uint32_t)(base[h] + __popc(mask[h] & ((1u << (j & 31)) - 1u)));
+    }
+};
+
+// Rows are read kRowBatch at a time so that many loads are in flight per wave.
+constexpr int kRowBatch = 16;
+
+// Tables from list rows (the rope, one-node and cell walks).
+__global__ void __launch_bounds__(nd::kBlock)
+k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int32_t n, int32_t cap,
+        int32_t cap2, int32_t lcap, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
+        int32_t *__restrict__ tcount) {
+    __shared__ TableLds T;
+    const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t lane = threadIdx.x & 63;
+    const int32_t i = blk * nd::kBlock + threadIdx.x;
+    T.init();
+    __syncthreads();
+    const int32_t cnt = i < n ? min(ncount[i], cap) : 0;
+    const int32_t *row = nbr + ((int64_t)(i >> 6) * cap) * 64 + lane;
+    int32_t wmax = cnt;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    wmax = __builtin_amdgcn_readfirstlane(wmax);
+    // consecutive entries of a list mostly share a group (the walks emit a
+    // leaf's atoms together): one hash update per run of a group
+    {
+        int32_t cg = -1;
+        uint32_t bits = 0u;
+        for (int32_t r0 = 0; r0 < wmax; r0 += kRowBatch) {
+            int32_t jj[kRowBatch];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) jj[u] = r0 + u < cnt ? row[(int64_t)(r0 + u) * 64] : -1;
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) {
+                const int32_t j = jj[u];
+                if (j < 0) continue;
+                const int32_t g = j >> 5;
+                if (g != cg) {
+                    if (cg >= 0) T.insert(cg, bits);
+                    cg = g;
+                    bits = 0u;
+                }
+                bits |= 1u << (j & 31);
+            }
+        }
+        if (cg >= 0) T.insert(cg, bits);
+    }
+    __syncthreads();
+    const int32_t U = T.finish(lcap);
+    if (threadIdx.x == 0) tcount[blk] = U;
+    if (U < 0) return;  // uniform: the step kernel reads the rows directly
+    T.write(tables + (int64_t)blk * lcap);
+    // waves past the last atom own no list block
+    if (((i >> 6) << 6) >= n) return;
+    // 16-bit lists, two entries per int32, one row of 64 per step of the
+    // wave; row 0 is written even for a wave without neighbours (the step
+    // kernel's look-ahead reads it)
+    uint32_t *out = nbr16 + ((int64_t)(i >> 6) * cap2) * 64 + lane;
+    const int32_t nrows = max((wmax + 1) >> 1, 1);
+    for (int32_t q0 = 0; q0 < nrows; q0 += kRowBatch / 2) {
+        int32_t jj[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            const int32_t r = 2 * q0 + u;
+            jj[u] = r < cnt ? row[(int64_t)r * 64] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u += 2) {
+            const int32_t q = q0 + u / 2;
+            if (q >= nrows) break;
+            const uint32_t a = jj[u] >= 0 ? T.local(jj[u]) : 0u;
+            const uint32_t b = jj[u + 1] >= 0 ? T.local(jj[u + 1]) : 0u;
+            out[(int64_t)q * 64] = a | (b << 16);
+        }
+    }
+}
+
+// Tables from the two-phase walk's records (k_nlist3<A, true>): a record per
+// (atom, leaf with neighbours), ~9 per atom at density 0.8, instead of a list
+// entry per neighbour (~76), so the walk writes and this kernel reads ~4x
+// fewer bytes.  While the records go into the hash they are copied to LDS;
+// once the table is laid out each LDS record gets its group's table base and
+// mask in use, and every lane expands its own records into its list with one
+// LDS read per record and a population count per entry (a lane reading its
+// records from memory instead would stall its whole wave at each one).
+// A workgroup whose table does not fit (lcap) gets list rows of sorted
+// indices (nbr) instead, as does one whose records do not fit in LDS.
+constexpr int kRecLds = 3072;  // records per workgroup held in LDS
+
+__global__ void __launch_bounds__(nd::kBlock)
+k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
+            const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
+            uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables, int32_t *__restrict__ tcount,
+            int32_t *__restrict__ nbr) {
+    constexpr int kW = nd::kBlock / 64;
+    __shared__ TableLds T;
+    __shared__ uint2 s_rec[kRecLds];     // (hash slot, then group base; mask)
+    __shared__ uint32_t s_gmask[kRecLds];  // the record's group mask in use
+    __shared__ int32_t s_wsum[kW];
+    const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int32_t i = blk * nd::kBlock + tid;
+    T.init();
+    const bool active = i < n;
+    const int32_t cnt = active ? min(ncount[i], cap) : 0;
+    const int32_t nr = active ? min(nrecs[i], cap) : 0;
+    const uint2 *my = recs + (active ? (int64_t)i * rec_stride(cap) : 0);
+    // this lane's first LDS record: exclusive scan of the record counts
+    int32_t incl = nr;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wsum[w] = incl;
+    __syncthreads();
+    int32_t roff = incl - nr, rtot = 0;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        roff += k < w ? s_wsum[k] : 0;
+        rtot += s_wsum[k];
+    }
+    const bool in_lds = rtot <= kRecLds;  // uniform over the workgroup
+    // records are fetched kRecBatch at a time, two per 16-byte load (the
+    // atom's block is 64-byte aligned; slots past nr are read and ignored)
+    constexpr int kRecBatch = 8;
+    auto fetch = [&](int32_t k0, uint2 *r) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(my + k0);
+#pragma unroll
+        for (int u = 0; u < kRecBatch / 2; ++u) {
+            const uint4 v = k0 + 2 * u < nr ? src[u] : make_uint4(0u, 0u, 0u, 0u);
+            r[2 * u] = make_uint2(v.x, v.y);
+            r[2 * u + 1] = make_uint2(v.z, v.w);
+        }
+    };
+    for (int32_t k0 = 0; k0 < nr; k0 += kRecBatch) {
+        uint2 r[kRecBatch];
+        fetch(k0, r);
+#pragma unroll
+        for (int u = 0; u < kRecBatch; ++u)
+            if (k0 + u < nr) {
+                const int32_t h = T.insert((int32_t)r[u].x, r[u].y);
+                if (in_lds) s_rec[roff + k0 + u] = make_uint2((uint32_t)h, r[u].y);
+            }
+    }
+    __syncthreads();
+    // LDS records carry hash slots: usable only if no insert found the hash full
+    const bool lds_ok = in_lds && T.fail == 0;
+    const int32_t U0 = T.finish(lcap);  // ends with a barrier
+    const int32_t U = lds_ok ? U0 : -1;
+    if (tid == 0) tcount[blk] = U;
+    if (U >= 0) {
+        T.write(tables + (int64_t)blk * lcap);
+        for (int32_t t = tid; t < rtot; t += nd::kBlock) {
+            const uint32_t h = s_rec[t].x;
+            s_rec[t].x = (uint32_t)T.base[h];
+            s_gmask[t] = T.mask[h];
+        }
+        __syncthreads();
+    }
+    if (((i >> 6) << 6) >= n) return;  // waves past the last atom own no list block
+    int32_t wmax = cnt;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    wmax = __builtin_amdgcn_readfirstlane(wmax);
+    if (U >= 0) {
+        // the lane's entries in record order (every record has a set bit)
+        int32_t rk = roff, cb = 0;
+        uint32_t bits = 0u, gm = 0u;
+        auto pop = [&]() -> uint32_t {  // table position; 0 past the end
+            if (bits == 0u) {
+                if (rk >= roff + nr) return 0u;
+                const uint2 r = s_rec[rk];
+                gm = s_gmask[rk];
+                ++rk;
+                cb = (int32_t)r.x;
+                bits = r.y;
+            }
+            const int32_t b = __builtin_ctz(bits);
+            bits &= bits - 1u;
+            return (uint32_t)(cb + __popc(gm & ((1u << b) - 1u)));
+        };
+        // 16-bit lists, two entries per int32, one row of 64 per step of the
+        // wave; row 0 is written even for a wave without neighbours (the
+        // step kernel's look-ahead reads it)
+        uint32_t *out = nbr16 + ((int64_t)(i >> 6) * cap2) * 64 + lane;
+        const int32_t nrows = max((wmax + 1) >> 1, 1);
+        for (int32_t r = 0; r < nrows; ++r) {
+            const uint32_t a = pop(), b = pop();
+            out[(int64_t)r * 64] = a | (b << 16);
+        }
+        return;
+    }
+    // sorted indices from the records in memory
+    uint2 q[kRecBatch];
+    int32_t qn = 0, rk = 0, gbase = 0;
+    uint32_t bits = 0u;
+    auto pop = [&]() -> int32_t {  // -1 when done
+        while (bits == 0u) {
+            if (qn == 0) {
+                if (rk >= nr) return -1;
+                fetch(rk, q);
+                qn = min(kRecBatch, nr - rk);
+                rk += qn;
+            }
+            gbase = (int32_t)q[0].x;
+            bits = q[0].y;
+#pragma unroll
+            for (int u = 0; u + 1 < kRecBatch; ++u) q[u] = q[u + 1];
+            --qn;
+        }
+        const int32_t b = __builtin_ctz(bits);
+        bits &= bits - 1u;
+        return (gbase << 5) + b;
+    };
+    int32_t *out = nbr + ((int64_t)(i >> 6) * cap) * 64 + lane;
+    for (int32_t r = 0; r < wmax; ++r) out[(int64_t)r * 64] = pop();
 }
 
 // Cell-list walk (list_walk 3), the comparison point for the LBVH walks: the
@@ -1158,6 +1541,9 @@ struct nd_sim {
         ncount, misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig, rep, boxes;
     nd::DevBuf cpos, cstart;  // cell-list walk: cell-sorted positions, cell starts
     nd::DevBuf path;          // uint16 path length of each atom since the last build
+    nd::DevBuf nbr16, tables, tcount;  // neighbourhood tables and 16-bit lists (k_table)
+    nd::DevBuf recs, nrecs;   // the two-phase walk's (group, mask) records per atom
+    int32_t lcap = 0;         // table capacity (records) per workgroup
     nd::CellGrid cg{};
     int32_t nrep = 0;  // packed replicas (0: one system)
     nd::SortBufs sort;
@@ -1212,7 +1598,10 @@ void alloc_lists(nd_sim *s, int32_t cap) {
     s->cap = cap;
     s->sp.cap = cap;
     // one block of cap rows x 64 slots per 64 atoms (n rounded up to 64)
-    s->nbr.reserve(sizeof(int32_t) * (size_t)cap * (((size_t)s->n + 63) & ~(size_t)63));
+    const size_t n64 = ((size_t)s->n + 63) & ~(size_t)63;
+    s->nbr.reserve(sizeof(int32_t) * (size_t)cap * n64);
+    s->nbr16.reserve(sizeof(uint32_t) * (size_t)((cap + 1) / 2) * n64);
+    s->recs.reserve(sizeof(uint2) * (size_t)rec_stride(cap) * n64);
     drop_graphs(s);
 }
 
@@ -1236,6 +1625,14 @@ int32_t resort_period(const nd_sim *s) {
 bool resort_due(const nd_sim *s, int64_t k) { return !s->tree_valid || k % resort_period(s) == 0; }
 
 void launch_sort_and_build(nd_sim *s);
+
+// Neighbourhood tables and 16-bit lists from the walk's lists.
+void launch_tables(nd_sim *s) {
+    k_table<<<nd::grid_for(s->n), nd::kBlock, 0, s->ctx->stream>>>(
+        s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->n, s->cap, (s->cap + 1) / 2, s->lcap,
+        s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(), s->tcount.as<int32_t>());
+    ND_HIP(hipGetLastError());
+}
 
 // Neighbour-list rebuild: a full re-sort + tree build, or a box refit of the
 // current tree, followed by the list walk.
@@ -1263,6 +1660,7 @@ void launch_rebuild(nd_sim *s, bool full) {
                                               s->nbr.as<int32_t>(), s->ncount.as<int32_t>(),
                                               reinterpret_cast<int32_t *>(misc));
         ND_HIP(hipGetLastError());
+        launch_tables(s);
         return;
     }
     if (full) {
@@ -1295,15 +1693,23 @@ void launch_rebuild(nd_sim *s, bool full) {
                                (s->apl == 4 || s->apl == 8 || s->apl == 16 || s->apl == 32);
         if (two_phase) nd::launch_lbvh_wide4(st, s->nodes.as<nd_node>(), s->nL, kids);
         else nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
-        if (two_phase && s->apl == 4)
-            k_nlist3<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
-        else if (two_phase && s->apl == 8)
-            k_nlist3<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
-        else if (two_phase && s->apl == 16)
-            k_nlist3<16><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
-        else if (two_phase && s->apl == 32)
-            k_nlist3<32><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr);
-        else switch (s->apl) {
+        uint2 *rc = s->recs.as<uint2>();
+        int32_t *nr = s->nrecs.as<int32_t>();
+        if (two_phase) {
+            switch (s->apl) {
+                case 4: k_nlist3<4, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
+                case 8: k_nlist3<8, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
+                case 16: k_nlist3<16, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
+                default: k_nlist3<32, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
+            }
+            ND_HIP(hipGetLastError());
+            k_table_rec<<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, (s->cap + 1) / 2, s->lcap,
+                                                  s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
+                                                  s->tcount.as<int32_t>(), nb);
+            ND_HIP(hipGetLastError());
+            return;
+        }
+        switch (s->apl) {
             case 4: k_nlist2<4><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 4, hl, rl2, s->cap, nb, nc, misc); break;
             case 8: k_nlist2<8><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 8, hl, rl2, s->cap, nb, nc, misc); break;
             case 16: k_nlist2<16><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, 16, hl, rl2, s->cap, nb, nc, misc); break;
@@ -1312,6 +1718,7 @@ void launch_rebuild(nd_sim *s, bool full) {
         }
     }
     ND_HIP(hipGetLastError());
+    launch_tables(s);
 }
 
 // New keys, stable sort, state permutation, tree build.
@@ -1389,11 +1796,23 @@ void launch_step_t(nd_sim *s, bool check) {
     hipStream_t st = s->ctx->stream;
     const unsigned g = nd::grid_for(s->n);
     const int a = s->cur ^ 1;
-    k_step<PosS, CalcS, AccS, MODE, E><<<g, nd::kBlock, 0, st>>>(
+    using G = std::conditional_t<sizeof(PosS) == 8 && sizeof(CalcS) == 4, int4, P4>;
+    const size_t lds = sizeof(G) * (size_t)s->lcap;
+    if (lds > 65536) {
+        // dynamic LDS past 64 KiB must be allowed per kernel (once)
+        static bool allowed = false;
+        if (!allowed) {
+            ND_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step<PosS, CalcS, AccS, MODE, E>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            allowed = true;
+        }
+    }
+    k_step<PosS, CalcS, AccS, MODE, E><<<g, nd::kBlock, lds, st>>>(
         s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
         s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s), s->path.as<uint16_t>(),
-        s->since == 0 ? 1 : 0, check ? 1 : 0);
+        s->since == 0 ? 1 : 0, check ? 1 : 0, s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
+        s->tcount.as<int32_t>(), s->lcap);
     if (MODE == kModeFused) s->since++;
 }
 
@@ -1784,6 +2203,21 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->kids.reserve(8 * sizeof(float4) * (size_t)(s->nL - 1));  // four-wide records
         s->ncount.reserve(sizeof(int32_t) * n);
         s->path.reserve(sizeof(uint16_t) * n);
+        s->nrecs.reserve(sizeof(int32_t) * n);
+        // neighbourhood tables: up to 2,560 records per workgroup in the step
+        // kernel's LDS, 40 KiB at 16 B (FP32, MIXED: four workgroups per CU)
+        // or 80 KiB at 32 B (FP64: two), within what the device allows one
+        // workgroup
+        {
+            int lds_max = 65536;
+            ND_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+            const int rec_bytes = p->precision == ND_PREC_FP64 ? 32 : 16;
+            const int fit = ((lds_max - 1024) / rec_bytes) & ~31;
+            s->lcap = fit < 2560 ? fit : 2560;
+            if (s->lcap < 32) s->lcap = 32;
+        }
+        s->tables.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n) * (size_t)s->lcap);
+        s->tcount.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n));
         s->misc.reserve(sizeof(SimMisc));
         s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
         s->rows.reserve(sizeof(double) * 5 * (size_t)s->max_rows);
@@ -1831,7 +2265,8 @@ int nd_sim_destroy(nd_sim *s) {
                           &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids, &s->rep, &s->boxes,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
                           &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
-                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos, &s->cstart, &s->path};
+                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos, &s->cstart, &s->path,
+                          &s->nbr16, &s->tables, &s->tcount, &s->recs, &s->nrecs};
     for (auto *b : bufs) b->release();
     delete s;
     return ND_OK;
