@@ -167,7 +167,10 @@ def precision_form(pkg, steps=500):
         res_k = float(np.mean([sim.time_fused_window(W) for _ in range(5)]))
         E0 = e0[0, 1] + e0[0, 2]
         E1 = e[-1, 1] + e[-1, 2]
+        st = sim.stats()
         res[prec] = dict(f0=f0, x=x, ms=ms, drift=(E1 - E0) / abs(E0), kms=res_k)
+        out[prec + "_list"] = {k: int(st[k]) for k in ("rebuilds", "stale_reruns", "forced_rebuilds",
+                                                     "max_neighbors", "capacity")}
         out[prec + "_steps_per_s"] = round(steps / (ms / 1e3), 2)
         out[prec + "_force_kernel_ms"] = round(res_k, 5)
         out[prec + "_energy_drift"] = float("%.3e" % res[prec]["drift"])

@@ -958,6 +958,7 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         (void *)(nbr + (int64_t)__builtin_amdgcn_readfirstlane(i >> 6) * cap * 64), 0, cap * 256, kBufWord3);
     uint32_t voff = (uint32_t)lane * 4u;  // byte offset of row cnt in this lane's column
     int32_t rcnt = 0, nrec = 0;           // REC: list entries and records so far
+    uint32_t pend_g = ~0u, pend_b = 0u;    // REC: the record being built (leaves of one group merge)
     uint2 *myrec = recs + (REC ? (int64_t)i * rec_stride(cap) : 0);
     const float4 p = active ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     // lanes without an atom get an empty query box (it overlaps nothing)
@@ -1026,9 +1027,17 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
                     if (self < (uint32_t)A) hits &= ~(1u << self);
                     if constexpr (REC) {
                         if (hits) {
-                            if (nrec < cap)
-                                myrec[nrec] = make_uint2((uint32_t)(base >> 5), hits << (base & 31));
-                            ++nrec;
+                            const uint32_t g = (uint32_t)(base >> 5), b = hits << (base & 31);
+                            if (g == pend_g) {
+                                pend_b |= b;
+                            } else {
+                                if (pend_g != ~0u) {
+                                    if (nrec < cap) myrec[nrec] = make_uint2(pend_g, pend_b);
+                                    ++nrec;
+                                }
+                                pend_g = g;
+                                pend_b = b;
+                            }
                             rcnt += __popc(hits);
                         }
                     } else {
@@ -1098,6 +1107,10 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         wave_lds_sync();
     }
     if (bad && lane == 0) atomicOr(&ctr[2], nd::kStatusTree);
+    if (REC && pend_g != ~0u) {
+        if (nrec < cap) myrec[nrec] = make_uint2(pend_g, pend_b);
+        ++nrec;
+    }
     const int32_t cnt = REC ? rcnt : (int32_t)((voff - (uint32_t)lane * 4u) >> 8);
     if (active) {
         ncount[i] = cnt;
@@ -1356,12 +1369,11 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             }
     }
     __syncthreads();
-    // LDS records carry hash slots: usable only if no insert found the hash full
-    const bool lds_ok = in_lds && T.fail == 0;
-    const int32_t U0 = T.finish(lcap);  // ends with a barrier
-    const int32_t U = lds_ok ? U0 : -1;
+    // U >= 0 only if every group found a hash slot and the table fits
+    const int32_t U = T.finish(lcap);  // ends with a barrier
+    const bool lds_ok = in_lds && U >= 0;
     if (tid == 0) tcount[blk] = U;
-    if (U >= 0) {
+    if (lds_ok) {
         T.write(tables + (int64_t)blk * lcap);
         for (int32_t t = tid; t < rtot; t += nd::kBlock) {
             const uint32_t h = s_rec[t].x;
@@ -1370,11 +1382,12 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
         }
         __syncthreads();
     }
+    if (U >= 0 && !in_lds) T.write(tables + (int64_t)blk * lcap);
     if (((i >> 6) << 6) >= n) return;  // waves past the last atom own no list block
     int32_t wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
     wmax = __builtin_amdgcn_readfirstlane(wmax);
-    if (U >= 0) {
+    if (lds_ok) {
         // the lane's entries in record order (every record has a set bit)
         int32_t rk = roff, cb = 0;
         uint32_t bits = 0u, gm = 0u;
@@ -1402,30 +1415,57 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
         }
         return;
     }
-    // sorted indices from the records in memory
-    uint2 q[kRecBatch];
-    int32_t qn = 0, rk = 0, gbase = 0;
-    uint32_t bits = 0u;
-    auto pop = [&]() -> int32_t {  // -1 when done
+    // Records in memory (too many for LDS): each lane expands its own, the
+    // next batch of kRecBatch records in flight while the current one is
+    // used (a lane waiting on a load would stall its whole wave), into table
+    // positions (a hash lookup per record) or, without a table, sorted indices.
+    const bool table = U >= 0;
+    uint2 q[kRecBatch], nq[kRecBatch];
+    int32_t qn = 0, nqn = 0, rk = 0, cb = 0, gbase = 0;
+    uint32_t bits = 0u, gm = 0u;
+    auto prefetch = [&]() {
+        nqn = min(kRecBatch, nr - rk);
+        if (nqn > 0) {
+            fetch(rk, nq);
+            rk += nqn;
+        }
+    };
+    prefetch();
+    auto pop = [&]() -> int32_t {  // table position / sorted index; -1 when done
         while (bits == 0u) {
             if (qn == 0) {
-                if (rk >= nr) return -1;
-                fetch(rk, q);
-                qn = min(kRecBatch, nr - rk);
-                rk += qn;
+                if (nqn <= 0) return -1;
+#pragma unroll
+                for (int u = 0; u < kRecBatch; ++u) q[u] = nq[u];
+                qn = nqn;
+                prefetch();
             }
             gbase = (int32_t)q[0].x;
             bits = q[0].y;
 #pragma unroll
             for (int u = 0; u + 1 < kRecBatch; ++u) q[u] = q[u + 1];
             --qn;
+            if (table) {
+                const int32_t h = T.slot(gbase);
+                cb = T.base[h];
+                gm = T.mask[h];
+            }
         }
         const int32_t b = __builtin_ctz(bits);
         bits &= bits - 1u;
-        return (gbase << 5) + b;
+        return table ? cb + __popc(gm & ((1u << b) - 1u)) : (gbase << 5) + b;
     };
-    int32_t *out = nbr + ((int64_t)(i >> 6) * cap) * 64 + lane;
-    for (int32_t r = 0; r < wmax; ++r) out[(int64_t)r * 64] = pop();
+    if (table) {
+        uint32_t *out = nbr16 + ((int64_t)(i >> 6) * cap2) * 64 + lane;
+        const int32_t nrows = max((wmax + 1) >> 1, 1);
+        for (int32_t r = 0; r < nrows; ++r) {
+            const int32_t a = pop(), b = pop();
+            out[(int64_t)r * 64] = (uint32_t)max(a, 0) | ((uint32_t)max(b, 0) << 16);
+        }
+    } else {
+        int32_t *out = nbr + ((int64_t)(i >> 6) * cap) * 64 + lane;
+        for (int32_t r = 0; r < wmax; ++r) out[(int64_t)r * 64] = pop();
+    }
 }
 
 // Cell-list walk (list_walk 3), the comparison point for the LBVH walks: the
@@ -1504,6 +1544,12 @@ k_pos32(const double4 *__restrict__ pos, int32_t n, float4 *__restrict__ out) {
     out[s] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
 }
 
+// Copies a raised stale flag to host memory (the first kernel of every
+// rebuild, so a run on the host sees a stale list a window or two later).
+__global__ void k_stale_to_host(const SimMisc *__restrict__ misc, volatile int32_t *hflag) {
+    if (threadIdx.x == 0 && misc->stale) *hflag = 1;
+}
+
 __global__ void k_misc_reset_counters(SimMisc *misc) {
     if (threadIdx.x == 0) { misc->overflow = 0; misc->status = 0; misc->maxcnt = 0; }
 }
@@ -1539,6 +1585,7 @@ struct nd_sim {
     bool wide = false;  // FP64 storage (MIXED, FP64)
     nd::DevBuf pos[2], vel[2], frc, orig[2], pos32, fix[2], codes, perm, nodes, store, kids, nbr,
         ncount, misc, partials, rows, snap_pos, snap_vel, snap_frc, snap_orig, rep, boxes;
+    nd::DevBuf wsnap_pos, wsnap_vel, wsnap_orig;  // start of a window in a checked rerun
     nd::DevBuf cpos, cstart;  // cell-list walk: cell-sorted positions, cell starts
     nd::DevBuf path;          // uint16 path length of each atom since the last build
     nd::DevBuf nbr16, tables, tcount;  // neighbourhood tables and 16-bit lists (k_table)
@@ -1554,6 +1601,11 @@ struct nd_sim {
     int64_t step = 0, rebuilds = 0, last_build_step = -1;
     int32_t since = 0;  // drifts launched since the last list build
     int64_t stale_reruns = 0, forced_rebuilds = 0;
+    // stale flag in pinned host memory (k_stale_to_host) and one event per
+    // window in flight, so a run can stop early when a list went stale
+    volatile int32_t *hstale = nullptr;
+    static constexpr int kInFlight = 3;
+    hipEvent_t wev[kInFlight + 1] = {};
     bool have_force = false, list_valid = false, have_state = false;
     // the tree in nodes/store matches the current atom order (a refit may reuse it)
     bool tree_valid = false;
@@ -1639,6 +1691,7 @@ void launch_tables(nd_sim *s) {
 void launch_rebuild(nd_sim *s, bool full) {
     s->since = 0;  // the next drift starts every path length at 0
     hipStream_t st = s->ctx->stream;
+    k_stale_to_host<<<1, 64, 0, st>>>(s->misc.as<SimMisc>(), const_cast<int32_t *>(s->hstale));
     const int32_t n = s->n;
     const unsigned g = nd::grid_for(n);
     SimMisc *misc = s->misc.as<SimMisc>();
@@ -2036,10 +2089,61 @@ bool take_stale(nd_sim *s) {
     return true;
 }
 
-// checked: eager launches with the stale flag read after every drift; a
-// flagged drift makes the next step rebuild the list first (a forced
-// rebuild), so no force is evaluated with a list that may miss a pair.
-void run_once(nd_sim *s, int32_t nsteps, bool checked) {
+// State at the start of a window, for a checked rerun: the atoms (positions,
+// velocities, original indices in the current buffers) and the host-side
+// counters; the stored force is not needed (a window's fused steps compute
+// theirs), and the tree and list are rebuilt by the redone window.
+struct WindowStart {
+    int cur, vcur, since;
+    int64_t rebuilds, last_build_step;
+    SimMisc misc;
+};
+
+WindowStart window_snapshot(nd_sim *s) {
+    hipStream_t st = s->ctx->stream;
+    const size_t pb = pos_bytes(s);
+    s->wsnap_pos.reserve(pb);
+    s->wsnap_vel.reserve(pb);
+    s->wsnap_orig.reserve(sizeof(int32_t) * (size_t)s->n);
+    ND_HIP(hipMemcpyAsync(s->wsnap_pos.p, s->pos[s->cur].p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->wsnap_vel.p, s->vel[s->vcur].p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->wsnap_orig.p, s->orig[s->vcur].p, sizeof(int32_t) * (size_t)s->n,
+                          hipMemcpyDeviceToDevice, st));
+    WindowStart w{s->cur, s->vcur, s->since, s->rebuilds, s->last_build_step, SimMisc{}};
+    read_misc(s, w.misc);  // synchronises
+    return w;
+}
+
+// Back to the window's start; the next rebuild re-sorts (it also rewrites
+// the fixed-point and FP32 copies of the positions from the restored ones).
+void window_restore(nd_sim *s, const WindowStart &w) {
+    hipStream_t st = s->ctx->stream;
+    s->cur = w.cur;
+    s->vcur = w.vcur;
+    s->since = w.since;
+    s->rebuilds = w.rebuilds;
+    s->last_build_step = w.last_build_step;
+    s->tree_valid = false;
+    const size_t pb = pos_bytes(s);
+    ND_HIP(hipMemcpyAsync(s->pos[s->cur].p, s->wsnap_pos.p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->vel[s->vcur].p, s->wsnap_vel.p, pb, hipMemcpyDeviceToDevice, st));
+    ND_HIP(hipMemcpyAsync(s->orig[s->vcur].p, s->wsnap_orig.p, sizeof(int32_t) * (size_t)s->n,
+                          hipMemcpyDeviceToDevice, st));
+    SimMisc m = w.misc;
+    m.stale = 0;
+    ND_HIP(hipMemcpyAsync(s->misc.p, &m, sizeof(SimMisc), hipMemcpyHostToDevice, st));
+    ND_HIP(hipStreamSynchronize(st));
+}
+
+// Steps before checked_until are checked (a rerun after a stale list): each
+// whole rebuild window is still replayed as a graph, then its stale flag is
+// read; a window that went stale is restored and redone with eager launches,
+// reading the flag after every drift, and a flagged drift makes the next step
+// rebuild the list first (a forced rebuild), so no force is evaluated with a
+// list that may miss a pair.  Later windows run unchecked, at most kInFlight
+// ahead of the device; when the host sees the stale flag of a finished
+// window it stops launching and returns false (the caller reruns).
+bool run_once(nd_sim *s, int32_t nsteps, int64_t checked_until) {
     const int32_t W = s->p.rebuild_every;
     int64_t k = s->step;
     const bool had_force = s->have_force;
@@ -2058,21 +2162,41 @@ void run_once(nd_sim *s, int32_t nsteps, bool checked) {
         s->p.energy_every = ee;
         s->have_force = true;
     }
-    if (nsteps == 0) return;
+    if (nsteps == 0) return true;
     // step k -> k+1 starts with a kick-drift from the stored force (after the
     // velocity rescale if step k is a thermostat step; the previous run's
     // final step recorded KE(v_k) for it)
     launch_kick_drift(s, thermo_step(s, k), k);
-    bool forced = checked && take_stale(s);
+    bool forced = k < checked_until && take_stale(s);
     const int64_t kend = k + nsteps;  // integer step reached at the end
     int64_t j = k + 1;                // next force evaluation index
-    const bool use_graph = !checked && graph_eligible(s);
+    const bool use_graph = graph_eligible(s);
+    int64_t windows = 0;  // unchecked windows launched
     while (j < kend) {
+        const bool checked = j < checked_until;
         if (j % W == 0 || forced) {
-            if (use_graph && j % W == 0 && j + W <= kend) {
+            if (use_graph && j % W == 0 && j + W <= kend && !forced) {
+                if (!checked) {
+                    if (windows >= s->kInFlight) {
+                        ND_HIP(hipEventSynchronize(s->wev[(windows - s->kInFlight) % (s->kInFlight + 1)]));
+                        if (*s->hstale) {
+                            s->step = j;  // where the host stopped (the caller restores)
+                            return false;
+                        }
+                    }
+                    run_window_graph(s, j);
+                    ND_HIP(hipEventRecord(s->wev[windows % (s->kInFlight + 1)], s->ctx->stream));
+                    ++windows;
+                    j += W;
+                    continue;
+                }
+                const WindowStart ws = window_snapshot(s);
                 run_window_graph(s, j);
-                j += W;
-                continue;
+                if (!take_stale(s)) {
+                    j += W;
+                    continue;
+                }
+                window_restore(s, ws);  // redo this window step by step
             }
             launch_rebuild(s, resort_due(s, j));
             s->rebuilds++;
@@ -2100,6 +2224,7 @@ void run_once(nd_sim *s, int32_t nsteps, bool checked) {
         launch_force_step(s, kModeFinal, kend);
     }
     s->step = kend;
+    return true;
 }
 
 void set_step_params(nd_sim *s) {
@@ -2233,6 +2358,13 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
             s->cstart.reserve(sizeof(int32_t) * ((size_t)s->cg.ncell + 1));
         }
         ND_HIP(hipMemsetAsync(s->misc.p, 0, sizeof(SimMisc), ctx->stream));
+        {
+            void *h = nullptr;
+            ND_HIP(hipHostMalloc(&h, 64, hipHostMallocDefault));
+            s->hstale = static_cast<volatile int32_t *>(h);
+            *s->hstale = 0;
+            for (auto &e : s->wev) ND_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
         int32_t cap = p->max_neighbors;
         if (cap <= 0) {
             const double vol = p->hi > p->lo ? (double)(p->hi - p->lo) * (p->hi - p->lo) * (p->hi - p->lo) : 0.0;
@@ -2261,12 +2393,16 @@ int nd_sim_destroy(nd_sim *s) {
     }
     drop_graphs(s);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+    for (auto e : s->wev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->hstale) (void)hipHostFree(const_cast<int32_t *>(s->hstale));
     nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
                           &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids, &s->rep, &s->boxes,
                           &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
                           &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
                           &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos, &s->cstart, &s->path,
-                          &s->nbr16, &s->tables, &s->tcount, &s->recs, &s->nrecs};
+                          &s->nbr16, &s->tables, &s->tcount, &s->recs, &s->nrecs,
+                          &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig};
     for (auto *b : bufs) b->release();
     delete s;
     return ND_OK;
@@ -2365,7 +2501,7 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
     ND_REQUIRE(s->have_state, "nd_sim_run: call nd_sim_set_state first");
     ND_HIP(hipSetDevice(ctx->device));
     s->ev_used = 0;
-    bool checked = false;
+    int64_t checked_until = -1;  // steps before this are run checked
     for (int attempt = 0; attempt < 8; ++attempt) {
         // snapshot for a rerun if the neighbour list overflows or goes stale
         SimMisc m0;
@@ -2375,7 +2511,9 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
                       fr0 = s->forced_rebuilds;
         const bool hf0 = s->have_force;
         snapshot(s);
-        run_once(s, nsteps, checked);
+        *s->hstale = 0;
+        const bool done = run_once(s, nsteps, checked_until);
+        const int64_t stop = s->step;
         SimMisc m;
         read_misc(s, m);
         if (m.status & nd::kStatusDomain)
@@ -2383,7 +2521,7 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
         if (m.status & nd::kStatusTree)
             throw nd::Error(ND_ERR_STATE, "simulation: malformed neighbour tree");
         s->last_maxcnt = m.maxcnt;
-        if (!m.overflow && !m.stale) {
+        if (done && !m.overflow && !m.stale) {
             if (s->timing && s->ev_used >= 2) {
                 // every force launch recorded an event pair
                 for (size_t e = 0; e + 1 < s->ev_used; e += 2) {
@@ -2409,10 +2547,14 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
             s->tree_valid = false;
         } else {
             // later rebuilds of the failed attempt re-sorted the atoms: the
-            // rerun builds a list for the restored state first
+            // rerun builds a list for the restored state first, and checks
+            // every window (a system fast enough to outrun the skin once
+            // usually does again soon, and an unchecked part would send the
+            // run back to its start each time)
             s->list_valid = false;
             s->tree_valid = false;
-            checked = true;
+            checked_until = step0 + nsteps + 1;
+            (void)stop;
             s->stale_reruns++;
         }
         m0.overflow = 0;
