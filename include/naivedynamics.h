@@ -80,6 +80,10 @@ int nd_memcpy_d2h(nd_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 int nd_memcpy_d2d(nd_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 int nd_memset(nd_ctx *ctx, void *d_dst, int32_t byte_value, size_t bytes);
 int nd_synchronize(nd_ctx *ctx);
+/* The context's HIP stream (hipStream_t) in *stream: every call on the
+ * context is ordered on it, so a caller can order its own HIP work or
+ * capture a graph around stream-ordered calls. */
+int nd_ctx_stream(nd_ctx *ctx, void **stream);
 /* HIP events on the context's stream: nd_timer_start records, nd_timer_stop
  * records, synchronises and returns the elapsed milliseconds. */
 int nd_timer_start(nd_ctx *ctx);
@@ -145,8 +149,10 @@ int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec
  * (device int32) receives status bits, 1 = a coordinate outside the key
  * domain (the synchronous form's ND_ERR_DOMAIN), 4 = a malformed tree.  Same
  * pairs in the same order as the synchronous form; the key sort covers all
- * 32 bits instead of probing which digits vary.  Work buffers grow on the
- * first call for a size, so call once before capturing a graph. */
+ * 32 bits instead of probing which digits vary.  Its work buffers belong to
+ * this form alone (no other call resizes them); they grow on the first call
+ * for a larger size, so call once at the largest size before capturing a
+ * graph, which then stays valid. */
 int nd_leafbuild_traverse_bvh_async(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
                                     int32_t predicate, int32_t morton_mode, nd_pair *d_out,
                                     int64_t cap, int64_t *d_count, int32_t *d_status);

@@ -101,6 +101,7 @@ SIGNATURES = {
     "nd_memcpy_d2d": [_P, _P, _P, _sz],
     "nd_memset": [_P, _P, _i32, _sz],
     "nd_synchronize": [_P],
+    "nd_ctx_stream": [_P, ctypes.POINTER(_P)],
     "nd_timer_start": [_P],
     "nd_timer_stop": [_P, ctypes.POINTER(_f32)],
     "nd_bvh_spec_init": [ctypes.POINTER(BvhSpec), _f32, _i64, _i64],

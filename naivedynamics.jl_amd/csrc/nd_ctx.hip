@@ -63,7 +63,7 @@ int nd_ctx_destroy(nd_ctx *ctx) {
                           &ctx->sort.keys_alt, &ctx->sort.vals_alt, &ctx->sort.hist,
                           &ctx->sort.flags, &ctx->cell_keys, &ctx->cell_vals, &ctx->cell_pos,
                           &ctx->cell_start, &ctx->trav_rec, &ctx->scan_tmp,
-                          &ctx->force_acc, &ctx->force_part};
+                          &ctx->force_acc, &ctx->force_part, &ctx->async_ws};
     for (auto *b : bufs) b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -124,6 +124,13 @@ int nd_memset(nd_ctx *ctx, void *d_dst, int32_t byte_value, size_t bytes) {
     ND_API_BEGIN
     ND_REQUIRE(ctx, "nd_memset: null context");
     if (bytes) ND_HIP(hipMemsetAsync(d_dst, byte_value, bytes, ctx->stream));
+    ND_API_END(ctx)
+}
+
+int nd_ctx_stream(nd_ctx *ctx, void **stream) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && stream, "nd_ctx_stream: null argument");
+    *stream = static_cast<void *>(ctx->stream);
     ND_API_END(ctx)
 }
 

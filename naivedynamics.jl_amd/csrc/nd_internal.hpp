@@ -74,10 +74,13 @@ struct nd_ctx {
     nd::DevBuf trav_rec;  // four-wide records of the one-call traversal
     nd::DevBuf scan_tmp;  // tile sums of scan_counts
     nd::DevBuf force_acc, force_part;  // pair-list forces: fixed-point sums, energy partials
+    nd::DevBuf async_ws;  // work buffers of nd_leafbuild_traverse_bvh_async (resized by it alone)
     nd::SortBufs sort;
-    // last tree built by nd_leafbuild_traverse_bvh
+    // last tree built by a one-call neighbour list (for nd_tree_export)
     int32_t tree_n = 0, tree_apl = 0;
     bool has_tree = false;
+    const nd_node *tree_nodes = nullptr;
+    const int32_t *tree_perm = nullptr, *tree_codes = nullptr;
 };
 
 namespace nd {
@@ -88,7 +91,11 @@ namespace nd {
 // nd_morton.hip
 void launch_morton(nd_ctx *ctx, const float *xyz, int32_t n, int32_t mode, const float lo[3],
                    const float scale[3], int32_t *codes, int32_t *idx, int32_t *status);
-void launch_bbox(nd_ctx *ctx, const float *xyz, int32_t n, float *bbox6);
+// the same with the box of the interleaved keys computed into box (6 floats,
+// device memory) on stream st: kernels only
+void launch_morton_box(hipStream_t st, float *box, const float *xyz, int32_t n, int32_t mode,
+                       int32_t *codes, int32_t *idx, int32_t *status);
+void launch_bbox(hipStream_t st, const float *xyz, int32_t n, float *bbox6);
 void launch_fill_i32(hipStream_t s, int32_t *p, int64_t n, int32_t v);
 void launch_iota(hipStream_t s, int32_t *p, int32_t n);
 void radix_sort_pairs_auto(nd_ctx *ctx, int32_t *keys, int32_t *vals, int32_t n);
@@ -155,6 +162,24 @@ int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted
 // or launch, synchronise and return the total.
 void launch_scan(nd_ctx *ctx, int64_t n);
 int64_t scan_counts(nd_ctx *ctx, int64_t n);
+
+// Work buffers of the stream-ordered one-call neighbour list, carved from one
+// block of device memory of list_work_bytes(n, apl) bytes.
+struct ListWork {
+    int32_t *codes, *perm, *keys_alt, *vals_alt, *hist, *store;
+    float *xyz_sorted, *box;
+    nd_node *nodes;
+    int64_t *counts, *offsets, *scan_tmp;
+    float4 *rec;
+};
+size_t list_work_bytes(int32_t n, int32_t apl);
+ListWork carve_list_work(void *base, int32_t n, int32_t apl);
+// keys, 32-bit sort, permutation, tree and traversal on st without host
+// synchronisation or allocation: the required pair count goes to *d_count,
+// status bits to *d_status, pairs to out when the count is <= cap.
+void launch_list_async(hipStream_t st, const ListWork &w, const float *d_xyz, int32_t n, int32_t apl,
+                       float r, int32_t pred, int32_t morton, nd_pair *out, int64_t cap, int64_t *d_count,
+                       int32_t *d_status);
 
 // Uniform cell grid: cell d-coordinate of x is clamp(floor((x - lo[d]) * inv[d]),
 // 0, g[d] - 1), computed in double so that it is exact to far below a cell;

@@ -135,36 +135,47 @@ void launch_gather_xyz(hipStream_t s, const float *xyz, const int32_t *perm, int
     ND_HIP(hipGetLastError());
 }
 
-void launch_bbox(nd_ctx *ctx, const float *xyz, int32_t n, float *bbox6) {
+void launch_bbox(hipStream_t st, const float *xyz, int32_t n, float *bbox6) {
     int32_t *bb = reinterpret_cast<int32_t *>(bbox6);
-    k_bbox_init<<<1, 64, 0, ctx->stream>>>(bb);
+    k_bbox_init<<<1, 64, 0, st>>>(bb);
     // few workgroups: their atomics on the six words serialise in L2
     unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(n), 64));
-    k_bbox<<<g, kBlock, 0, ctx->stream>>>(xyz, n, bb);
-    k_bbox_finalize<<<1, 64, 0, ctx->stream>>>(bb);
+    k_bbox<<<g, kBlock, 0, st>>>(xyz, n, bb);
+    k_bbox_finalize<<<1, 64, 0, st>>>(bb);
+    ND_HIP(hipGetLastError());
+}
+
+void launch_morton_box(hipStream_t st, float *box, const float *xyz, int32_t n, int32_t mode,
+                       int32_t *codes, int32_t *idx, int32_t *status) {
+    if (n <= 0) return;
+    if (mode == ND_MORTON_MASKED) {
+        k_morton_masked<<<grid_for(n), kBlock, 0, st>>>(xyz, n, codes, idx, status);
+    } else {
+        launch_bbox(st, xyz, n, box);
+        if (mode == ND_MORTON_HILBERT)
+            k_morton_interleave<true><<<grid_for(n), kBlock, 0, st>>>(xyz, n, box, codes, idx, status);
+        else
+            k_morton_interleave<false><<<grid_for(n), kBlock, 0, st>>>(xyz, n, box, codes, idx, status);
+    }
     ND_HIP(hipGetLastError());
 }
 
 void launch_morton(nd_ctx *ctx, const float *xyz, int32_t n, int32_t mode, const float lo[3],
                    const float scale[3], int32_t *codes, int32_t *idx, int32_t *status) {
     if (n <= 0) return;
-    if (mode == ND_MORTON_MASKED) {
-        k_morton_masked<<<grid_for(n), kBlock, 0, ctx->stream>>>(xyz, n, codes, idx, status);
-    } else {
-        ctx->bbox.reserve(64);
-        float *box = ctx->bbox.as<float>();
-        if (lo && scale) {
-            float h[6] = {lo[0], lo[1], lo[2], scale[0], scale[1], scale[2]};
-            ND_HIP(hipMemcpyAsync(box, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
-        } else {
-            launch_bbox(ctx, xyz, n, box);
-        }
+    ctx->bbox.reserve(64);
+    float *box = ctx->bbox.as<float>();
+    if (mode != ND_MORTON_MASKED && lo && scale) {
+        float h[6] = {lo[0], lo[1], lo[2], scale[0], scale[1], scale[2]};
+        ND_HIP(hipMemcpyAsync(box, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
         if (mode == ND_MORTON_HILBERT)
             k_morton_interleave<true><<<grid_for(n), kBlock, 0, ctx->stream>>>(xyz, n, box, codes, idx, status);
         else
             k_morton_interleave<false><<<grid_for(n), kBlock, 0, ctx->stream>>>(xyz, n, box, codes, idx, status);
+        ND_HIP(hipGetLastError());
+        return;
     }
-    ND_HIP(hipGetLastError());
+    launch_morton_box(ctx->stream, box, xyz, n, mode, codes, idx, status);
 }
 
 }  // namespace nd

@@ -573,20 +573,31 @@ k_bruteforce(const float *__restrict__ xyz, int32_t n, float r, int64_t *__restr
 
 namespace nd {
 
-void launch_scan(nd_ctx *ctx, int64_t n) {
-    // counts in ctx->counts (n), offsets to ctx->offsets (n+1), no host sync
+// int64 entries of the tile sums a scan of n counts needs (none when one
+// workgroup scans them all)
+inline int64_t scan_tmp_entries(int64_t n) {
+    return n <= 4 * kScanTile ? 0 : 2 * ((n + kScanTile - 1) / kScanTile) + 1;
+}
+
+// counts (n) -> exclusive offsets (n + 1) on st; tmp holds scan_tmp_entries(n)
+void launch_scan_buf(hipStream_t st, const int64_t *counts, int64_t n, int64_t *offsets, int64_t *tmp) {
     if (n <= 4 * kScanTile) {
-        k_scan_i64<<<1, 1024, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, ctx->offsets.as<int64_t>());
+        k_scan_i64<<<1, 1024, 0, st>>>(counts, n, offsets);
     } else {
         const int64_t nt = (n + kScanTile - 1) / kScanTile;
-        ctx->scan_tmp.reserve(sizeof(int64_t) * (size_t)(2 * nt + 1));
-        int64_t *sums = ctx->scan_tmp.as<int64_t>(), *off = sums + nt;
-        k_scan_tile_sums<<<(unsigned)nt, kBlock, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, sums);
-        k_scan_i64<<<1, 1024, 0, ctx->stream>>>(sums, nt, off);
-        k_scan_tile_apply<<<(unsigned)nt, kBlock, 0, ctx->stream>>>(ctx->counts.as<int64_t>(), n, off, nt,
-                                                                     ctx->offsets.as<int64_t>());
+        int64_t *sums = tmp, *off = sums + nt;
+        k_scan_tile_sums<<<(unsigned)nt, kBlock, 0, st>>>(counts, n, sums);
+        k_scan_i64<<<1, 1024, 0, st>>>(sums, nt, off);
+        k_scan_tile_apply<<<(unsigned)nt, kBlock, 0, st>>>(counts, n, off, nt, offsets);
     }
     ND_HIP(hipGetLastError());
+}
+
+void launch_scan(nd_ctx *ctx, int64_t n) {
+    // counts in ctx->counts (n), offsets to ctx->offsets (n+1), no host sync
+    ctx->scan_tmp.reserve(sizeof(int64_t) * (size_t)(scan_tmp_entries(n) + 1));
+    launch_scan_buf(ctx->stream, ctx->counts.as<int64_t>(), n, ctx->offsets.as<int64_t>(),
+                    ctx->scan_tmp.as<int64_t>());
 }
 
 int64_t scan_counts(nd_ctx *ctx, int64_t n) {
@@ -693,7 +704,121 @@ int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted
     return total;
 }
 
+namespace {
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// counts of the traversal: per (leaf, slice) for leaf sizes dividing 64 (the
+// one-walk-per-wave traversal of a library-built tree), else per query leaf
+int64_t list_count_slots(int32_t n, int32_t apl) {
+    const int32_t nL = n / apl;
+    return (apl <= 64 && 64 % apl == 0) ? (int64_t)nL * trav_slices(nL) : (int64_t)nL;
+}
+
+}  // namespace
+
+size_t list_work_bytes(int32_t n, int32_t apl) {
+    const int32_t nL = n / apl;
+    const size_t nn = (size_t)n, ntiles = radix_scratch_blocks(n);
+    const int64_t nc = list_count_slots(n, apl);
+    size_t b = 0;
+    b += 4 * align256(sizeof(int32_t) * nn);                           // codes, perm, keys_alt, vals_alt
+    b += align256(sizeof(int32_t) * (256 * ntiles + 256));             // 8-bit digit histogram
+    b += align256(sizeof(int32_t) * lbvh_scratch_ints(nL));            // CAS slots, ranges, chunks
+    b += align256(sizeof(float) * 3 * nn) + 256;                       // sorted positions, box
+    b += align256(sizeof(nd_node) * (size_t)(2 * nL - 1));             // tree
+    b += 2 * align256(sizeof(int64_t) * (size_t)(nc + 1));             // counts, offsets
+    b += align256(sizeof(int64_t) * (size_t)(scan_tmp_entries(nc + 1) + 1));
+    b += align256(8 * sizeof(float4) * (size_t)(nL > 1 ? nL - 1 : 1));  // four-wide records
+    return b;
+}
+
+ListWork carve_list_work(void *base, int32_t n, int32_t apl) {
+    const int32_t nL = n / apl;
+    const size_t nn = (size_t)n, ntiles = radix_scratch_blocks(n);
+    const int64_t nc = list_count_slots(n, apl);
+    char *p = static_cast<char *>(base);
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += align256(bytes);
+        return q;
+    };
+    ListWork w;
+    w.codes = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * nn));
+    w.perm = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * nn));
+    w.keys_alt = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * nn));
+    w.vals_alt = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * nn));
+    w.hist = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * (256 * ntiles + 256)));
+    w.store = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * lbvh_scratch_ints(nL)));
+    w.xyz_sorted = reinterpret_cast<float *>(take(sizeof(float) * 3 * nn));
+    w.box = reinterpret_cast<float *>(take(256));
+    w.nodes = reinterpret_cast<nd_node *>(take(sizeof(nd_node) * (size_t)(2 * nL - 1)));
+    w.counts = reinterpret_cast<int64_t *>(take(sizeof(int64_t) * (size_t)(nc + 1)));
+    w.offsets = reinterpret_cast<int64_t *>(take(sizeof(int64_t) * (size_t)(nc + 1)));
+    w.scan_tmp = reinterpret_cast<int64_t *>(take(sizeof(int64_t) * (size_t)(scan_tmp_entries(nc + 1) + 1)));
+    w.rec = reinterpret_cast<float4 *>(take(8 * sizeof(float4) * (size_t)(nL > 1 ? nL - 1 : 1)));
+    return w;
+}
+
+void launch_list_async(hipStream_t st, const ListWork &w, const float *d_xyz, int32_t n, int32_t apl,
+                       float r, int32_t pred, int32_t morton, nd_pair *out, int64_t cap, int64_t *d_count,
+                       int32_t *d_status) {
+    const int32_t nL = n / apl;
+    launch_fill_i32(st, d_status, 1, 0);
+    launch_morton_box(st, w.box, d_xyz, n, morton, w.codes, w.perm, d_status);
+    // every key bit (no host round trip to find the varying digits); the
+    // sort's buffers are views of the work block, sized so nothing grows
+    SortBufs sb;
+    const size_t ntiles = radix_scratch_blocks(n);
+    sb.keys_alt = DevBuf{w.keys_alt, sizeof(int32_t) * (size_t)n};
+    sb.vals_alt = DevBuf{w.vals_alt, sizeof(int32_t) * (size_t)n};
+    sb.hist = DevBuf{w.hist, sizeof(int32_t) * (256 * ntiles + 256)};
+    radix_sort_pairs(st, sb, w.codes, w.perm, n, 0, 32);
+    launch_gather_xyz(st, d_xyz, w.perm, n, w.xyz_sorted);
+    launch_lbvh(st, w.xyz_sorted, 3, w.codes, nL, apl, r, 1, w.nodes, w.store, d_status);
+    // traversal: count, scan, emit (the emitting pass checks the total
+    // against cap itself), total to *d_count
+    const float r2 = r * r;
+    const int64_t nc = list_count_slots(n, apl);
+    const bool wave_walk = apl <= 64 && 64 % apl == 0;
+    auto pass = [&](bool emit) {
+        if (wave_walk) {
+            if (emit)
+                launch_traverse_w<true>(st, apl, w.nodes, w.rec, w.xyz_sorted, w.perm, nL, r, r2, pred, w.counts,
+                                        w.offsets, out, d_status, cap);
+            else
+                launch_traverse_w<false>(st, apl, w.nodes, w.rec, w.xyz_sorted, w.perm, nL, r, r2, pred, w.counts,
+                                         w.offsets, nullptr, d_status, cap);
+        } else if (apl == 4) {
+            if (emit)
+                k_leaf_traverse4<true><<<grid_for(nL), kBlock, 0, st>>>(w.nodes, w.xyz_sorted, w.perm, nL, r, r2,
+                                                                        pred, w.counts, w.offsets, out, d_status, cap);
+            else
+                k_leaf_traverse4<false><<<grid_for(nL), kBlock, 0, st>>>(w.nodes, w.xyz_sorted, w.perm, nL, r, r2,
+                                                                         pred, w.counts, w.offsets, nullptr, d_status, cap);
+        } else {
+            if (emit)
+                k_leaf_traverse<true><<<grid_for(nL), kBlock, 0, st>>>(w.nodes, w.xyz_sorted, w.perm, nL, apl, r,
+                                                                       r2, pred, w.counts, w.offsets, out, d_status, cap);
+            else
+                k_leaf_traverse<false><<<grid_for(nL), kBlock, 0, st>>>(w.nodes, w.xyz_sorted, w.perm, nL, apl, r,
+                                                                        r2, pred, w.counts, w.offsets, nullptr, d_status, cap);
+        }
+        ND_HIP(hipGetLastError());
+    };
+    if (wave_walk) {
+        k_trav_records<<<grid_for(nL - 1), kBlock, 0, st>>>(w.nodes, nL, w.rec);
+        ND_HIP(hipGetLastError());
+    }
+    pass(false);
+    launch_scan_buf(st, w.counts, nc, w.offsets, w.scan_tmp);
+    if (out) pass(true);
+    k_copy_total<<<1, 64, 0, st>>>(w.offsets + nc, d_count);
+    ND_HIP(hipGetLastError());
+}
+
 }  // namespace nd
+
 
 namespace {
 
@@ -767,6 +892,9 @@ int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec
     ctx->has_tree = true;
     ctx->tree_n = n;
     ctx->tree_apl = apl;
+    ctx->tree_nodes = ctx->nodes.as<nd_node>();
+    ctx->tree_perm = ctx->perm.as<int32_t>();
+    ctx->tree_codes = ctx->codes_sorted.as<int32_t>();
     const int64_t total = nd::leaf_traverse(ctx, ctx->nodes.as<nd_node>(), ctx->xyz_sorted.as<float>(),
                                             ctx->perm.as<int32_t>(), n, apl, chk.neighbor_distance,
                                             predicate, d_out, cap, true);
@@ -786,27 +914,19 @@ int nd_leafbuild_traverse_bvh_async(nd_ctx *ctx, const float *d_xyz, const nd_bv
     ND_REQUIRE(morton_mode == ND_MORTON_MASKED || morton_mode == ND_MORTON_INTERLEAVE,
                "unknown key mode");
     ND_HIP(hipSetDevice(ctx->device));
-    const int32_t n = chk.atom_count, apl = chk.atomsperleaf, nL = chk.leaves_count;
-    hipStream_t st = ctx->stream;
-    ctx->has_tree = false;
-    ctx->codes_sorted.reserve(sizeof(int32_t) * (size_t)n);
-    ctx->perm.reserve(sizeof(int32_t) * (size_t)n);
-    ctx->xyz_sorted.reserve(sizeof(float) * 3 * (size_t)n);
-    ctx->nodes.reserve(sizeof(nd_node) * (size_t)(2 * nL - 1));
-    ctx->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(nL));
-    nd::launch_fill_i32(st, d_status, 1, 0);
-    nd::launch_morton(ctx, d_xyz, n, morton_mode, nullptr, nullptr, ctx->codes_sorted.as<int32_t>(),
-                      ctx->perm.as<int32_t>(), d_status);
-    // every key bit: the synchronous form's varying-digit probe reads back
-    nd::radix_sort_pairs(st, ctx->sort, ctx->codes_sorted.as<int32_t>(), ctx->perm.as<int32_t>(), n, 0, 32);
-    nd::launch_gather_xyz(st, d_xyz, ctx->perm.as<int32_t>(), n, ctx->xyz_sorted.as<float>());
-    nd::launch_lbvh(st, ctx->xyz_sorted.as<float>(), 3, ctx->codes_sorted.as<int32_t>(), nL, apl,
-                    chk.neighbor_distance, 1, ctx->nodes.as<nd_node>(), ctx->store.as<int32_t>(), d_status);
+    const int32_t n = chk.atom_count, apl = chk.atomsperleaf;
+    // this form's own work block: no other entry point resizes it, so a graph
+    // captured after a first call of this size stays valid
+    ctx->async_ws.reserve(nd::list_work_bytes(n, apl));
+    const nd::ListWork w = nd::carve_list_work(ctx->async_ws.p, n, apl);
+    nd::launch_list_async(ctx->stream, w, d_xyz, n, apl, chk.neighbor_distance, predicate, morton_mode, d_out,
+                          cap, d_count, d_status);
     ctx->has_tree = true;
     ctx->tree_n = n;
     ctx->tree_apl = apl;
-    nd::leaf_traverse(ctx, ctx->nodes.as<nd_node>(), ctx->xyz_sorted.as<float>(), ctx->perm.as<int32_t>(), n,
-                      apl, chk.neighbor_distance, predicate, d_out, cap, true, d_status, d_count);
+    ctx->tree_nodes = w.nodes;
+    ctx->tree_perm = w.perm;
+    ctx->tree_codes = w.codes;
     ND_API_END(ctx)
 }
 
@@ -839,13 +959,13 @@ int nd_tree_export(nd_ctx *ctx, nd_node *h_nodes, int32_t *h_perm, int32_t *h_co
     if (!ctx->has_tree) throw nd::Error(ND_ERR_STATE, "nd_tree_export: no tree has been built");
     const size_t n = (size_t)ctx->tree_n, nL = n / (size_t)ctx->tree_apl;
     if (h_nodes)
-        ND_HIP(hipMemcpyAsync(h_nodes, ctx->nodes.p, sizeof(nd_node) * (2 * nL - 1),
+        ND_HIP(hipMemcpyAsync(h_nodes, ctx->tree_nodes, sizeof(nd_node) * (2 * nL - 1),
                               hipMemcpyDeviceToHost, ctx->stream));
     if (h_perm)
-        ND_HIP(hipMemcpyAsync(h_perm, ctx->perm.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+        ND_HIP(hipMemcpyAsync(h_perm, ctx->tree_perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
                               ctx->stream));
     if (h_codes_sorted)
-        ND_HIP(hipMemcpyAsync(h_codes_sorted, ctx->codes_sorted.p, sizeof(int32_t) * n,
+        ND_HIP(hipMemcpyAsync(h_codes_sorted, ctx->tree_codes, sizeof(int32_t) * n,
                               hipMemcpyDeviceToHost, ctx->stream));
     ND_HIP(hipStreamSynchronize(ctx->stream));
     ND_API_END(ctx)

@@ -306,3 +306,68 @@ def test_async_entry_point_matches_sync(pkg, apl, pred):
     bad[3, 1] = 3.0e9  # beyond round(Int32, x / binwidth) in the reference key
     _, status = call(out, nref + 16, pkg.DeviceArray.from_host(ctx, bad), pkg.MORTON_MASKED)
     assert status & 1
+
+
+def _hip():
+    """The HIP runtime the library is linked against (already loaded)."""
+    import ctypes
+    h = ctypes.CDLL("libamdhip64.so.7")
+    P = ctypes.c_void_p
+    h.hipStreamBeginCapture.argtypes = [P, ctypes.c_int]
+    h.hipStreamEndCapture.argtypes = [P, ctypes.POINTER(P)]
+    h.hipGraphInstantiate.argtypes = [ctypes.POINTER(P), P, P, P, ctypes.c_size_t]
+    h.hipGraphLaunch.argtypes = [P, P]
+    h.hipGraphExecDestroy.argtypes = [P]
+    h.hipGraphDestroy.argtypes = [P]
+    return h
+
+
+def test_async_entry_point_in_a_captured_graph(pkg):
+    """The stream-ordered one-call list captured once in a hipGraph and
+    replayed on new positions written into the same input buffer: every
+    replay equals the synchronous call on those positions, also after other
+    entry points on the context grew their own work buffers (the async form's
+    buffers are its own), with host work and synchronisation between replays."""
+    import ctypes
+    hip = _hip()
+    n, apl = 8000, 4
+    L = (n / 0.8) ** (1 / 3)
+    spec = pkg.SpheresBVHSpecs(neighbor_distance=2.5, atom_count=n, atomsperleaf=apl)
+    ctx = pkg.Context(0)
+    lib = pkg.library()
+    xs = [_rand(n, L, 900 + k) for k in range(4)]
+    d = pkg.DeviceArray.from_host(ctx, xs[0])
+    cnt = pkg.DeviceArray.from_host(ctx, np.zeros(1, np.int64))
+    st = pkg.DeviceArray.from_host(ctx, np.zeros(1, np.int32))
+    cap = 400000
+    out = pkg.DeviceArray(ctx, (cap,), pkg.PAIR_DTYPE)
+    args = (ctx.handle, d.ptr, ctypes.byref(spec.c), pkg.PRED_SCALAR, pkg.MORTON_INTERLEAVE, out.ptr, cap,
+            cnt.ptr, st.ptr)
+    assert lib.nd_leafbuild_traverse_bvh_async(*args) == 0      # sizes its work buffers
+    ctx.synchronize()
+    stream = ctypes.c_void_p()
+    assert lib.nd_ctx_stream(ctx.handle, ctypes.byref(stream)) == 0
+    graph, gexec = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipStreamBeginCapture(stream, 2) == 0            # relaxed
+    assert lib.nd_leafbuild_traverse_bvh_async(*args) == 0
+    assert hip.hipStreamEndCapture(stream, ctypes.byref(graph)) == 0
+    assert hip.hipGraphInstantiate(ctypes.byref(gexec), graph, None, None, 0) == 0
+    try:
+        for k, x in enumerate(xs):
+            if k == 2:
+                # other entry points grow the context's shared buffers
+                big = _rand(3 * n, (3 * n / 0.8) ** (1 / 3), 77)
+                pkg.cells.neighborlist(big, 2.5, ctx=ctx)
+                pkg.threshold_unique_pairs(big[:5000], 2.5, ctx=ctx)
+            ref_out, nref = pkg.neighborlist_device(x, spec, pkg.PRED_SCALAR, pkg.MORTON_INTERLEAVE, ctx=ctx)
+            ref = pairs_to_rows(ref_out.to_host(nref))
+            check = ctypes.c_void_p(d.ptr.value)
+            assert lib.nd_memcpy_h2d(ctx.handle, check, x.ctypes.data_as(ctypes.c_void_p), x.nbytes) == 0
+            assert hip.hipGraphLaunch(gexec, stream) == 0
+            ctx.synchronize()
+            assert int(cnt.to_host()[0]) == nref and int(st.to_host()[0]) == 0, k
+            assert np.array_equal(pairs_to_rows(out.to_host(nref)), ref), k
+    finally:
+        hip.hipGraphExecDestroy(gexec)
+        hip.hipGraphDestroy(graph)
+        ctx.close()
