@@ -156,6 +156,18 @@ int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec
 int nd_leafbuild_traverse_bvh_async(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
                                     int32_t predicate, int32_t morton_mode, nd_pair *d_out,
                                     int64_t cap, int64_t *d_count, int32_t *d_status);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  nd_leafbuild_traverse_bvh_async with its work buffers in a
+ * static array of the library (32 MiB per GPU, held until the process ends)
+ * instead of allocated memory: it calls no allocator.  Sizes whose work
+ * buffers exceed the array (about 290,000 atoms at 4 atoms per leaf, more for
+ * larger leaves) are rejected with ND_ERR_CAPACITY.  Every call on one GPU
+ * uses that GPU's array, so calls that overlap on one GPU (two contexts or
+ * streams) race; the array keeps the last call's tree for nd_tree_export. */
+int nd_leafbuild_traverse_bvh_noalloc(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
+                                      int32_t predicate, int32_t morton_mode, nd_pair *d_out,
+                                      int64_t cap, int64_t *d_count, int32_t *d_status);
 /* Host-buffer form of the same call (stages h_xyz in and pairs out). */
 int nd_leafbuild_traverse_bvh_host(nd_ctx *ctx, const float *h_xyz, const nd_bvh_spec *spec,
                                    int32_t predicate, int32_t morton_mode, nd_pair *h_out,

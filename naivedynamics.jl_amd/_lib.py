@@ -113,6 +113,7 @@ SIGNATURES = {
     "nd_leafbuild_traverse_bvh": [_P, _P, ctypes.POINTER(BvhSpec), _i32, _i32, _P, _i64,
                                   ctypes.POINTER(_i64)],
     "nd_leafbuild_traverse_bvh_async": [_P, _P, ctypes.POINTER(BvhSpec), _i32, _i32, _P, _i64, _P, _P],
+    "nd_leafbuild_traverse_bvh_noalloc": [_P, _P, ctypes.POINTER(BvhSpec), _i32, _i32, _P, _i64, _P, _P],
     "nd_leafbuild_traverse_bvh_host": [_P, _P, ctypes.POINTER(BvhSpec), _i32, _i32, _P, _i64,
                                        ctypes.POINTER(_i64)],
     "nd_tree_export": [_P, _P, _P, _P],

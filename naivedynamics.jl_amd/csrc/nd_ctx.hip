@@ -39,6 +39,7 @@ int nd_ctx_create(int32_t device, nd_ctx **out) {
         ctx->device = device;
         ND_HIP(hipSetDevice(device));
         ND_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        (void)nd::static_scratch(device);  // the device's scratch address, once per device
         ND_HIP(hipEventCreate(&ctx->ev0));
         ND_HIP(hipEventCreate(&ctx->ev1));
         nd::clear_status(ctx);

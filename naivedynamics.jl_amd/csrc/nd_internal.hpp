@@ -180,6 +180,9 @@ ListWork carve_list_work(void *base, int32_t n, int32_t apl);
 void launch_list_async(hipStream_t st, const ListWork &w, const float *d_xyz, int32_t n, int32_t apl,
                        float r, int32_t pred, int32_t morton, nd_pair *out, int64_t cap, int64_t *d_count,
                        int32_t *d_status);
+// Device address of the library's static scratch array on the current device
+// (looked up once per device).
+void *static_scratch(int device);
 
 // Uniform cell grid: cell d-coordinate of x is clamp(floor((x - lo[d]) * inv[d]),
 // 0, g[d] - 1), computed in double so that it is exact to far below a cell;

@@ -10,6 +10,8 @@
 // query leaf; a counting pass, an exclusive scan and an emitting pass place
 // every leaf's pairs at a fixed offset, so the output is in exactly the order
 // of a single-threaded reference run.
+#include <mutex>
+
 #include "nd_internal.hpp"
 
 namespace {
@@ -819,6 +821,36 @@ void launch_list_async(hipStream_t st, const ListWork &w, const float *d_xyz, in
 
 }  // namespace nd
 
+// ---- static scratch of the allocation-free one-call list ----
+namespace {
+
+// Work buffers of nd_leafbuild_traverse_bvh_noalloc: one array per GPU, held
+// by the library until the process ends.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+}  // namespace
+
+namespace nd {
+
+void *static_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(ND_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        // the device's own copy of the array (host code never uses its name
+        // as an address); the caller has made `device` current
+        void *p = nullptr;
+        ND_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[device] = p;
+    }
+    return g_scratch[device];
+}
+
+}  // namespace nd
 
 namespace {
 
@@ -919,6 +951,35 @@ int nd_leafbuild_traverse_bvh_async(nd_ctx *ctx, const float *d_xyz, const nd_bv
     // captured after a first call of this size stays valid
     ctx->async_ws.reserve(nd::list_work_bytes(n, apl));
     const nd::ListWork w = nd::carve_list_work(ctx->async_ws.p, n, apl);
+    nd::launch_list_async(ctx->stream, w, d_xyz, n, apl, chk.neighbor_distance, predicate, morton_mode, d_out,
+                          cap, d_count, d_status);
+    ctx->has_tree = true;
+    ctx->tree_n = n;
+    ctx->tree_apl = apl;
+    ctx->tree_nodes = w.nodes;
+    ctx->tree_perm = w.perm;
+    ctx->tree_codes = w.codes;
+    ND_API_END(ctx)
+}
+
+int nd_leafbuild_traverse_bvh_noalloc(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec,
+                                      int32_t predicate, int32_t morton_mode, nd_pair *d_out, int64_t cap,
+                                      int64_t *d_count, int32_t *d_status) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && d_xyz && d_count && d_status, "nd_leafbuild_traverse_bvh_noalloc: null argument");
+    nd_bvh_spec chk;
+    check_spec(spec, chk);
+    check_pred(predicate);
+    ND_REQUIRE(morton_mode == ND_MORTON_MASKED || morton_mode == ND_MORTON_INTERLEAVE,
+               "unknown key mode");
+    const int32_t n = chk.atom_count, apl = chk.atomsperleaf;
+    if (nd::list_work_bytes(n, apl) > kScratchBytes)
+        throw nd::Error(ND_ERR_CAPACITY, "nd_leafbuild_traverse_bvh_noalloc: work buffers for " + std::to_string(n) +
+                                             " atoms (" + std::to_string(nd::list_work_bytes(n, apl)) +
+                                             " bytes) exceed the static scratch of " +
+                                             std::to_string(kScratchBytes) + " bytes");
+    ND_HIP(hipSetDevice(ctx->device));
+    const nd::ListWork w = nd::carve_list_work(nd::static_scratch(ctx->device), n, apl);
     nd::launch_list_async(ctx->stream, w, d_xyz, n, apl, chk.neighbor_distance, predicate, morton_mode, d_out,
                           cap, d_count, d_status);
     ctx->has_tree = true;
