@@ -275,6 +275,64 @@ def adjoint_form(pkg, steps=2000):
     return out
 
 
+def adjoint_fd_table(pkg, horizons=(50, 100, 250, 500)):
+    """The differentiation form's gradients at its own size (32,768 atoms, LJ +
+    Coulomb, shifted force, checkpoints every 50 steps) over growing horizons,
+    against central finite differences of the same forward model: FP64 with
+    h = 1e-5 and 1e-3, FP32 with h = 1e-3; and the FP32 adjoint against the
+    FP64 one.  Relative errors |ad - fd| / |fd| per parameter."""
+    n = 32768
+    L = pkg.synth.box_side(n)
+    x = pkg.synth.lattice_positions(n, L, 5)
+    v = pkg.synth.gaussian_velocities(n, 1.0, 5)
+    q = pkg.synth.balanced_charges(n, 5)
+    names = ["eps", "sigma", "q77", "q12345", "x0[0].x", "x0[4321].z"]
+    rows = []
+    for T in horizons:
+        grads = {}
+        for prec, hs in (("fp64", (1e-5, 1e-3)), ("fp32", (1e-3,))):
+            adj = pkg.Adjoint(n, T, dt=0.001, precision=prec, checkpoint_every=50, rebuild_every=10,
+                              nbins=50, rdf_rmax=2.5, rdf_h=0.1, volume=L ** 3)
+            adj.set_potential(1.1, 0.95)
+            _, target = adj.forward(x, v, q)
+            adj.set_potential(1.0, 1.0)
+            adj.forward(x, v, q, target)
+            g = adj.backward()
+            ad = [g["deps"], g["dsigma"], g["dq"][77], g["dq"][12345], g["dx0"][0, 0], g["dx0"][4321, 2]]
+            grads[prec] = ad
+
+            def loss(eps=1.0, sigma=1.0, qv=q, xv=x):
+                adj.set_potential(eps, sigma)
+                return adj.forward(xv, v, qv, target)[0]
+
+            for h in hs:
+                fd = []
+                fd.append((loss(eps=1 + h) - loss(eps=1 - h)) / (2 * h))
+                fd.append((loss(sigma=1 + h) - loss(sigma=1 - h)) / (2 * h))
+                for i in (77, 12345):
+                    qp, qm = q.copy(), q.copy()
+                    qp[i] += h
+                    qm[i] -= h
+                    fd.append((loss(qv=qp) - loss(qv=qm)) / (2 * h))
+                for i, c in ((0, 0), (4321, 2)):
+                    xp, xm = x.copy(), x.copy()
+                    xp[i, c] += h
+                    xm[i, c] -= h
+                    fd.append((loss(xv=xp) - loss(xv=xm)) / (2 * h))
+                rel = [abs(a - f) / max(abs(f), 1e-300) for a, f in zip(ad, fd)]
+                rows.append({"steps": T, "precision": prec, "h": h, "max_rel_err": float("%.3e" % max(rel)),
+                             "rel_err": {k: float("%.2e" % r) for k, r in zip(names, rel)},
+                             "adjoint": {k: float("%.6e" % a) for k, a in zip(names, ad)},
+                             "fd": {k: float("%.6e" % f) for k, f in zip(names, fd)}})
+                log("adjoint fd: %d steps %s h=%g: max rel err %.2e" % (T, prec, h, max(rel)))
+            adj.close()
+        d = [abs(a - b) / max(abs(b), 1e-300) for a, b in zip(grads["fp32"], grads["fp64"])]
+        rows.append({"steps": T, "fp32_vs_fp64_rel": {k: float("%.2e" % r) for k, r in zip(names, d)},
+                     "max_rel": float("%.3e" % max(d))})
+        log("adjoint fd: %d steps fp32 vs fp64: max rel %.2e" % (T, max(d)))
+    return {"workload": "ljc-adjoint-32768", "n": n, "checkpoint_every": 50, "rows": rows}
+
+
 def cpu_baseline(c, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the CPU restatement of the reference path
@@ -300,7 +358,7 @@ def main():
                     help="only the FP32 timed region (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 5 x 256 MiB device-to-device (counter calibration run)")
-    ap.add_argument("--form", choices=("precision", "batched", "adjoint", "search"),
+    ap.add_argument("--form", choices=("precision", "batched", "adjoint", "adjoint_fd", "search"),
                     help="only this form, with --form-steps steps (for profiler runs)")
     ap.add_argument("--form-steps", type=int, default=0)
     args = ap.parse_args()
@@ -316,6 +374,7 @@ def main():
         return
     if args.form:
         fn = {"precision": precision_form, "batched": batched_form, "adjoint": adjoint_form,
+              "adjoint_fd": lambda pkg: adjoint_fd_table(pkg),
               "search": lambda pkg: search_comparison(pkg, pkg.synth.plain_scale(seed=2))}[args.form]
         out = fn(pkg, args.form_steps) if args.form_steps > 0 else fn(pkg)
         print(json.dumps({"form": args.form, "result": out}, default=float), flush=True)

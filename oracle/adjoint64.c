@@ -84,6 +84,40 @@ static void forces(const double *x, const double *q, int n, const adj64_params *
         }
 }
 
+/* The shifted-force potential energy from its definition,
+ * U_sf(r) = U(r) - U(rc) - (r - rc) U'(rc) for r < rc, summed over pairs:
+ * U_LJ = 4 eps ((s/r)^12 - (s/r)^6) and U'_LJ(rc) from its own closed form
+ * here, U_C = k q_i q_j / r, U'_C = -k q_i q_j / r^2.  It is the function the
+ * forces above must be minus the gradient of. */
+static double lj_u(double r, double e, double s) {
+    double sr6 = pow(s / r, 6.0);
+    return 4.0 * e * (sr6 * sr6 - sr6);
+}
+static double energy(const double *x, const double *q, int n, const adj64_params *p) {
+    double U = 0.0, R = reach(p);
+    double dlj_rc = 4.0 * p->eps * (-12.0 * pow(p->sigma / p->rc, 12.0) + 6.0 * pow(p->sigma / p->rc, 6.0)) / p->rc;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double r2 = 0.0;
+            for (int c = 0; c < 3; ++c) { double d = x[3 * i + c] - x[3 * j + c]; r2 += d * d; }
+            if (r2 >= R * R) continue;
+            double r = sqrt(r2);
+            if (r < p->rc)
+                U += lj_u(r, p->eps, p->sigma) - lj_u(p->rc, p->eps, p->sigma) - (r - p->rc) * dlj_rc;
+            if (q && p->rc_coul > 0.0 && r < p->rc_coul) {
+                double kqq = p->k_coul * q[i] * q[j];
+                U += kqq / r - kqq / p->rc_coul + (r - p->rc_coul) * kqq / (p->rc_coul * p->rc_coul);
+            }
+        }
+    return U;
+}
+
+/* The oracle's forward-model forces F = -grad U_sf (n x 3) and U_sf, for tests. */
+void adj64_forces(const double *x, const double *q, int n, const adj64_params *p, double *F, double *U) {
+    forces(x, q, n, p, F);
+    if (U) *U = energy(x, q, n, p);
+}
+
 /* lam += J(x)^T a (J = dF/dx = -Hess U, symmetric); grads += (dF/dtheta)^T a */
 static void jt_product(const double *x, const double *q, int n, const adj64_params *p,
                        const double *a, double *lam, double *dq, double *dpar) {

@@ -97,6 +97,8 @@ def lib():
                                         ctypes.c_double, P, P]
         L.base_run.argtypes = [P, P, P, i32, P, i32, P, P]
         L.adj64_run.argtypes = [P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P]
+        L.adj64_forces.argtypes = [P, P, ctypes.c_int, P, P, P]
+        L.adj64_forces.restype = None
         _lib = L
     return _lib
 
@@ -331,3 +333,14 @@ def adj64_run(x0, v0, q, params, target=None, grads=False):
     if grads:
         return L.value, g, dx, dv, dq, dp
     return L.value, g
+
+
+def adj64_forces(x, q, params):
+    """Forces and shifted-force energy of the adjoint oracle's forward model."""
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    n = len(x)
+    qq = None if q is None else np.ascontiguousarray(q, np.float64)
+    F = np.zeros((n, 3))
+    U = ctypes.c_double(0)
+    lib().adj64_forces(_p(x), _p(qq), n, ctypes.byref(params), _p(F), ctypes.byref(U))
+    return F, U.value

@@ -118,3 +118,53 @@ def test_size_rule(pkg):
     # the adjoint's list walk uses leaves of 4, 8 or 16 atoms
     with pytest.raises(pkg._lib.SpecError):
         pkg.Adjoint(50, 10)
+
+
+def test_config_scale_gradients_vs_central_differences(pkg):
+    """The differentiation workload at its own size (32,768 atoms, LJ +
+    Coulomb shifted force, checkpoints every 50 steps) on a 50-step horizon:
+    every gradient component checked (eps, sigma, two charges, two position
+    components) within 1e-4 relative of central finite differences of the
+    same FP64 forward model (h = 1e-5); the FP32 adjoint within 2e-4 of the
+    FP64 one.  (Over longer horizons the FP64 check holds to 6e-6 at 500
+    steps; FP32 drifts to ~6e-4 of FP64 by 250 steps; at 2,000 steps the
+    trajectory is chaotic and h = 1e-3 leaves the linear regime.)"""
+    n, T = 32768, 50
+    L = pkg.synth.box_side(n)
+    x = pkg.synth.lattice_positions(n, L, 5)
+    v = pkg.synth.gaussian_velocities(n, 1.0, 5)
+    q = pkg.synth.balanced_charges(n, 5)
+    grads = {}
+    for prec in ("fp64", "fp32"):
+        adj = pkg.Adjoint(n, T, dt=0.001, precision=prec, checkpoint_every=50, rebuild_every=10, nbins=50,
+                          rdf_rmax=2.5, rdf_h=0.1, volume=L ** 3)
+        adj.set_potential(1.1, 0.95)
+        _, target = adj.forward(x, v, q)
+        adj.set_potential(1.0, 1.0)
+        adj.forward(x, v, q, target)
+        g = adj.backward()
+        grads[prec] = [g["deps"], g["dsigma"], g["dq"][77], g["dq"][12345], g["dx0"][0, 0], g["dx0"][4321, 2]]
+        if prec == "fp64":
+            h = 1e-5
+
+            def loss(eps=1.0, sigma=1.0, qv=q, xv=x):
+                adj.set_potential(eps, sigma)
+                return adj.forward(xv, v, qv, target)[0]
+
+            fd = [(loss(eps=1 + h) - loss(eps=1 - h)) / (2 * h),
+                  (loss(sigma=1 + h) - loss(sigma=1 - h)) / (2 * h)]
+            for i in (77, 12345):
+                qp, qm = q.copy(), q.copy()
+                qp[i] += h
+                qm[i] -= h
+                fd.append((loss(qv=qp) - loss(qv=qm)) / (2 * h))
+            for i, c in ((0, 0), (4321, 2)):
+                xp, xm = x.copy(), x.copy()
+                xp[i, c] += h
+                xm[i, c] -= h
+                fd.append((loss(xv=xp) - loss(xv=xm)) / (2 * h))
+            for k, (a, f) in enumerate(zip(grads[prec], fd)):
+                assert abs(a - f) <= 1e-4 * abs(f), (k, a, f)
+        adj.close()
+    for k, (a, b) in enumerate(zip(grads["fp32"], grads["fp64"])):
+        assert abs(a - b) <= 2e-4 * abs(b), (k, a, b)

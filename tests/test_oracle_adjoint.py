@@ -80,3 +80,56 @@ def test_rdf_of_two_atoms_is_the_closed_form(O):
     K = np.where(np.abs(u) < 1, (1 - u * u) ** 3, 0.0)
     # the one step of dt = 1e-9 moves the atoms by ~1e-16
     assert np.allclose(g, 10.0 / (4 * np.pi * rb * rb * dr) * K, rtol=1e-9, atol=1e-12)
+
+
+def _cluster(n, seed, L=4.0):
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(n ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    x = g * (L / side) + rng.uniform(-0.1, 0.1, size=(n, 3))
+    q = rng.permutation(np.r_[np.ones(n // 2), -np.ones(n - n // 2)])
+    return x, q
+
+
+def test_shifted_force_law_is_minus_the_energy_gradient(O):
+    """The forward model's forces are -grad of U_sf(r) = U(r) - U(rc) - (r -
+    rc) U'(rc) (energy from the potentials' definitions, central differences
+    with h = 1e-6), and they vanish continuously at the cutoffs."""
+    x, q = _cluster(27, 5)
+    p = O.Adj64Params(eps=1.3, sigma=0.9, rc=2.5, k_coul=0.7, rc_coul=2.2, dt=0.001, rdf_rmax=2.0,
+                      rdf_h=0.1, volume=64.0, nbins=10, nsteps=1)
+    F, U = O.adj64_forces(x, q, p)
+    h = 1e-6
+    for (i, c) in ((0, 0), (4, 1), (13, 2), (26, 0)):
+        xp, xm = x.copy(), x.copy()
+        xp[i, c] += h
+        xm[i, c] -= h
+        fd = -(O.adj64_forces(xp, q, p)[1] - O.adj64_forces(xm, q, p)[1]) / (2 * h)
+        assert abs(F[i, c] - fd) < 1e-6 * max(1.0, abs(fd)), (i, c, F[i, c], fd)
+    # a pair just inside and just beyond each cutoff: force and energy -> 0
+    for r in (2.5 - 1e-7, 2.5 + 1e-7):
+        x2 = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        F2, U2 = O.adj64_forces(x2, np.array([1.0, -1.0]), p)
+        assert np.abs(F2).max() < 1e-6 and abs(U2) < 1e-12
+
+
+@pytest.mark.parametrize("rc", [3.0, 10.0, 100.0])
+def test_shifted_force_differs_from_textbook_by_the_cutoff_slope(O, rc):
+    """Against the textbook forces of oracle64 (pinned by closed forms): for
+    one pair, F_sf = F_textbook + U'(rc) r^ with U'_LJ(rc) = 4 eps (6 s^6 /
+    rc^7 - 12 s^12 / rc^13) and U'_C(rc) = -k q1 q2 / rc^2, so the two agree
+    to O(1/rc^2) in the large-cutoff limit."""
+    eps, sig, k = 1.3, 0.9, 0.7
+    q = np.array([1.0, -1.0])
+    pa = O.Adj64Params(eps=eps, sigma=sig, rc=rc, k_coul=k, rc_coul=rc, dt=0.001, rdf_rmax=2.0,
+                       rdf_h=0.1, volume=64.0, nbins=10, nsteps=1)
+    pt = O.o64_params(eps=eps, sigma=sig, rc_lj=rc, k_coul=k, rc_coul=rc)
+    for r in (0.95, 1.3, 2.2):
+        x = np.array([[0.2, -0.1, 0.3], [0.2 + r * 0.6, -0.1 + r * 0.8, 0.3]])
+        Fa, _ = O.adj64_forces(x, q, pa)
+        Ft, _, _, _ = O.forces64(x, q, pt)
+        rhat = (x[0] - x[1]) / r
+        slope = 4 * eps * (6 * sig ** 6 / rc ** 7 - 12 * sig ** 12 / rc ** 13) - k * q[0] * q[1] / rc ** 2
+        assert np.allclose(Fa[0], Ft[0] + slope * rhat, rtol=1e-11, atol=1e-13)
+        assert np.allclose(Fa[1], -Fa[0], rtol=1e-12, atol=1e-14)
+        assert np.abs(Fa[0] - Ft[0]).max() <= 1.01 * abs(slope)
