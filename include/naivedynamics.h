@@ -39,6 +39,7 @@ typedef enum {
 typedef enum { ND_PRED_SCALAR = 0, ND_PRED_SIMD = 1 } nd_predicate;
 typedef enum { ND_MORTON_MASKED = 0, ND_MORTON_INTERLEAVE = 1, ND_MORTON_HILBERT = 2 } nd_morton_mode;
 typedef enum { ND_PREC_FP32 = 0, ND_PREC_MIXED = 1, ND_PREC_FP64 = 2 } nd_precision;
+typedef enum { ND_QUERY_OWN_LEAF = 0, ND_QUERY_ROOT = 1 } nd_query_mode;
 
 /* GridKey{Float32,Int32} (BVHTraverse.jl:102-109), same 32-byte layout. */
 typedef struct {
@@ -176,6 +177,39 @@ int nd_leafbuild_traverse_bvh_host(nd_ctx *ctx, const float *h_xyz, const nd_bvh
  * (BVHTraverse.jl:596): nodes (2nL-1), sort permutation and sorted codes of
  * the last nd_leafbuild_traverse_bvh call, copied to host (any may be NULL). */
 int nd_tree_export(nd_ctx *ctx, nd_node *h_nodes, int32_t *h_perm, int32_t *h_codes_sorted);
+
+/* ---- point-query neighbour list -----------------------------------------
+ * neighbor_traverse (BVHTraverse.jl:1125-1234, with proximity_test! :1057-1087
+ * and overlap_test :1089-1092) for ND_QUERY_OWN_LEAF: query atom q (sorted
+ * order, all but the last) walks from its own leaf; a node is entered iff q
+ * lies strictly inside its box on every axis; a leaf keeps its atoms s after
+ * q with s_qs < fl(r*r).  ND_QUERY_ROOT: gpuneighbor_traverse!
+ * (ext/NaiveKA.jl:352-452), every atom walks from the root and keeps subjects
+ * whose original index exceeds its own.  Pairs (original query index,
+ * original subject index, sqrt(s_qs)) query-major in walk order (the order of
+ * a single-threaded reference run).  Capacity protocol as above. */
+int nd_neighbor_traverse(nd_ctx *ctx, const nd_node *d_nodes, const float *d_xyz_sorted,
+                         const int32_t *d_index_sorted, const nd_bvh_spec *spec, int32_t query_mode,
+                         nd_pair *d_out, int64_t cap, int64_t *count);
+/* build_traverse_bvh (BVHTraverse.jl:1416-1421) = TreeData (:500-543:
+ * codes, stable sort, cluster_primitives' +- r leaf boxes, the build, the
+ * root box overwritten) + neighbor_traverse, for ND_QUERY_OWN_LEAF; the tree
+ * and traversal of gpubvh_neighborlist (ext/NaiveKA.jl:470-558; root box
+ * kept) for ND_QUERY_ROOT.  The tree stays for nd_tree_export and
+ * nd_tree_store_export. */
+int nd_build_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec, int32_t query_mode,
+                          int32_t morton_mode, nd_pair *d_out, int64_t cap, int64_t *count);
+/* store of TreeData / gpubvh_neighborlist's treedata (BVHTraverse.jl:521-542,
+ * NaiveKA.jl:557): the nL-1 compare-and-swap slots of the last one-call
+ * build, as the build left them.  Slot k (split after leaf k+1, 1-based
+ * leaves) holds the range end parked by the child that arrived first: the
+ * left child's first leaf or the right child's last leaf (a one-thread
+ * reference build always leaves the former; this build's lanes race). */
+int nd_tree_store_export(nd_ctx *ctx, int32_t *h_store);
+/* quantized_positions! (BVHTraverse.jl:290-314): d_quantized (N x 3 int32)
+ * = 1-based rank of each atom in a stable sort by x, then by y over that
+ * order, then by z over that order (Julia isless: -0 before +0). */
+int nd_quantized_positions(nd_ctx *ctx, const float *d_xyz, int32_t n, int32_t *d_quantized);
 
 /* threshold_pairs(unique_pairs(x), threshold) (AllToAll.jl:65-88): all i<j
  * with sqrt(s) < threshold, in (i, j) lexicographic order.  O(N^2). */

@@ -10,11 +10,13 @@ The directory name is not a Python identifier; load the package with
 """
 from . import _lib, adjoint, batched, bvh, cells, forces, simulator, synth
 from ._lib import (MORTON_HILBERT, MORTON_INTERLEAVE, MORTON_MASKED, NODE_DTYPE, PAIR_DTYPE, PRED_SCALAR,
-                   PRED_SIMD, PREC_FP32, PREC_FP64, PREC_MIXED, CapacityError, Context,
-                   DeviceArray, DomainError, NDError, SpecError, default_context)
-from .bvh import (SpheresBVHSpecs, binwidth, bounding_volume_hierarchy, canonical,
-                  exptsimdbuild_traverse_bvh, gpubvh_neighborlist, leafbuild_traverse_bvh,
-                  leafneighbor_traverse, leafTreeData, mortoncodes, neighborlist_device,
+                   PRED_SIMD, PREC_FP32, PREC_FP64, PREC_MIXED, QUERY_OWN_LEAF, QUERY_ROOT,
+                   CapacityError, Context, DeviceArray, DomainError, NDError, SpecError,
+                   default_context)
+from .bvh import (SpheresBVHSpecs, TreeData, binwidth, bounding_volume_hierarchy,
+                  build_traverse_bvh, canonical, exptsimdbuild_traverse_bvh, gpubvh_neighborlist,
+                  leafbuild_traverse_bvh, leafneighbor_traverse, leafTreeData, mortoncodes,
+                  neighbor_traverse, neighborlist_device, quantized_positions,
                   simdbuild_traverse_bvh, sortperm, threshold_unique_pairs)
 from .cells import neighborlist as cell_neighborlist
 from .forces import force_coulomb_, force_lennardjones_, sum_forces_

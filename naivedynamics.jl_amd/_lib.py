@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libnaivehip.so")
 ND_OK, ND_ERR_ARG, ND_ERR_DOMAIN, ND_ERR_CAPACITY, ND_ERR_HIP, ND_ERR_STATE = 0, -1, -2, -3, -4, -5
 PRED_SCALAR, PRED_SIMD = 0, 1
 MORTON_MASKED, MORTON_INTERLEAVE, MORTON_HILBERT = 0, 1, 2
+QUERY_OWN_LEAF, QUERY_ROOT = 0, 1
 PREC_FP32, PREC_MIXED, PREC_FP64 = 0, 1, 2
 
 NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,)), ("left", "<i4"),
@@ -117,6 +118,12 @@ SIGNATURES = {
     "nd_leafbuild_traverse_bvh_host": [_P, _P, ctypes.POINTER(BvhSpec), _i32, _i32, _P, _i64,
                                        ctypes.POINTER(_i64)],
     "nd_tree_export": [_P, _P, _P, _P],
+    "nd_neighbor_traverse": [_P, _P, _P, _P, ctypes.POINTER(BvhSpec), _i32, _P, _i64,
+                             ctypes.POINTER(_i64)],
+    "nd_build_traverse_bvh": [_P, _P, ctypes.POINTER(BvhSpec), _i32, _i32, _P, _i64,
+                              ctypes.POINTER(_i64)],
+    "nd_tree_store_export": [_P, _P],
+    "nd_quantized_positions": [_P, _P, _i32, _P],
     "nd_unique_threshold_pairs": [_P, _P, _i32, _f32, _P, _i64, ctypes.POINTER(_i64)],
     "nd_cell_neighborlist": [_P, _P, _i32, _f32, _i32, _P, _i64, ctypes.POINTER(_i64)],
     "nd_cell_neighborlist_host": [_P, _P, _i32, _f32, _i32, _P, _i64, ctypes.POINTER(_i64)],

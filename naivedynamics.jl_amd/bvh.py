@@ -13,7 +13,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (MORTON_INTERLEAVE, MORTON_MASKED, NODE_DTYPE, PAIR_DTYPE, PRED_SCALAR,
-                   PRED_SIMD, BvhSpec, CapacityError, DeviceArray, SpecError, check, load, ptr)
+                   PRED_SIMD, QUERY_OWN_LEAF, QUERY_ROOT, BvhSpec, CapacityError, DeviceArray,
+                   SpecError, check, load, ptr)
 
 binwidth = np.float32(1.0 / 1023.0)  # BVHTraverse.jl:184
 
@@ -200,19 +201,94 @@ def tree_export(spec, position, ctx=None):
     return nodes, prim, perm
 
 
+def _pairs_with_capacity(call, what, ctx, cap):
+    """Runs call(out_ptr, cap, count_ref) under the capacity protocol."""
+    out = _pairbuf.get(ctx, cap)
+    count = ctypes.c_int64(0)
+    rc = call(out.ptr, out.shape[0], ctypes.byref(count))
+    if rc == _lib.ND_ERR_CAPACITY:
+        out = _pairbuf.get(ctx, count.value + count.value // 8 + 1024)
+        rc = call(out.ptr, out.shape[0], ctypes.byref(count))
+    check(rc, ctx, what, count.value)
+    return out.to_host(count.value)
+
+
+def build_traverse_bvh(position, spec, morton=MORTON_MASKED, query=QUERY_OWN_LEAF, ctx=None):
+    """build_traverse_bvh(position, spec) (BVHTraverse.jl:1416-1421): TreeData's
+    +-r tree, then neighbor_traverse's point queries from each atom's own leaf
+    (query=QUERY_ROOT: the root-started walk of gpuneighbor_traverse!).  Pairs
+    (query, subject, d) with 0-based original indices, query-major."""
+    ctx = _ctx(ctx)
+    d, _ = _dev_xyz(ctx, position)
+    if d.shape[0] != spec.atom_count:
+        raise SpecError(_lib.ND_ERR_ARG, "position length does not match atom_count")
+    L = load()
+    return _pairs_with_capacity(
+        lambda o, cap, cnt: L.nd_build_traverse_bvh(ctx.handle, d.ptr, ctypes.byref(spec.c), query, morton,
+                                                    o, cap, cnt),
+        "build_traverse_bvh", ctx, max(1024, 48 * spec.atom_count))
+
+
+def tree_store(spec, ctx=None):
+    """store of the last one-call build: nL-1 int32 CAS slots (1-based leaves)."""
+    ctx = _ctx(ctx)
+    st = np.zeros(spec.leaves_count - 1, np.int32)
+    check(load().nd_tree_store_export(ctx.handle, ptr(st)), ctx, "tree_store")
+    return st
+
+
+def TreeData(position, spec, morton=MORTON_MASKED, ctx=None):
+    """TreeData(position, spec) (BVHTraverse.jl:500-543) -> (keys, (index,
+    morton_code, position) sorted, mortsort, store).  Built on the device by the
+    one-call point-query list, whose pairs are dropped."""
+    ctx = _ctx(ctx)
+    build_traverse_bvh(position, spec, morton, QUERY_OWN_LEAF, ctx)
+    nodes, prim, perm = tree_export(spec, position, ctx)
+    return nodes, prim, perm, tree_store(spec, ctx)
+
+
+def neighbor_traverse(keys, positions, spec, query=QUERY_OWN_LEAF, ctx=None):
+    """neighbor_traverse(keys, positions, spec) (BVHTraverse.jl:1125-1234) over
+    a given tree; positions is TreeData's sorted primitive dict (index,
+    position)."""
+    ctx = _ctx(ctx)
+    nodes = DeviceArray.from_host(ctx, np.ascontiguousarray(keys, NODE_DTYPE))
+    x = DeviceArray.from_host(ctx, np.ascontiguousarray(positions["position"], np.float32).reshape(-1, 3))
+    ix = DeviceArray.from_host(ctx, np.ascontiguousarray(positions["index"], np.int32))
+    L = load()
+    return _pairs_with_capacity(
+        lambda o, cap, cnt: L.nd_neighbor_traverse(ctx.handle, nodes.ptr, x.ptr, ix.ptr, ctypes.byref(spec.c),
+                                                   query, o, cap, cnt),
+        "neighbor_traverse", ctx, max(1024, 48 * spec.atom_count))
+
+
+def quantized_positions(position, ctx=None):
+    """quantized_positions! (BVHTraverse.jl:290-314) -> (N, 3) int32: the
+    1-based rank of each atom in the stable sorts by x, then y, then z."""
+    ctx = _ctx(ctx)
+    d, _ = _dev_xyz(ctx, position)
+    n = d.shape[0]
+    q = DeviceArray(ctx, (n, 3), np.int32)
+    check(load().nd_quantized_positions(ctx.handle, d.ptr, n, q.ptr), ctx, "quantized_positions!")
+    return q.to_host()
+
+
 def gpubvh_neighborlist(backend, position, spec, morton=MORTON_MASKED, ctx=None):
     """gpubvh_neighborlist(backend, position, spec) (ext/NaiveKA.jl:470-558): returns
     {pairlist, treedata: {tree, mortonpositions, quantizedposition, store}}.
+    The pairs are the root-started point queries of gpuneighbor_traverse!
+    (:352-452) over the +-r tree (root box kept); quantizedposition is
+    quantized_positions! of the input; store holds the build's CAS slots.
     `backend` is accepted for signature parity and must be None or "hip"."""
     if backend not in (None, "hip", "rocm", "gpu"):
         raise SpecError(_lib.ND_ERR_ARG, "only the HIP backend is available")
     ctx = _ctx(ctx)
-    out, n = neighborlist_device(position, spec, PRED_SCALAR, morton, ctx)
-    pairs = out.to_host(n)
-    nodes, prim, perm = tree_export(spec, position, ctx)
+    quantized = quantized_positions(position, ctx)
+    pairs = build_traverse_bvh(position, spec, morton, QUERY_ROOT, ctx)
+    nodes, prim, _ = tree_export(spec, position, ctx)
     return {"pairlist": pairs,
-            "treedata": {"tree": nodes, "mortonpositions": prim, "quantizedposition": None,
-                         "store": None}}
+            "treedata": {"tree": nodes, "mortonpositions": prim, "quantizedposition": quantized,
+                         "store": tree_store(spec, ctx)}}
 
 
 def threshold_unique_pairs(position, threshold, ctx=None):
@@ -246,6 +322,7 @@ def canonical(pairs):
 
 __all__ = ["SpheresBVHSpecs", "binwidth", "mortoncodes", "sortperm", "bounding_volume_hierarchy",
            "leafneighbor_traverse", "leafbuild_traverse_bvh", "simdbuild_traverse_bvh",
-           "exptsimdbuild_traverse_bvh", "leafTreeData", "gpubvh_neighborlist",
+           "exptsimdbuild_traverse_bvh", "leafTreeData", "gpubvh_neighborlist", "TreeData",
+           "build_traverse_bvh", "neighbor_traverse", "quantized_positions", "tree_store",
            "threshold_unique_pairs", "neighborlist_device", "canonical", "tree_export",
            "CapacityError"]

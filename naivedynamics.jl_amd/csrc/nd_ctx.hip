@@ -63,7 +63,8 @@ int nd_ctx_destroy(nd_ctx *ctx) {
                           &ctx->offsets, &ctx->xyz_in, &ctx->pairs_out, &ctx->bbox,
                           &ctx->sort.keys_alt, &ctx->sort.vals_alt, &ctx->sort.hist,
                           &ctx->sort.flags, &ctx->cell_keys, &ctx->cell_vals, &ctx->cell_pos,
-                          &ctx->cell_start, &ctx->trav_rec, &ctx->scan_tmp,
+                          &ctx->cell_start, &ctx->trav_rec, &ctx->scan_tmp, &ctx->stage_store,
+                          &ctx->rank_keys,
                           &ctx->force_acc, &ctx->force_part, &ctx->async_ws};
     for (auto *b : bufs) b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

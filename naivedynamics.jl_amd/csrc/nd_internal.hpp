@@ -75,12 +75,15 @@ struct nd_ctx {
     nd::DevBuf scan_tmp;  // tile sums of scan_counts
     nd::DevBuf force_acc, force_part;  // pair-list forces: fixed-point sums, energy partials
     nd::DevBuf async_ws;  // work buffers of nd_leafbuild_traverse_bvh_async (resized by it alone)
+    nd::DevBuf stage_store;  // CAS slots etc. of the stage-level nd_bounding_volume_hierarchy
+    nd::DevBuf rank_keys;    // nd_quantized_positions: keys and atom order (2N int32)
     nd::SortBufs sort;
     // last tree built by a one-call neighbour list (for nd_tree_export)
     int32_t tree_n = 0, tree_apl = 0;
     bool has_tree = false;
     const nd_node *tree_nodes = nullptr;
     const int32_t *tree_perm = nullptr, *tree_codes = nullptr;
+    const int32_t *tree_store = nullptr;  // the build's CAS slots (nL - 1)
 };
 
 namespace nd {
@@ -161,6 +164,10 @@ int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted
 // Exclusive scan of ctx->counts (n int64) into ctx->offsets (n + 1): launch only,
 // or launch, synchronise and return the total.
 void launch_scan(nd_ctx *ctx, int64_t n);
+// Point-query walk (query_mode ND_QUERY_OWN_LEAF / ND_QUERY_ROOT) with the
+// counting/scan/emit protocol of leaf_traverse; synchronous.
+int64_t point_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted, const int32_t *idx_sorted,
+                       int32_t n, int32_t apl, float r, int32_t mode, nd_pair *out, int64_t cap);
 int64_t scan_counts(nd_ctx *ctx, int64_t n);
 
 // Work buffers of the stream-ordered one-call neighbour list, carved from one

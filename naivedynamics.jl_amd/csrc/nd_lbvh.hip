@@ -416,9 +416,10 @@ extern "C" int nd_bounding_volume_hierarchy(nd_ctx *ctx, const float *d_xyz_sort
                "nd_bounding_volume_hierarchy: invalid SpheresBVHSpecs");
     ND_HIP(hipSetDevice(ctx->device));
     const int32_t nL = chk.leaves_count;
-    ctx->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(nL));
+    // a buffer of its own: the one-call lists' store stays valid for export
+    ctx->stage_store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(nL));
     nd::launch_lbvh(ctx->stream, d_xyz_sorted, 3, d_codes_sorted, nL, chk.atomsperleaf,
-                    chk.neighbor_distance, override_root, d_nodes, ctx->store.as<int32_t>(),
+                    chk.neighbor_distance, override_root, d_nodes, ctx->stage_store.as<int32_t>(),
                     ctx->status.as<int32_t>());
     if (nd::read_status(ctx) & nd::kStatusTree)
         throw nd::Error(ND_ERR_DOMAIN, "bounding_volume_hierarchy!: codes outside the range the "

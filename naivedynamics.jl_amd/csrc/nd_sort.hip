@@ -236,6 +236,28 @@ __global__ void __launch_bounds__(nd::kBlock) k_iota(int32_t *p, int32_t n) {
     if (i < n) p[i] = i;
 }
 
+// quantized_positions! (BVHTraverse.jl:290-314), one axis: the coordinate of
+// atom order[i] (i itself on the first axis) as a signed key in Julia's
+// isless order on Float32 (-0 before +0, every NaN after +Inf and tied).
+__global__ void __launch_bounds__(nd::kBlock)
+k_rank_keys(const float *__restrict__ xyz, int32_t n, int32_t axis, int32_t first, int32_t *__restrict__ keys,
+            int32_t *__restrict__ order) {
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int32_t a = first ? i : order[i];
+    const float v = xyz[3 * (int64_t)a + axis];
+    const int32_t b = __float_as_int(v);
+    keys[i] = (v != v) ? INT32_MAX : (b < 0 ? (b ^ 0x7FFFFFFF) : b);
+    order[i] = a;
+}
+
+// q[atom][axis] = 1-based place of the atom in the axis' stable sort
+__global__ void __launch_bounds__(nd::kBlock)
+k_rank_scatter(const int32_t *__restrict__ order, int32_t n, int32_t axis, int32_t *__restrict__ q) {
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) q[3 * (int64_t)order[i] + axis] = i + 1;
+}
+
 }  // namespace
 
 namespace nd {
@@ -327,6 +349,25 @@ extern "C" int nd_sortperm(nd_ctx *ctx, const int32_t *d_codes, int32_t n,
                               hipMemcpyDeviceToDevice, ctx->stream));
     nd::launch_iota(ctx->stream, d_perm, n);
     nd::radix_sort_pairs_auto(ctx, d_codes_sorted, d_perm, n);
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    ND_API_END(ctx)
+}
+
+extern "C" int nd_quantized_positions(nd_ctx *ctx, const float *d_xyz, int32_t n, int32_t *d_quantized) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && n >= 0 && (n == 0 || (d_xyz && d_quantized)), "nd_quantized_positions: bad argument");
+    ND_HIP(hipSetDevice(ctx->device));
+    if (n == 0) return ND_OK;
+    ctx->rank_keys.reserve(2 * sizeof(int32_t) * (size_t)n);
+    int32_t *keys = ctx->rank_keys.as<int32_t>(), *order = keys + n;
+    // x, then y and z, each a stable sort of the order the previous axis left
+    for (int32_t axis = 0; axis < 3; ++axis) {
+        k_rank_keys<<<nd::grid_for(n), nd::kBlock, 0, ctx->stream>>>(d_xyz, n, axis, axis == 0, keys, order);
+        ND_HIP(hipGetLastError());
+        nd::radix_sort_pairs_auto(ctx, keys, order, n);
+        k_rank_scatter<<<nd::grid_for(n), nd::kBlock, 0, ctx->stream>>>(order, n, axis, d_quantized);
+        ND_HIP(hipGetLastError());
+    }
     ND_HIP(hipStreamSynchronize(ctx->stream));
     ND_API_END(ctx)
 }

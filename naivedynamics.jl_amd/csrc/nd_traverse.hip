@@ -10,6 +10,7 @@
 // query leaf; a counting pass, an exclusive scan and an emitting pass place
 // every leaf's pairs at a fixed offset, so the output is in exactly the order
 // of a single-threaded reference run.
+#include <cfloat>
 #include <mutex>
 
 #include "nd_internal.hpp"
@@ -179,6 +180,69 @@ k_leaf_traverse4(const nd_node *__restrict__ nodes, const float *__restrict__ xy
         }
     }
     if (!EMIT) counts[q - 1] = sink.n;
+}
+
+// ---- point-query walks ----
+//
+// neighbor_traverse (BVHTraverse.jl:1125-1234; proximity_test! :1057-1087,
+// overlap_test :1089-1092): one lane per query atom q (sorted, q < N-1),
+// starting at q's own leaf; a node is entered iff the point lies strictly
+// inside its box on all three axes; a leaf tests its atoms s > q with
+// s_qs < fl(r*r) and follows its rope.  ROOT: gpuneighbor_traverse!
+// (ext/NaiveKA.jl:352-452), every atom queries from the root and keeps the
+// subjects whose original index exceeds its own.  Counting pass, exclusive
+// scan over queries, emitting pass: pairs land query-major in walk order,
+// the order of a single-threaded reference run.
+template <bool EMIT, bool ROOT>
+__global__ void __launch_bounds__(nd::kBlock)
+k_point_traverse(const nd_node *__restrict__ nodes, const float *__restrict__ xyz,
+                 const int32_t *__restrict__ idx, int32_t n, int32_t nL, int32_t apl, float r2,
+                 int64_t *__restrict__ counts, const int64_t *__restrict__ offsets,
+                 nd_pair *__restrict__ out, int32_t *__restrict__ status, int64_t cap) {
+    const int32_t nq = ROOT ? n : n - 1;
+    const int32_t q = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= nq) return;
+    if (EMIT && offsets[nq] > cap) return;  // too small a buffer: write nothing
+    Sink<EMIT> sink;
+    if (EMIT) {
+        sink.out = out + offsets[q];
+        sink.limit = offsets[q + 1] - offsets[q];
+    }
+    const float px = xyz[3 * (int64_t)q], py = xyz[3 * (int64_t)q + 1], pz = xyz[3 * (int64_t)q + 2];
+    const int32_t qi = (ROOT || EMIT) ? idx[q] : 0;
+    int32_t t = ROOT ? nL + 1 : q / apl + 1;
+    // each node at most once: more visits or an index outside [1, 2nL-1]
+    // means a malformed tree
+    int32_t visits = 0;
+    while (t != 0) {
+        if (++visits > 2 * nL || t < 1 || t > 2 * nL - 1) {
+            atomicOr(status, nd::kStatusTree);
+            break;
+        }
+        const Box b = load_node(nodes, t);
+        const bool inside = (b.mn[0] < px) & (px < b.mx[0]) & (b.mn[1] < py) & (py < b.mx[1]) &
+                            (b.mn[2] < pz) & (pz < b.mx[2]);
+        if (!inside) {
+            t = b.skip;
+            continue;
+        }
+        if (b.left != 0) {
+            t = b.left;
+            continue;
+        }
+        const int64_t lo = (int64_t)(t - 1) * apl;
+        // own-leaf form: subjects after the query only
+        const int32_t e0 = ROOT ? 0 : (int32_t)max((int64_t)0, (int64_t)q + 1 - lo);
+        for (int32_t e = e0; e < apl; ++e) {
+            const int64_t s = lo + e;
+            const int32_t si = (ROOT || EMIT) ? idx[s] : 0;
+            if (ROOT && !(qi < si)) continue;
+            const float d2 = sqdist(px, py, pz, xyz[3 * s], xyz[3 * s + 1], xyz[3 * s + 2]);
+            if (d2 < r2) sink.push(qi, si, sqrtf(d2));
+        }
+        t = b.skip;
+    }
+    if (!EMIT) counts[q] = sink.n;
 }
 
 // ---- wave-cooperative form (trees built by this library) ----
@@ -706,6 +770,46 @@ int64_t leaf_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted
     return total;
 }
 
+int64_t point_traverse(nd_ctx *ctx, const nd_node *nodes, const float *xyz_sorted, const int32_t *idx_sorted,
+                       int32_t n, int32_t apl, float r, int32_t mode, nd_pair *out, int64_t cap) {
+    const int32_t nL = n / apl;
+    const float r2 = r * r;  // (neighbor_distance)^2 in Float32
+    const bool root = mode == ND_QUERY_ROOT;
+    const int64_t nq = root ? n : n - 1;
+    ctx->counts.reserve(sizeof(int64_t) * (size_t)(nq + 1));
+    ctx->offsets.reserve(sizeof(int64_t) * (size_t)(nq + 1));
+    int64_t *cnt = ctx->counts.as<int64_t>();
+    const int64_t *off = ctx->offsets.as<int64_t>();
+    int32_t *st = ctx->status.as<int32_t>();
+    auto pass = [&](bool emit) {
+        const unsigned g = grid_for(nq);
+        if (root) {
+            if (emit)
+                k_point_traverse<true, true><<<g, kBlock, 0, ctx->stream>>>(nodes, xyz_sorted, idx_sorted, n, nL,
+                                                                           apl, r2, cnt, off, out, st, cap);
+            else
+                k_point_traverse<false, true><<<g, kBlock, 0, ctx->stream>>>(nodes, xyz_sorted, idx_sorted, n, nL,
+                                                                            apl, r2, cnt, off, nullptr, st, cap);
+        } else {
+            if (emit)
+                k_point_traverse<true, false><<<g, kBlock, 0, ctx->stream>>>(nodes, xyz_sorted, idx_sorted, n, nL,
+                                                                            apl, r2, cnt, off, out, st, cap);
+            else
+                k_point_traverse<false, false><<<g, kBlock, 0, ctx->stream>>>(nodes, xyz_sorted, idx_sorted, n, nL,
+                                                                             apl, r2, cnt, off, nullptr, st, cap);
+        }
+        ND_HIP(hipGetLastError());
+    };
+    pass(false);
+    const int64_t total = scan_counts(ctx, nq);
+    if (nd::read_status(ctx) & nd::kStatusTree)
+        throw nd::Error(ND_ERR_STATE, "neighbor_traverse: malformed tree (rope cycle or bad index)");
+    if (total > cap || out == nullptr) return total;
+    pass(true);
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    return total;
+}
+
 namespace {
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -866,6 +970,10 @@ void check_pred(int32_t pred) {
     ND_REQUIRE(pred == ND_PRED_SCALAR || pred == ND_PRED_SIMD, "unknown pair predicate");
 }
 
+void check_query(int32_t mode) {
+    ND_REQUIRE(mode == ND_QUERY_OWN_LEAF || mode == ND_QUERY_ROOT, "unknown query mode");
+}
+
 }  // namespace
 
 extern "C" {
@@ -927,6 +1035,7 @@ int nd_leafbuild_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec
     ctx->tree_nodes = ctx->nodes.as<nd_node>();
     ctx->tree_perm = ctx->perm.as<int32_t>();
     ctx->tree_codes = ctx->codes_sorted.as<int32_t>();
+    ctx->tree_store = ctx->store.as<int32_t>();
     const int64_t total = nd::leaf_traverse(ctx, ctx->nodes.as<nd_node>(), ctx->xyz_sorted.as<float>(),
                                             ctx->perm.as<int32_t>(), n, apl, chk.neighbor_distance,
                                             predicate, d_out, cap, true);
@@ -959,6 +1068,7 @@ int nd_leafbuild_traverse_bvh_async(nd_ctx *ctx, const float *d_xyz, const nd_bv
     ctx->tree_nodes = w.nodes;
     ctx->tree_perm = w.perm;
     ctx->tree_codes = w.codes;
+    ctx->tree_store = w.store;
     ND_API_END(ctx)
 }
 
@@ -988,6 +1098,7 @@ int nd_leafbuild_traverse_bvh_noalloc(nd_ctx *ctx, const float *d_xyz, const nd_
     ctx->tree_nodes = w.nodes;
     ctx->tree_perm = w.perm;
     ctx->tree_codes = w.codes;
+    ctx->tree_store = w.store;
     ND_API_END(ctx)
 }
 
@@ -1028,6 +1139,82 @@ int nd_tree_export(nd_ctx *ctx, nd_node *h_nodes, int32_t *h_perm, int32_t *h_co
     if (h_codes_sorted)
         ND_HIP(hipMemcpyAsync(h_codes_sorted, ctx->tree_codes, sizeof(int32_t) * n,
                               hipMemcpyDeviceToHost, ctx->stream));
+    ND_HIP(hipStreamSynchronize(ctx->stream));
+    ND_API_END(ctx)
+}
+
+int nd_neighbor_traverse(nd_ctx *ctx, const nd_node *d_nodes, const float *d_xyz_sorted,
+                         const int32_t *d_index_sorted, const nd_bvh_spec *spec, int32_t query_mode,
+                         nd_pair *d_out, int64_t cap, int64_t *count) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && d_nodes && d_xyz_sorted && d_index_sorted && count, "nd_neighbor_traverse: null argument");
+    nd_bvh_spec chk;
+    check_spec(spec, chk);
+    check_query(query_mode);
+    ND_HIP(hipSetDevice(ctx->device));
+    const int64_t total = nd::point_traverse(ctx, d_nodes, d_xyz_sorted, d_index_sorted, chk.atom_count,
+                                             chk.atomsperleaf, chk.neighbor_distance, query_mode, d_out, cap);
+    *count = total;
+    if (total > cap) throw nd::Error(ND_ERR_CAPACITY, "pair buffer too small; *count = required");
+    ND_API_END(ctx)
+}
+
+int nd_build_traverse_bvh(nd_ctx *ctx, const float *d_xyz, const nd_bvh_spec *spec, int32_t query_mode,
+                          int32_t morton_mode, nd_pair *d_out, int64_t cap, int64_t *count) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && d_xyz && count, "nd_build_traverse_bvh: null argument");
+    nd_bvh_spec chk;
+    check_spec(spec, chk);
+    check_query(query_mode);
+    ND_REQUIRE(morton_mode == ND_MORTON_MASKED || morton_mode == ND_MORTON_INTERLEAVE, "unknown key mode");
+    // the build halves its radius argument for the leaf half-width; 2r is
+    // exact in Float32 below this bound, so the leaves get +- r exactly
+    ND_REQUIRE(chk.neighbor_distance <= 0.5f * FLT_MAX, "nd_build_traverse_bvh: neighbor_distance too large");
+    ND_HIP(hipSetDevice(ctx->device));
+    const int32_t n = chk.atom_count, apl = chk.atomsperleaf, nL = chk.leaves_count;
+    ctx->has_tree = false;
+    ctx->codes_sorted.reserve(sizeof(int32_t) * (size_t)n);
+    ctx->perm.reserve(sizeof(int32_t) * (size_t)n);
+    ctx->xyz_sorted.reserve(sizeof(float) * 3 * (size_t)n);
+    ctx->nodes.reserve(sizeof(nd_node) * (size_t)(2 * nL - 1));
+    ctx->store.reserve(sizeof(int32_t) * nd::lbvh_scratch_ints(nL));
+    // mortoncodes! + sortperm + permute! (BVHTraverse.jl:511-516)
+    nd::launch_morton(ctx, d_xyz, n, morton_mode, nullptr, nullptr, ctx->codes_sorted.as<int32_t>(),
+                      ctx->perm.as<int32_t>(), ctx->status.as<int32_t>());
+    if (nd::read_status(ctx) & nd::kStatusDomain)
+        throw nd::Error(ND_ERR_DOMAIN,
+                        "mortoncodes!: coordinate outside the Int32 quantisation domain "
+                        "(reference throws InexactError)");
+    nd::radix_sort_pairs_auto(ctx, ctx->codes_sorted.as<int32_t>(), ctx->perm.as<int32_t>(), n);
+    nd::launch_gather_xyz(ctx->stream, d_xyz, ctx->perm.as<int32_t>(), n, ctx->xyz_sorted.as<float>());
+    // cluster_primitives (+- r leaves) + bounding_volume_hierarchy!; TreeData
+    // overwrites the root box (BVHTraverse.jl:856-858), the root-started form's
+    // build kernel keeps it (NaiveKA.jl:301-302)
+    nd::launch_lbvh(ctx->stream, ctx->xyz_sorted.as<float>(), 3, ctx->codes_sorted.as<int32_t>(), nL, apl,
+                    2.0f * chk.neighbor_distance, query_mode == ND_QUERY_OWN_LEAF ? 1 : 0,
+                    ctx->nodes.as<nd_node>(), ctx->store.as<int32_t>(), ctx->status.as<int32_t>());
+    ctx->has_tree = true;
+    ctx->tree_n = n;
+    ctx->tree_apl = apl;
+    ctx->tree_nodes = ctx->nodes.as<nd_node>();
+    ctx->tree_perm = ctx->perm.as<int32_t>();
+    ctx->tree_codes = ctx->codes_sorted.as<int32_t>();
+    ctx->tree_store = ctx->store.as<int32_t>();
+    const int64_t total = nd::point_traverse(ctx, ctx->nodes.as<nd_node>(), ctx->xyz_sorted.as<float>(),
+                                             ctx->perm.as<int32_t>(), n, apl, chk.neighbor_distance, query_mode,
+                                             d_out, cap);
+    *count = total;
+    if (total > cap) throw nd::Error(ND_ERR_CAPACITY, "pair buffer too small; *count = required");
+    ND_API_END(ctx)
+}
+
+int nd_tree_store_export(nd_ctx *ctx, int32_t *h_store) {
+    ND_API_BEGIN
+    ND_REQUIRE(ctx && h_store, "nd_tree_store_export: null argument");
+    if (!ctx->has_tree) throw nd::Error(ND_ERR_STATE, "nd_tree_store_export: no tree has been built");
+    const size_t nL = (size_t)ctx->tree_n / (size_t)ctx->tree_apl;
+    ND_HIP(hipMemcpyAsync(h_store, ctx->tree_store, sizeof(int32_t) * (nL - 1), hipMemcpyDeviceToHost,
+                          ctx->stream));
     ND_HIP(hipStreamSynchronize(ctx->stream));
     ND_API_END(ctx)
 }

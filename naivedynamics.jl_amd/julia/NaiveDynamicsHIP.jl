@@ -103,19 +103,96 @@ function NaiveDynamics.simdbuild_traverse_bvh(b::HIPBackend, position, spec)
     return pairlist(b, position, spec, Int32(1))
 end
 
-# gpubvh_neighborlist(backend, position, spec) (ext/NaiveKA.jl:470-558)
-function NaiveDynamics.gpubvh_neighborlist(b::HIPBackend, position, spec)
-    pairs = pairlist(b, position, spec, Int32(0))
+# device staging for the entry points that take device buffers
+function devbuf(b::HIPBackend, bytes::Integer)
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(b, ccall((:nd_malloc, libnaivehip), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}),
+                   b.handle, max(bytes, 1), p), "nd_malloc")
+    return p[]
+end
+devfree(b::HIPBackend, p) = ccall((:nd_free, libnaivehip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), b.handle, p)
+h2d(b::HIPBackend, d, h::Array) = check(b, ccall((:nd_memcpy_h2d, libnaivehip), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Csize_t), b.handle, d, h, sizeof(h)), "nd_memcpy_h2d")
+d2h(b::HIPBackend, h::Array, d) = check(b, ccall((:nd_memcpy_d2h, libnaivehip), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Csize_t), b.handle, h, d, sizeof(h)), "nd_memcpy_d2h")
+
+const ND_QUERY_OWN_LEAF = Int32(0)
+const ND_QUERY_ROOT = Int32(1)
+
+# point-query list (nd_build_traverse_bvh) on a staged copy of position
+function pointquery(b::HIPBackend, dxyz, spec::SpheresBVHSpecs{Float32,Int32}, query::Int32)
+    s = Ref(ndspec(spec))
+    cap = Int64(48) * spec.atom_count
+    while true
+        dout = devbuf(b, 12cap)
+        count = Ref{Int64}(0)
+        rc = ccall((:nd_build_traverse_bvh, libnaivehip), Cint,
+                   (Ptr{Cvoid}, Ptr{Cvoid}, Ref{NDSpec}, Int32, Int32, Ptr{Cvoid}, Int64, Ref{Int64}),
+                   b.handle, dxyz, s, query, Int32(0), dout, cap, count)
+        if rc == ND_ERR_CAPACITY
+            devfree(b, dout)
+            cap = count[] + count[] ÷ 8 + 1024
+            continue
+        end
+        check(b, rc, "build_traverse_bvh")
+        out = Vector{NTuple{3,UInt32}}(undef, count[])
+        count[] > 0 && d2h(b, out, dout)
+        devfree(b, dout)
+        return [(reinterpret(Int32, t[1]) + Int32(1), reinterpret(Int32, t[2]) + Int32(1),
+                 reinterpret(Float32, t[3])) for t in out]
+    end
+end
+
+# tree, sorted primitives and store of the last one-call build (1-based)
+function exporttree(b::HIPBackend, position, spec)
     nL = spec.leaves_count
     tree = Vector{NDNode}(undef, 2nL - 1)
     perm = Vector{Int32}(undef, spec.atom_count)
     codes = Vector{Int32}(undef, spec.atom_count)
-    rc = ccall((:nd_tree_export, libnaivehip), Cint, (Ptr{Cvoid}, Ptr{NDNode}, Ptr{Int32}, Ptr{Int32}),
-               b.handle, tree, perm, codes)
-    check(b, rc, "gpubvh_neighborlist")
+    store = Vector{Int32}(undef, nL - 1)
+    check(b, ccall((:nd_tree_export, libnaivehip), Cint, (Ptr{Cvoid}, Ptr{NDNode}, Ptr{Int32}, Ptr{Int32}),
+                   b.handle, tree, perm, codes), "tree export")
+    check(b, ccall((:nd_tree_store_export, libnaivehip), Cint, (Ptr{Cvoid}, Ptr{Int32}), b.handle, store),
+          "store export")
     keys = [GridKey{Float32,Int32}(n.min, n.max, n.left, n.skip) for n in tree]
     prim = APointPrimitive{Float32,Int32}(perm .+ Int32(1), codes, [SVector{3,Float32}(position[p + 1]) for p in perm])
-    return (pairlist = pairs, treedata = (tree = keys, mortonpositions = prim, quantizedposition = nothing, store = nothing))
+    return keys, prim, perm .+ Int32(1), store
+end
+
+# build_traverse_bvh(position, spec) (BVHTraverse.jl:1416-1421): TreeData + neighbor_traverse
+function NaiveDynamics.build_traverse_bvh(b::HIPBackend, position, spec)
+    xyz = packxyz(position)
+    dxyz = devbuf(b, sizeof(xyz))
+    try
+        h2d(b, dxyz, xyz)
+        return pointquery(b, dxyz, spec, ND_QUERY_OWN_LEAF)
+    finally
+        devfree(b, dxyz)
+    end
+end
+
+# gpubvh_neighborlist(backend, position, spec) (ext/NaiveKA.jl:470-558): root-started
+# point queries over the +-r tree, with the whole treedata
+function NaiveDynamics.gpubvh_neighborlist(b::HIPBackend, position, spec)
+    xyz = packxyz(position)
+    n = size(xyz, 2)
+    dxyz = devbuf(b, sizeof(xyz))
+    dq = devbuf(b, 12n)
+    try
+        h2d(b, dxyz, xyz)
+        check(b, ccall((:nd_quantized_positions, libnaivehip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+                       b.handle, dxyz, Int32(n), dq), "quantized_positions!")
+        q = Matrix{Int32}(undef, 3, n)
+        d2h(b, q, dq)
+        pairs = pointquery(b, dxyz, spec, ND_QUERY_ROOT)
+        keys, prim, _, store = exporttree(b, position, spec)
+        quantized = [SVector{3,Int32}(q[1, i], q[2, i], q[3, i]) for i in 1:n]
+        return (pairlist = pairs, treedata = (tree = keys, mortonpositions = prim,
+                                              quantizedposition = quantized, store = store))
+    finally
+        devfree(b, dxyz)
+        devfree(b, dq)
+    end
 end
 
 # neighborlist(x, cutoff) of CellListMap (assets/BVHBenchSuite.jl:307, src/MDInput.jl:290):

@@ -19,6 +19,7 @@ PAIR_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("d", "<f4")])
 
 PRED_SCALAR, PRED_SIMD = 0, 1
 MORTON_MASKED, MORTON_INTERLEAVE, MORTON_HILBERT = 0, 1, 2
+QUERY_OWN_LEAF, QUERY_ROOT = 0, 1
 
 
 class Adj64Params(ctypes.Structure):
@@ -86,6 +87,16 @@ def lib():
         L.ref_leafbuild_traverse_bvh.restype = i64
         L.ref_bruteforce.argtypes = [P, i32, f32, P, i64]
         L.ref_bruteforce.restype = i64
+        L.ref_cluster_boxes.argtypes = [P, i32, i32, f32, P]
+        L.ref_cluster_boxes.restype = None
+        L.ref_neighbor_traverse.argtypes = [P, P, P, i32, i32, f32, ctypes.c_int, P, i64,
+                                            ctypes.c_int]
+        L.ref_neighbor_traverse.restype = i64
+        L.ref_build_traverse_bvh.argtypes = [P, i32, i32, f32, ctypes.c_int, ctypes.c_int, P, P,
+                                             P, i64, ctypes.c_int, P, P]
+        L.ref_build_traverse_bvh.restype = i64
+        L.ref_quantized_positions.argtypes = [P, i32, P]
+        L.ref_quantized_positions.restype = None
         L.o64_forces.argtypes = [P, P, i32, P, P, P, P]
         L.o64_forces.restype = i64
         L.o64_forces_pairs.argtypes = [P, P, i32, P, P, i64, P, P, P, P]
@@ -228,6 +239,62 @@ def leafbuild_traverse_bvh(xyz, apl, r, predicate=PRED_SCALAR, morton=MORTON_MAS
                                           _p(timing))
     assert n2 == n
     return out, tree, timing
+
+
+def cluster_boxes(xyz_sorted, apl, r):
+    """cluster_primitives: leaf boxes of +- r (nL nodes, left = skip = 0)."""
+    xyz_sorted = _f32(xyz_sorted).reshape(-1, 3)
+    nL = len(xyz_sorted) // apl
+    nodes = np.zeros(nL, NODE_DTYPE)
+    lib().ref_cluster_boxes(_p(xyz_sorted), nL, apl, np.float32(r), _p(nodes))
+    return nodes
+
+
+def neighbor_traverse(nodes, xyz_sorted, idx_sorted, apl, r, mode=QUERY_OWN_LEAF, nthreads=1):
+    """Point-query walk over a given tree (neighbor_traverse / the root form)."""
+    xyz_sorted = _f32(xyz_sorted).reshape(-1, 3)
+    idx_sorted = np.ascontiguousarray(idx_sorted, np.int32)
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    N = len(xyz_sorted)
+    args = (_p(nodes), _p(xyz_sorted), _p(idx_sorted), N, apl, np.float32(r), mode)
+    n = lib().ref_neighbor_traverse(*args, None, 0, nthreads)
+    if n < 0:
+        raise ValueError("oracle: bad BVH specification")
+    out = np.empty(n, PAIR_DTYPE)
+    assert lib().ref_neighbor_traverse(*args, _p(out), n, nthreads) == n
+    return out
+
+
+def build_traverse_bvh(xyz, apl, r, mode=QUERY_OWN_LEAF, morton=MORTON_MASKED, lo=None,
+                       scale=None, nthreads=1):
+    """TreeData + neighbor_traverse: returns (pairs, tree dict with store)."""
+    xyz = _f32(xyz).reshape(-1, 3)
+    N = len(xyz)
+    if N % apl != 0 or N // apl < 2:
+        raise ValueError("bad BVH specification")
+    nL = N // apl
+    lo = _f32(lo if lo is not None else np.zeros(3))
+    scale = _f32(scale if scale is not None else np.ones(3))
+    tree = {"nodes": np.zeros(2 * nL - 1, NODE_DTYPE), "perm": np.zeros(N, np.int32),
+            "codes_sorted": np.zeros(N, np.int32), "store": np.zeros(nL - 1, np.int32)}
+    tout = TreeOut(_p(tree["nodes"]).value, _p(tree["perm"]).value, _p(tree["codes_sorted"]).value)
+    head = (_p(xyz), N, apl, np.float32(r), mode, morton, _p(lo), _p(scale))
+    n = lib().ref_build_traverse_bvh(*head, None, 0, nthreads, None, None)
+    if n < 0:
+        raise ValueError("oracle: error %d" % n)
+    out = np.empty(n, PAIR_DTYPE)
+    n2 = lib().ref_build_traverse_bvh(*head, _p(out), n, nthreads, ctypes.byref(tout),
+                                      _p(tree["store"]))
+    assert n2 == n
+    return out, tree
+
+
+def quantized_positions(xyz):
+    """quantized_positions!: (N, 3) int32, 1-based rank of each atom per axis."""
+    xyz = _f32(xyz).reshape(-1, 3)
+    q = np.zeros((len(xyz), 3), np.int32)
+    lib().ref_quantized_positions(_p(xyz), len(xyz), _p(q))
+    return q
 
 
 def bruteforce(xyz, r):

@@ -195,10 +195,10 @@ static inline float jl_max(float x, float y) {
 /* ------------------------------------------------------------------------- */
 /* A7: leafcluster_primitives (BVHTraverse.jl:376-410): leaf l covers sorted  */
 /* atoms l*apl .. l*apl+apl-1, box = [min(x - r/2), max(x + r/2)].            */
+/* A17: cluster_primitives (BVHTraverse.jl:341-374): the same with +- r.      */
 /* ------------------------------------------------------------------------- */
-void ref_leaf_boxes(const float *xyz_sorted, int32_t nL, int32_t apl, float r,
-                    ref_node *nodes) {
-    const float h = r / 2.0f; /* neighbor_distance/2 in Float32 */
+static void leaf_boxes_h(const float *xyz_sorted, int32_t nL, int32_t apl, float h,
+                         ref_node *nodes) {
     for (int32_t l = 0; l < nL; ++l) {
         const float *p0 = xyz_sorted + 3 * (int64_t)l * apl;
         float mn[3], mx[3];
@@ -215,6 +215,16 @@ void ref_leaf_boxes(const float *xyz_sorted, int32_t nL, int32_t apl, float r,
         k->left = 0;
         k->skip = 0;
     }
+}
+
+void ref_leaf_boxes(const float *xyz_sorted, int32_t nL, int32_t apl, float r,
+                    ref_node *nodes) {
+    leaf_boxes_h(xyz_sorted, nL, apl, r / 2.0f /* neighbor_distance/2 in Float32 */, nodes);
+}
+
+void ref_cluster_boxes(const float *xyz_sorted, int32_t nL, int32_t apl, float r,
+                       ref_node *nodes) {
+    leaf_boxes_h(xyz_sorted, nL, apl, r, nodes);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -324,14 +334,19 @@ static void *build_worker(void *arg) {
     return NULL;
 }
 
-int ref_build_tree(const float *xyz_sorted, const int32_t *codes_sorted, int32_t N,
-                   int32_t apl, float r, ref_node *nodes, int override_root,
-                   int nthreads) {
+/* The build over leaf boxes of half-width h; store_out (nL-1 Int32, may be
+ * NULL) receives the CAS slots as the build leaves them (TreeData's store,
+ * BVHTraverse.jl:521-525, 542).  With one thread the walkers run for
+ * i = 1..nL in order, so every slot holds the first leaf of its node's left
+ * child (that child completes first). */
+static int build_tree_h(const float *xyz_sorted, const int32_t *codes_sorted, int32_t N,
+                        int32_t apl, float h, ref_node *nodes, int override_root, int nthreads,
+                        int32_t *store_out) {
     if (apl < 1 || N % apl != 0) return REF_ERR_ARG;
     int32_t nL = N / apl;
     if (nL < 2) return REF_ERR_ARG;
     memset(nodes, 0, sizeof(ref_node) * (size_t)(2 * nL - 1));
-    ref_leaf_boxes(xyz_sorted, nL, apl, r, nodes);
+    leaf_boxes_h(xyz_sorted, nL, apl, h, nodes);
     _Atomic int32_t *store = (_Atomic int32_t *)calloc((size_t)(nL - 1), sizeof(int32_t));
     build_ctx c = {nodes, store, codes_sorted, nL, apl};
     if (nthreads <= 1) {
@@ -353,6 +368,8 @@ int ref_build_tree(const float *xyz_sorted, const int32_t *codes_sorted, int32_t
         }
         for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
     }
+    if (store_out)
+        for (int32_t k = 0; k < nL - 1; ++k) store_out[k] = atomic_load(&store[k]);
     free((void *)store);
     if (override_root) {
         /* bounding_volume_hierarchy! overwrites the root box (BVHTraverse.jl:856-858) */
@@ -360,6 +377,13 @@ int ref_build_tree(const float *xyz_sorted, const int32_t *codes_sorted, int32_t
         for (int d = 0; d < 3; ++d) { root->min[d] = 0.0f; root->max[d] = 1.0f; }
     }
     return REF_OK;
+}
+
+int ref_build_tree(const float *xyz_sorted, const int32_t *codes_sorted, int32_t N,
+                   int32_t apl, float r, ref_node *nodes, int override_root,
+                   int nthreads) {
+    return build_tree_h(xyz_sorted, codes_sorted, N, apl, r / 2.0f, nodes, override_root, nthreads,
+                        NULL);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -547,6 +571,171 @@ int64_t ref_leafbuild_traverse_bvh(const float *xyz, int32_t N, int32_t apl, flo
     if (timing) { timing[0] = t1 - t0; timing[1] = t2 - t1; timing[2] = t3 - t2; }
     free(codes); free(perm); free(xs); free(cs); free(nodes);
     return cnt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* A17: the point-query traversals.                                          */
+/* REF_QUERY_OWN_LEAF: neighbor_traverse (BVHTraverse.jl:1125-1234) with     */
+/*   proximity_test! (:1057-1087) and overlap_test (:1089-1092): query q     */
+/*   (sorted, 1-based 1..N-1) starts at its own leaf ceil(q/apl); a node is  */
+/*   entered iff all(min .< p .< max); a leaf tests its atoms s with         */
+/*   q < s (q <= s and a different atom), s = sum((p - x_s).^2) < fl(r*r).  */
+/* REF_QUERY_ROOT: gpuneighbor_traverse! (ext/NaiveKA.jl:352-452): every     */
+/*   query starts at the root and keeps subjects whose ORIGINAL index        */
+/*   exceeds the query's.                                                    */
+/* Pairs are (original query index, original subject index, sqrt(s)); one    */
+/* thread gives the order q = 1, 2, ... and walk order within a query.       */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    const ref_node *nodes;
+    const float *xyz;
+    const int32_t *idx;
+    int32_t N, nL, apl, mode;
+    float r2;
+} pq_ctx;
+
+static void ref_point_query(const pq_ctx *c, int32_t q, pairvec *out) {
+    const float *p = c->xyz + 3 * (int64_t)q;
+    int32_t t = (c->mode == REF_QUERY_ROOT) ? c->nL + 1 : q / c->apl + 1;
+    int64_t visits = 0;
+    while (t != 0) {
+        if (++visits > 2 * (int64_t)c->nL) break; /* malformed tree */
+        const ref_node *k = &c->nodes[t - 1];
+        int inside = 1;
+        for (int d = 0; d < 3; ++d) inside &= (k->min[d] < p[d]) & (p[d] < k->max[d]);
+        if (!inside) { t = k->skip; continue; }
+        if (k->left != 0) { t = k->left; continue; }
+        const int64_t lo = (int64_t)(t - 1) * c->apl;
+        for (int32_t e = 0; e < c->apl; ++e) {
+            const int64_t s = lo + e;
+            const int keep = (c->mode == REF_QUERY_ROOT) ? (c->idx[q] < c->idx[s]) : (s > q);
+            if (!keep) continue;
+            const float d2 = ref_sqdist(p, c->xyz + 3 * s);
+            if (d2 < c->r2) pv_push(out, c->idx[q], c->idx[s], sqrtf(d2));
+        }
+        t = k->skip;
+    }
+}
+
+typedef struct {
+    const pq_ctx *c;
+    int chunk, nthreads;
+    pairvec out;
+} pq_task;
+
+static void *pq_worker(void *arg) {
+    pq_task *t = (pq_task *)arg;
+    /* for query_index in chunk:threads:N-1 (BVHTraverse.jl:1143); the root
+     * form queries every atom (ndrange = atom_count, NaiveKA.jl:540) */
+    const int32_t last = (t->c->mode == REF_QUERY_ROOT) ? t->c->N : t->c->N - 1;
+    for (int32_t q1 = t->chunk; q1 <= last; q1 += t->nthreads) ref_point_query(t->c, q1 - 1, &t->out);
+    return NULL;
+}
+
+int64_t ref_neighbor_traverse(const ref_node *nodes, const float *xyz_sorted,
+                              const int32_t *idx_sorted, int32_t N, int32_t apl, float r,
+                              int mode, ref_pair *out, int64_t cap, int nthreads) {
+    if (apl < 1 || N % apl != 0 || N / apl < 2) return REF_ERR_ARG;
+    pq_ctx c = {nodes, xyz_sorted, idx_sorted, N, N / apl, apl, mode, r * r};
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pq_task tk[256];
+    pthread_t th[256];
+    for (int t = 0; t < nthreads; ++t) {
+        tk[t] = (pq_task){&c, t + 1, nthreads, {NULL, 0, 0}};
+        if (nthreads == 1) pq_worker(&tk[t]);
+        else pthread_create(&th[t], NULL, pq_worker, &tk[t]);
+    }
+    if (nthreads > 1)
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    /* reduce(vcat, neighbor_vec) (BVHTraverse.jl:1178) */
+    int64_t total = 0;
+    for (int t = 0; t < nthreads; ++t) total += tk[t].out.n;
+    if (out && total <= cap) {
+        int64_t o = 0;
+        for (int t = 0; t < nthreads; ++t) {
+            if (tk[t].out.n) memcpy(out + o, tk[t].out.v, sizeof(ref_pair) * (size_t)tk[t].out.n);
+            o += tk[t].out.n;
+        }
+    }
+    for (int t = 0; t < nthreads; ++t) free(tk[t].out.v);
+    return total;
+}
+
+/* TreeData (BVHTraverse.jl:500-543) + neighbor_traverse = build_traverse_bvh
+ * (:1416-1421): codes, stable sortperm, +- r leaf boxes (cluster_primitives),
+ * the build with its store, the root box overwritten; then the walk.  The
+ * root form (gpubvh_neighborlist, NaiveKA.jl:470-558) keeps the root's own
+ * box (its build kernel does not overwrite it).  store (nL-1) may be NULL. */
+int64_t ref_build_traverse_bvh(const float *xyz, int32_t N, int32_t apl, float r, int mode,
+                               int morton_mode, const float lo[3], const float scale[3],
+                               ref_pair *out, int64_t cap, int nthreads, ref_tree_out *tree,
+                               int32_t *store) {
+    if (apl < 1 || N % apl != 0 || N / apl < 2) return REF_ERR_ARG;
+    int32_t nL = N / apl;
+    int32_t *codes = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    int rc = (morton_mode == REF_MORTON_MASKED) ? ref_morton_masked(xyz, N, codes)
+                                                : ref_morton_interleave(xyz, N, lo, scale, codes);
+    if (rc != REF_OK) { free(codes); return rc; }
+    int32_t *perm = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    ref_sortperm(codes, N, perm);
+    float *xs = (float *)malloc(sizeof(float) * 3 * (size_t)N);
+    int32_t *cs = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    for (int32_t s = 0; s < N; ++s) {
+        int32_t o = perm[s];
+        cs[s] = codes[o];
+        for (int d = 0; d < 3; ++d) xs[3 * s + d] = xyz[3 * o + d];
+    }
+    ref_node *nodes = (ref_node *)malloc(sizeof(ref_node) * (size_t)(2 * nL - 1));
+    build_tree_h(xs, cs, N, apl, r, nodes, mode == REF_QUERY_OWN_LEAF, nthreads, store);
+    int64_t cnt = ref_neighbor_traverse(nodes, xs, perm, N, apl, r, mode, out, cap, nthreads);
+    if (tree) {
+        if (tree->nodes) memcpy(tree->nodes, nodes, sizeof(ref_node) * (size_t)(2 * nL - 1));
+        if (tree->perm) memcpy(tree->perm, perm, sizeof(int32_t) * (size_t)N);
+        if (tree->codes_sorted) memcpy(tree->codes_sorted, cs, sizeof(int32_t) * (size_t)N);
+    }
+    free(codes); free(perm); free(xs); free(cs); free(nodes);
+    return cnt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* quantized_positions! (BVHTraverse.jl:290-314): per axis d = x, y, z in     */
+/* turn, a stable sort of the atoms by coordinate d (Julia isless: -0 < +0,   */
+/* NaN after everything) starting from the order the previous axis left,     */
+/* and q[i][d] = 1-based rank of atom i in that sort.                         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t key;
+    int32_t pos; /* place in the order before this sort: makes qsort stable */
+    int32_t atom;
+} rank_item;
+
+static int rank_cmp(const void *a, const void *b) {
+    const rank_item *x = (const rank_item *)a, *y = (const rank_item *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return (x->pos > y->pos) - (x->pos < y->pos);
+}
+
+/* total order of isless on Float32 as an unsigned key */
+static inline uint32_t isless_key(float v) {
+    if (isnan(v)) return 0xFFFFFFFFu;
+    uint32_t b;
+    memcpy(&b, &v, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+void ref_quantized_positions(const float *xyz, int32_t N, int32_t *q) {
+    rank_item *it = (rank_item *)malloc(sizeof(rank_item) * (size_t)(N > 0 ? N : 1));
+    for (int32_t i = 0; i < N; ++i) it[i].atom = i;
+    for (int d = 0; d < 3; ++d) {
+        for (int32_t k = 0; k < N; ++k) {
+            it[k].pos = k;
+            it[k].key = isless_key(xyz[3 * (int64_t)it[k].atom + d]);
+        }
+        qsort(it, (size_t)N, sizeof(rank_item), rank_cmp);
+        for (int32_t k = 0; k < N; ++k) q[3 * (int64_t)it[k].atom + d] = k + 1;
+    }
+    free(it);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -13,6 +13,7 @@ extern "C" {
 enum { REF_OK = 0, REF_ERR_ARG = -1, REF_ERR_DOMAIN = -2 };
 enum { REF_PRED_SCALAR = 0, REF_PRED_SIMD = 1 };
 enum { REF_MORTON_MASKED = 0, REF_MORTON_INTERLEAVE = 1 };
+enum { REF_QUERY_OWN_LEAF = 0, REF_QUERY_ROOT = 1 };
 
 /* GridKey{Float32,Int32} bit layout (BVHTraverse.jl:102-109): 32 bytes. */
 typedef struct {
@@ -59,6 +60,20 @@ int64_t ref_leafbuild_traverse_bvh(const float *xyz, int32_t N, int32_t apl, flo
                                    const float scale[3], ref_pair *out, int64_t cap,
                                    int nthreads, ref_tree_out *tree, double *timing);
 int64_t ref_bruteforce(const float *xyz, int32_t N, float r, ref_pair *out, int64_t cap);
+/* cluster_primitives: leaf boxes of +- r (BVHTraverse.jl:341-374). */
+void ref_cluster_boxes(const float *xyz_sorted, int32_t nL, int32_t apl, float r,
+                       ref_node *nodes);
+/* neighbor_traverse / gpuneighbor_traverse! over a tree (mode REF_QUERY_*). */
+int64_t ref_neighbor_traverse(const ref_node *nodes, const float *xyz_sorted,
+                              const int32_t *idx_sorted, int32_t N, int32_t apl, float r,
+                              int mode, ref_pair *out, int64_t cap, int nthreads);
+/* TreeData + neighbor_traverse (build_traverse_bvh); store: nL-1 CAS slots. */
+int64_t ref_build_traverse_bvh(const float *xyz, int32_t N, int32_t apl, float r, int mode,
+                               int morton_mode, const float lo[3], const float scale[3],
+                               ref_pair *out, int64_t cap, int nthreads, ref_tree_out *tree,
+                               int32_t *store);
+/* quantized_positions!: q (N x 3) = 1-based rank per axis, cascaded stable sorts. */
+void ref_quantized_positions(const float *xyz, int32_t N, int32_t *q);
 
 /* ---- oracle64.c: FP64 textbook physics ---------------------------------- */
 typedef struct {
