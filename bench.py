@@ -333,16 +333,69 @@ def adjoint_fd_table(pkg, horizons=(50, 100, 250, 500)):
     return {"workload": "ljc-adjoint-32768", "n": n, "checkpoint_every": 50, "rows": rows}
 
 
-def cpu_baseline(c, steps, threads):
+def cpu_threads():
+    """Threads for the CPU leg: the cores this process may run on (its
+    affinity mask), capped by OMP_NUM_THREADS where that is set (the host's
+    share of cores for one GPU)."""
+    aff = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(aff, omp) if omp > 0 else aff), aff, os.cpu_count()
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(c, threads, full=False):
+    """The reference's threaded CPU path restated in C (oracle/), timed as
+    planned for the GPU comparison: per variant a warm-up run, then the median
+    of 3 runs of a 50-step window holding 5 list builds (the workload's 1 build
+    per 10 steps).  Variants: the interleaved key (the ordering the GPU path
+    also sorts by) and the reference's masked key, each with the scalar pair
+    test and the 4x4 SIMD tile (sqrt(s) < r).  With the masked key a list
+    build takes seconds (its codes are not a locality-preserving order at
+    this box size), so outside full mode it is timed once over a 10-step
+    window with 1 build."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the CPU restatement of the reference path
     O.lib()
-    bp = O.BaseParams(1.0, 1.0, c["rc"], 1.0, c["rc_coul"], c["dt"], c["skin"], 0.0, c["L"], 4,
-                      c["rebuild_every"], 1, O.PRED_SCALAR, O.MORTON_MASKED, threads)
-    t0 = time.perf_counter()
-    _, _, times, pe = O.base_run(c["x"], c["v"], c["q"], bp, steps)
-    wall = time.perf_counter() - t0
-    return wall, times, pe
+
+    def run(morton, pred, steps):
+        bp = O.BaseParams(1.0, 1.0, c["rc"], 1.0, c["rc_coul"], c["dt"], c["skin"], 0.0, c["L"], 4,
+                          c["rebuild_every"], 1, pred, morton, threads)
+        t0 = time.perf_counter()
+        _, _, times, pe = O.base_run(c["x"], c["v"], c["q"], bp, steps)
+        return time.perf_counter() - t0, times, pe
+
+    window = 5 * c["rebuild_every"]
+    variants = {}
+    for key, morton in (("interleaved", O.MORTON_INTERLEAVE), ("masked", O.MORTON_MASKED)):
+        for pname, pred in (("simd_tile", O.PRED_SIMD), ("scalar", O.PRED_SCALAR)):
+            quick = key == "masked" and not full
+            steps = c["rebuild_every"] if quick else window
+            if not quick:
+                run(morton, pred, c["rebuild_every"])  # warm-up
+            runs = [run(morton, pred, steps) for _ in range(1 if quick else 3)]
+            runs.sort(key=lambda r: r[0])
+            wall, times, pe = runs[len(runs) // 2]
+            builds = (steps + c["rebuild_every"] - 1) // c["rebuild_every"]
+            variants["%s_%s" % (key, pname)] = {
+                "steps_per_s": round(steps / wall, 4), "steps": steps, "list_builds": builds,
+                "runs": len(runs), "wall_s": [round(r[0], 3) for r in runs],
+                "list_build_s": round((times[0] + times[1] + times[2]) / builds, 4),
+                "forces_integration_s_per_step": round(times[3] / steps, 5),
+                "pair_evals_per_s": round(pe / wall, 1)}
+            log("cpu %s %s: %.4f steps/s (median of %d, %d steps)" % (key, pname, steps / wall, len(runs),
+                                                                      steps))
+    best = max(("interleaved_simd_tile", "interleaved_scalar"), key=lambda k: variants[k]["steps_per_s"])
+    return best, variants
 
 
 def main():
@@ -352,7 +405,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed/FP64/masked forms")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the masked-key CPU variants with the full protocol too")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the FP32 timed region (for profiler runs)")
@@ -480,20 +534,24 @@ def main():
 
     # ---- CPU baseline: the reference's threaded CPU path (C restatement) ----
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        wall, times, pe = cpu_baseline(c, args.cpu_steps, threads)
-        cpu_sps = args.cpu_steps / wall
+        threads, aff, ncpu = cpu_threads()
+        if args.cpu_threads:
+            threads = args.cpu_threads
+        best, variants = cpu_baseline(c, threads, args.cpu_full)
+        b = variants[best]
+        m = variants["masked_simd_tile"]
         result["cpu_baseline"] = {
-            "value": round(cpu_sps, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "ljc-262144, %d steps incl. %d list build(s) with the masked reference "
-                      "key, %d threads; wall %.2f s (build %.2f s: keys+sort %.2f, tree %.2f, "
-                      "traversal %.2f; forces+integration %.2f s)" % (
-                          args.cpu_steps, (args.cpu_steps + c["rebuild_every"] - 1) // c["rebuild_every"],
-                          threads, wall, times[0] + times[1] + times[2], times[0], times[1],
-                          times[2], times[3]),
-            "pair_evals_per_s": round(pe / wall, 1)}
-        result["gpu_over_cpu"] = round(steps_per_s / cpu_sps, 1)
-        log("cpu baseline: %.4f steps/s on %d threads" % (cpu_sps, threads))
+            "value": b["steps_per_s"], "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "ljc-262144 on %d threads (%s; affinity %d of %d cpus): the reference's threaded CPU "
+                      "path restated in C, %s; median of %d runs of %d steps with %d list builds after a "
+                      "warm-up run" % (threads, cpu_model(), aff, ncpu, best.replace("_", " "), b["runs"],
+                                       b["steps"], b["list_builds"]),
+            "pair_evals_per_s": b["pair_evals_per_s"], "list_build_s": b["list_build_s"],
+            "reference_key_steps_per_s": m["steps_per_s"], "variants": variants}
+        result["gpu_over_cpu"] = round(steps_per_s / b["steps_per_s"], 1)
+        mk = result.get("forms", {}).get("fp32_masked_key_steps_per_s")
+        if mk:
+            result["gpu_over_cpu_reference_key"] = round(mk / m["steps_per_s"], 1)
 
     print(json.dumps(result), flush=True)
 

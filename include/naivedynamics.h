@@ -297,6 +297,8 @@ typedef struct {
     int64_t forced_rebuilds;  /* list rebuilds outside the rebuild_every schedule in those
                                  reruns (made before the first force the stale list would
                                  have served) */
+    int64_t graph_recaptures; /* window graphs captured again because a buffer they use had
+                                 moved since capture (a guard: expected 0) */
 } nd_sim_stats;
 
 typedef struct nd_sim nd_sim;

@@ -8,6 +8,7 @@ Pair lists are numpy structured arrays with fields (i, j, d), the layout of
 Vector{Tuple{Int32,Int32,Float32}}.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -123,16 +124,19 @@ def leafneighbor_traverse(keys, xyz_sorted, index_sorted, spec, predicate=PRED_S
 
 
 class _PairBuffer:
-    """Grow-only device output buffer per context (capacity protocol)."""
+    """Grow-only device output buffer per context (capacity protocol).  Keyed
+    by the context object itself (weakly), so a buffer never outlives or
+    changes hands between contexts; a buffer freed by its context's close is
+    replaced."""
 
     def __init__(self):
-        self.buf = {}
+        self.buf = weakref.WeakKeyDictionary()
 
     def get(self, ctx, cap):
-        d = self.buf.get(id(ctx))
-        if d is None or d.shape[0] < cap:
+        d = self.buf.get(ctx)
+        if d is None or d.ptr is None or d.shape[0] < cap:
             d = DeviceArray(ctx, (cap,), PAIR_DTYPE)
-            self.buf[id(ctx)] = d
+            self.buf[ctx] = d
         return d
 
 

@@ -1601,6 +1601,7 @@ struct nd_sim {
     int64_t step = 0, rebuilds = 0, last_build_step = -1;
     int32_t since = 0;  // drifts launched since the last list build
     int64_t stale_reruns = 0, forced_rebuilds = 0;
+    int64_t graph_recaptures = 0;  // graphs dropped because a buffer moved (expected 0)
     // stale flag in pinned host memory (k_stale_to_host) and one event per
     // window in flight, so a run can stop early when a list went stale
     volatile int32_t *hstale = nullptr;
@@ -1614,6 +1615,7 @@ struct nd_sim {
     // graphs of one rebuild window, per buffer parity (bits 0-1) and kind
     // (bit 2: window that re-sorts)
     hipGraphExec_t gexec[8] = {};
+    uint64_t gsig[8] = {};  // buffer set each graph was captured with
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -1640,6 +1642,65 @@ int4 *fix_ptr(nd_sim *s, int b) { return s->p.precision == ND_PREC_MIXED ? s->fi
 void drop_graphs(nd_sim *s) {
     for (auto &g : s->gexec)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+// Every device buffer of a simulation (teardown releases them all; the
+// buffer-set signature covers them all).
+template <class F>
+void for_each_buf(nd_sim *s, F &&f) {
+    nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
+                          &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes,
+                          &s->store, &s->kids, &s->rep, &s->boxes, &s->nbr, &s->ncount, &s->misc,
+                          &s->partials, &s->rows, &s->snap_pos, &s->snap_vel, &s->snap_frc, &s->snap_orig,
+                          &s->sort.keys_alt, &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos,
+                          &s->cstart, &s->path, &s->nbr16, &s->tables, &s->tcount, &s->recs, &s->nrecs,
+                          &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig};
+    for (auto *b : bufs) f(*b);
+}
+
+// FNV-1a over the address and size of every buffer a captured launch may
+// use: a graph bakes the addresses in, so it may be replayed only while this
+// is unchanged.
+uint64_t buf_signature(nd_sim *s) {
+    // the snapshot buffers are copied by the host, never by a captured launch
+    const nd::DevBuf *host_only[] = {&s->snap_pos, &s->snap_vel, &s->snap_frc, &s->snap_orig,
+                                     &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig};
+    uint64_t h = 1469598103934665603ull;
+    for_each_buf(s, [&](const nd::DevBuf &b) {
+        for (const nd::DevBuf *x : host_only)
+            if (x == &b) return;
+        const uint64_t w[2] = {(uint64_t)(uintptr_t)b.p, (uint64_t)b.bytes};
+        for (uint64_t v : w)
+            for (int k = 0; k < 8; ++k) h = (h ^ ((v >> (8 * k)) & 0xFF)) * 1099511628211ull;
+    });
+    return h;
+}
+
+// Host-issued copies of the simulation, each checked against the live
+// allocation it names: a copy into a released buffer or past a buffer's end
+// is an error here rather than a fault inside the runtime.
+void check_span(const nd::DevBuf &b, size_t off, size_t bytes) {
+    if (!b.p || off > b.bytes || bytes > b.bytes - off)
+        throw nd::Error(ND_ERR_STATE, "simulation: copy outside a live device buffer");
+}
+
+void copy_d2d(const nd::DevBuf &dst, const nd::DevBuf &src, size_t bytes, hipStream_t st) {
+    if (!bytes) return;
+    check_span(dst, 0, bytes);
+    check_span(src, 0, bytes);
+    ND_HIP(hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, st));
+}
+
+void copy_d2h(void *dst, const nd::DevBuf &src, size_t bytes, hipStream_t st) {
+    if (!bytes) return;
+    check_span(src, 0, bytes);
+    ND_HIP(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st));
+}
+
+void copy_h2d(const nd::DevBuf &dst, const void *src, size_t bytes, hipStream_t st, size_t off = 0) {
+    if (!bytes) return;
+    check_span(dst, off, bytes);
+    ND_HIP(hipMemcpyAsync(static_cast<char *>(dst.p) + off, src, bytes, hipMemcpyHostToDevice, st));
 }
 
 void alloc_lists(nd_sim *s, int32_t cap) {
@@ -1948,7 +2009,7 @@ void launch_middle_step(nd_sim *s, int64_t k) {
 }
 
 void read_misc(nd_sim *s, SimMisc &m) {
-    ND_HIP(hipMemcpyAsync(&m, s->misc.p, sizeof(SimMisc), hipMemcpyDeviceToHost, s->ctx->stream));
+    copy_d2h(&m, s->misc, sizeof(SimMisc), s->ctx->stream);
     ND_HIP(hipStreamSynchronize(s->ctx->stream));
 }
 
@@ -1985,21 +2046,19 @@ void snapshot(nd_sim *s) {
     s->snap_vel.reserve(pb);
     s->snap_frc.reserve(pb);
     s->snap_orig.reserve(sizeof(int32_t) * (size_t)s->n);
-    ND_HIP(hipMemcpyAsync(s->snap_pos.p, s->pos[s->cur].p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->snap_vel.p, s->vel[s->vcur].p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->snap_frc.p, s->frc.p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->snap_orig.p, s->orig[s->vcur].p, sizeof(int32_t) * (size_t)s->n,
-                          hipMemcpyDeviceToDevice, st));
+    copy_d2d(s->snap_pos, s->pos[s->cur], pb, st);
+    copy_d2d(s->snap_vel, s->vel[s->vcur], pb, st);
+    copy_d2d(s->snap_frc, s->frc, pb, st);
+    copy_d2d(s->snap_orig, s->orig[s->vcur], sizeof(int32_t) * (size_t)s->n, st);
 }
 
 void restore(nd_sim *s) {
     hipStream_t st = s->ctx->stream;
     const size_t pb = pos_bytes(s);
-    ND_HIP(hipMemcpyAsync(s->pos[s->cur].p, s->snap_pos.p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->vel[s->vcur].p, s->snap_vel.p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->frc.p, s->snap_frc.p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->orig[s->vcur].p, s->snap_orig.p, sizeof(int32_t) * (size_t)s->n,
-                          hipMemcpyDeviceToDevice, st));
+    copy_d2d(s->pos[s->cur], s->snap_pos, pb, st);
+    copy_d2d(s->vel[s->vcur], s->snap_vel, pb, st);
+    copy_d2d(s->frc, s->snap_frc, pb, st);
+    copy_d2d(s->orig[s->vcur], s->snap_orig, sizeof(int32_t) * (size_t)s->n, st);
 }
 
 bool graph_eligible(const nd_sim *s) {
@@ -2024,6 +2083,14 @@ void launch_window(nd_sim *s, int64_t k0, bool full) {
 // as it found it.
 hipGraphExec_t window_graph(nd_sim *s, int64_t k0, int cur, int vcur, bool full) {
     const int key = cur | (vcur << 1) | (full ? 4 : 0);
+    const uint64_t sig = buf_signature(s);
+    if (s->gexec[key] && s->gsig[key] != sig) {
+        // a buffer moved since capture (every reallocation drops the graphs,
+        // so this is a guard): never replay stale addresses, capture again
+        (void)hipGraphExecDestroy(s->gexec[key]);
+        s->gexec[key] = nullptr;
+        s->graph_recaptures++;
+    }
     if (s->gexec[key]) return s->gexec[key];
     hipStream_t st = s->ctx->stream;
     hipGraph_t graph;
@@ -2056,6 +2123,7 @@ hipGraphExec_t window_graph(nd_sim *s, int64_t k0, int cur, int vcur, bool full)
     const hipError_t e = hipGraphInstantiate(&s->gexec[key], graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     ND_HIP(e);
+    s->gsig[key] = sig;
     return s->gexec[key];
 }
 
@@ -2083,8 +2151,7 @@ bool take_stale(nd_sim *s) {
     read_misc(s, m);
     if (!m.stale) return false;
     m.stale = 0;
-    ND_HIP(hipMemcpyAsync(&reinterpret_cast<SimMisc *>(s->misc.p)->stale, &m.stale, sizeof(int32_t),
-                          hipMemcpyHostToDevice, s->ctx->stream));
+    copy_h2d(s->misc, &m.stale, sizeof(int32_t), s->ctx->stream, offsetof(SimMisc, stale));
     ND_HIP(hipStreamSynchronize(s->ctx->stream));
     return true;
 }
@@ -2105,10 +2172,9 @@ WindowStart window_snapshot(nd_sim *s) {
     s->wsnap_pos.reserve(pb);
     s->wsnap_vel.reserve(pb);
     s->wsnap_orig.reserve(sizeof(int32_t) * (size_t)s->n);
-    ND_HIP(hipMemcpyAsync(s->wsnap_pos.p, s->pos[s->cur].p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->wsnap_vel.p, s->vel[s->vcur].p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->wsnap_orig.p, s->orig[s->vcur].p, sizeof(int32_t) * (size_t)s->n,
-                          hipMemcpyDeviceToDevice, st));
+    copy_d2d(s->wsnap_pos, s->pos[s->cur], pb, st);
+    copy_d2d(s->wsnap_vel, s->vel[s->vcur], pb, st);
+    copy_d2d(s->wsnap_orig, s->orig[s->vcur], sizeof(int32_t) * (size_t)s->n, st);
     WindowStart w{s->cur, s->vcur, s->since, s->rebuilds, s->last_build_step, SimMisc{}};
     read_misc(s, w.misc);  // synchronises
     return w;
@@ -2125,13 +2191,12 @@ void window_restore(nd_sim *s, const WindowStart &w) {
     s->last_build_step = w.last_build_step;
     s->tree_valid = false;
     const size_t pb = pos_bytes(s);
-    ND_HIP(hipMemcpyAsync(s->pos[s->cur].p, s->wsnap_pos.p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->vel[s->vcur].p, s->wsnap_vel.p, pb, hipMemcpyDeviceToDevice, st));
-    ND_HIP(hipMemcpyAsync(s->orig[s->vcur].p, s->wsnap_orig.p, sizeof(int32_t) * (size_t)s->n,
-                          hipMemcpyDeviceToDevice, st));
+    copy_d2d(s->pos[s->cur], s->wsnap_pos, pb, st);
+    copy_d2d(s->vel[s->vcur], s->wsnap_vel, pb, st);
+    copy_d2d(s->orig[s->vcur], s->wsnap_orig, sizeof(int32_t) * (size_t)s->n, st);
     SimMisc m = w.misc;
     m.stale = 0;
-    ND_HIP(hipMemcpyAsync(s->misc.p, &m, sizeof(SimMisc), hipMemcpyHostToDevice, st));
+    copy_h2d(s->misc, &m, sizeof(SimMisc), st);
     ND_HIP(hipStreamSynchronize(st));
 }
 
@@ -2396,14 +2461,7 @@ int nd_sim_destroy(nd_sim *s) {
     for (auto e : s->wev)
         if (e) (void)hipEventDestroy(e);
     if (s->hstale) (void)hipHostFree(const_cast<int32_t *>(s->hstale));
-    nd::DevBuf *bufs[] = {&s->pos[0], &s->pos[1], &s->vel[0], &s->vel[1], &s->frc, &s->orig[0],
-                          &s->orig[1], &s->pos32, &s->fix[0], &s->fix[1], &s->codes, &s->perm, &s->nodes, &s->store, &s->kids, &s->rep, &s->boxes,
-                          &s->nbr, &s->ncount, &s->misc, &s->partials, &s->rows, &s->snap_pos,
-                          &s->snap_vel, &s->snap_frc, &s->snap_orig, &s->sort.keys_alt,
-                          &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos, &s->cstart, &s->path,
-                          &s->nbr16, &s->tables, &s->tcount, &s->recs, &s->nrecs,
-                          &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig};
-    for (auto *b : bufs) b->release();
+    for_each_buf(s, [](nd::DevBuf &b) { b.release(); });
     delete s;
     return ND_OK;
 }
@@ -2427,8 +2485,8 @@ int nd_sim_set_state(nd_sim *s, const double *h_x, const double *h_v, const doub
             x[i] = make_double4(h_x[3 * i], h_x[3 * i + 1], h_x[3 * i + 2], h_q ? h_q[i] : 0.0);
             v[i] = make_double4(h_v[3 * i], h_v[3 * i + 1], h_v[3 * i + 2], 1.0 / m);
         }
-        ND_HIP(hipMemcpyAsync(s->pos[0].p, x.data(), sizeof(double4) * n, hipMemcpyHostToDevice, ctx->stream));
-        ND_HIP(hipMemcpyAsync(s->vel[0].p, v.data(), sizeof(double4) * n, hipMemcpyHostToDevice, ctx->stream));
+        copy_h2d(s->pos[0], x.data(), sizeof(double4) * n, ctx->stream);
+        copy_h2d(s->vel[0], v.data(), sizeof(double4) * n, ctx->stream);
         ND_HIP(hipStreamSynchronize(ctx->stream));
     } else {
         std::vector<float4> x(n), v(n);
@@ -2440,11 +2498,11 @@ int nd_sim_set_state(nd_sim *s, const double *h_x, const double *h_v, const doub
             v[i] = make_float4((float)h_v[3 * i], (float)h_v[3 * i + 1], (float)h_v[3 * i + 2],
                                (float)(1.0 / m));
         }
-        ND_HIP(hipMemcpyAsync(s->pos[0].p, x.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
-        ND_HIP(hipMemcpyAsync(s->vel[0].p, v.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
+        copy_h2d(s->pos[0], x.data(), sizeof(float4) * n, ctx->stream);
+        copy_h2d(s->vel[0], v.data(), sizeof(float4) * n, ctx->stream);
         ND_HIP(hipStreamSynchronize(ctx->stream));
     }
-    ND_HIP(hipMemcpyAsync(s->orig[0].p, orig.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    copy_h2d(s->orig[0], orig.data(), sizeof(int32_t) * n, ctx->stream);
     ND_HIP(hipMemsetAsync(s->misc.p, 0, sizeof(SimMisc), ctx->stream));
     ND_HIP(hipStreamSynchronize(ctx->stream));
     s->step = 0;
@@ -2465,7 +2523,7 @@ int nd_sim_get_state(nd_sim *s, double *h_x, double *h_v, double *h_f) {
     ND_HIP(hipSetDevice(ctx->device));
     const size_t n = (size_t)s->n;
     std::vector<int32_t> orig(n);
-    ND_HIP(hipMemcpyAsync(orig.data(), s->orig[s->vcur].p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+    copy_d2h(orig.data(), s->orig[s->vcur], sizeof(int32_t) * n, ctx->stream);
     auto unpack = [&](const auto &x, const auto &v, const auto &f) {
         ND_HIP(hipStreamSynchronize(ctx->stream));
         for (size_t k = 0; k < n; ++k) {
@@ -2480,15 +2538,15 @@ int nd_sim_get_state(nd_sim *s, double *h_x, double *h_v, double *h_f) {
     };
     if (s->wide) {
         std::vector<double4> x(n), v(n), f(n);
-        ND_HIP(hipMemcpyAsync(x.data(), s->pos[s->cur].p, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
-        ND_HIP(hipMemcpyAsync(v.data(), s->vel[s->vcur].p, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
-        ND_HIP(hipMemcpyAsync(f.data(), s->frc.p, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        copy_d2h(x.data(), s->pos[s->cur], sizeof(double4) * n, ctx->stream);
+        copy_d2h(v.data(), s->vel[s->vcur], sizeof(double4) * n, ctx->stream);
+        copy_d2h(f.data(), s->frc, sizeof(double4) * n, ctx->stream);
         unpack(x, v, f);
     } else {
         std::vector<float4> x(n), v(n), f(n);
-        ND_HIP(hipMemcpyAsync(x.data(), s->pos[s->cur].p, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
-        ND_HIP(hipMemcpyAsync(v.data(), s->vel[s->vcur].p, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
-        ND_HIP(hipMemcpyAsync(f.data(), s->frc.p, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        copy_d2h(x.data(), s->pos[s->cur], sizeof(float4) * n, ctx->stream);
+        copy_d2h(v.data(), s->vel[s->vcur], sizeof(float4) * n, ctx->stream);
+        copy_d2h(f.data(), s->frc, sizeof(float4) * n, ctx->stream);
         unpack(x, v, f);
     }
     ND_API_END(ctx)
@@ -2560,7 +2618,7 @@ int nd_sim_run(nd_sim *s, int32_t nsteps) {
         m0.overflow = 0;
         m0.maxcnt = 0;
         m0.stale = 0;
-        ND_HIP(hipMemcpyAsync(s->misc.p, &m0, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
+        copy_h2d(s->misc, &m0, sizeof(SimMisc), ctx->stream);
         ND_HIP(hipStreamSynchronize(ctx->stream));
         s->step = step0;
         s->rebuilds = rb0;
@@ -2603,9 +2661,9 @@ int nd_sim_energies(nd_sim *s, double *h_rows, int32_t max_rows, int32_t *rows) 
     int32_t r = m.rowctr < s->max_rows ? m.rowctr : s->max_rows;
     if (r > max_rows) r = max_rows;
     if (r > 0 && h_rows)
-        ND_HIP(hipMemcpyAsync(h_rows, s->rows.p, sizeof(double) * 5 * (size_t)r, hipMemcpyDeviceToHost, ctx->stream));
+        copy_d2h(h_rows, s->rows, sizeof(double) * 5 * (size_t)r, ctx->stream);
     m.rowctr = 0;
-    ND_HIP(hipMemcpyAsync(s->misc.p, &m, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
+    copy_h2d(s->misc, &m, sizeof(SimMisc), ctx->stream);
     ND_HIP(hipStreamSynchronize(ctx->stream));
     *rows = r;
     if (r > 0 && h_rows) s->pairs_last = (int64_t)h_rows[5 * (size_t)(r - 1) + 4];
@@ -2619,7 +2677,7 @@ int nd_sim_stats_get(nd_sim *s, nd_sim_stats *out) {
     std::vector<int32_t> cnt((size_t)s->n);
     int64_t entries = 0;
     if (s->list_valid) {
-        ND_HIP(hipMemcpyAsync(cnt.data(), s->ncount.p, sizeof(int32_t) * (size_t)s->n, hipMemcpyDeviceToHost, ctx->stream));
+        copy_d2h(cnt.data(), s->ncount, sizeof(int32_t) * (size_t)s->n, ctx->stream);
         ND_HIP(hipStreamSynchronize(ctx->stream));
         for (int32_t c : cnt) entries += c;
     }
@@ -2631,6 +2689,7 @@ int nd_sim_stats_get(nd_sim *s, nd_sim_stats *out) {
     out->pairs_in_cutoff = s->pairs_last;
     out->stale_reruns = s->stale_reruns;
     out->forced_rebuilds = s->forced_rebuilds;
+    out->graph_recaptures = s->graph_recaptures;
     ND_API_END(ctx)
 }
 
@@ -2664,7 +2723,7 @@ int nd_sim_time_fused_window(nd_sim *s, int32_t nsteps, double *avg_fused_ms) {
     read_misc(s, m);  // kernels are stream-ordered; this also syncs the final step
     // keep the device step counter in line with the host's
     m.step = s->step;
-    ND_HIP(hipMemcpyAsync(s->misc.p, &m, sizeof(SimMisc), hipMemcpyHostToDevice, ctx->stream));
+    copy_h2d(s->misc, &m, sizeof(SimMisc), ctx->stream);
     ND_HIP(hipStreamSynchronize(ctx->stream));
     ND_API_END(ctx)
 }
@@ -2687,9 +2746,8 @@ int nd_sim_set_replicas(nd_sim *s, int32_t nrep, const int32_t *h_rep, const dou
             ND_REQUIRE(h_boxes[6 * r + 3 + d] > h_boxes[6 * r + d], "nd_sim_set_replicas: empty replica box");
     s->rep.reserve(sizeof(int32_t) * (size_t)s->n);
     s->boxes.reserve(sizeof(double) * 6 * (size_t)nrep);
-    ND_HIP(hipMemcpyAsync(s->rep.p, h_rep, sizeof(int32_t) * (size_t)s->n, hipMemcpyHostToDevice, ctx->stream));
-    ND_HIP(hipMemcpyAsync(s->boxes.p, h_boxes, sizeof(double) * 6 * (size_t)nrep, hipMemcpyHostToDevice,
-                          ctx->stream));
+    copy_h2d(s->rep, h_rep, sizeof(int32_t) * (size_t)s->n, ctx->stream);
+    copy_h2d(s->boxes, h_boxes, sizeof(double) * 6 * (size_t)nrep, ctx->stream);
     ND_HIP(hipStreamSynchronize(ctx->stream));
     s->nrep = nrep;
     drop_graphs(s);  // captured launches carry the wall pointers

@@ -444,6 +444,40 @@ typedef struct {
     int predicate;
 } trav_ctx;
 
+/* 4 x 4 tile of twocluster_proximitytest! / the SIMD tile (ext/NaiveSIMD.jl:
+ * 567-608) as four 4-lane vectors (one per query row): the same IEEE
+ * operations per pair as ref_sqdist / ref_accept, so the same pairs and
+ * distance bits in the same (row-major) order, computed 4 pairs at a time. */
+typedef float v4f __attribute__((vector_size(16)));
+typedef int32_t v4i __attribute__((vector_size(16)));
+
+static inline void ref_tile4(const trav_ctx *c, int64_t qlo, int64_t tlo, pairvec *out) {
+    const float *t = c->xyz + 3 * tlo;
+    const v4f tx = {t[0], t[3], t[6], t[9]}, ty = {t[1], t[4], t[7], t[10]},
+              tz = {t[2], t[5], t[8], t[11]};
+    const v4f r2 = {c->r2, c->r2, c->r2, c->r2}, rr = {c->r, c->r, c->r, c->r};
+    for (int a = 0; a < 4; ++a) {
+        const float *p = c->xyz + 3 * (qlo + a);
+        const v4f dx = p[0] - tx, dy = p[1] - ty, dz = p[2] - tz;
+        const v4f sx = dx * dx, sy = dy * dy, sz = dz * dz;
+        const v4f s = (sx + sy) + sz;
+        v4f d;
+        v4i keep;
+        if (c->predicate == REF_PRED_SCALAR) {
+            keep = s < r2;
+            d = s;
+        } else {
+            d = __builtin_ia32_sqrtps(s);
+            keep = d < rr;
+        }
+        if (!(keep[0] | keep[1] | keep[2] | keep[3])) continue;
+        for (int b = 0; b < 4; ++b)
+            if (keep[b])
+                pv_push(out, c->idx[qlo + a], c->idx[tlo + b],
+                        c->predicate == REF_PRED_SCALAR ? sqrtf(d[b]) : d[b]);
+    }
+}
+
 static void ref_traverse_leaf(const trav_ctx *c, int32_t q, pairvec *out, int64_t *tests) {
     const int32_t apl = c->apl;
     const ref_node *ql = &c->nodes[q - 1];
@@ -462,6 +496,11 @@ static void ref_traverse_leaf(const trav_ctx *c, int32_t q, pairvec *out, int64_
             if (tn->left == 0) {
                 const int64_t tlo = (int64_t)(t - 1) * apl;
                 if (tests) (*tests)++;
+                if (apl == 4) {
+                    ref_tile4(c, qlo, tlo, out);
+                    t = tn->skip;
+                    continue;
+                }
                 for (int32_t a = 0; a < apl; ++a)
                     for (int32_t b = 0; b < apl; ++b) {
                         float s = ref_sqdist(c->xyz + 3 * (qlo + a), c->xyz + 3 * (tlo + b));

@@ -331,6 +331,8 @@ def test_prepared_graphs_give_the_same_trajectory(pkg, resort_every):
         sim.run(95)
         out.append(sim.get_state())
         assert sim.stats()["rebuilds"] == 11   # steps 0, 10, ..., 100
+        # no graph ever replayed buffers that had moved since its capture
+        assert sim.stats()["graph_recaptures"] == 0
         sim.close()
     for k in (1, 2):
         for a, b in zip(out[0], out[k]):
