@@ -256,17 +256,22 @@ struct PairAcc {
         add_d<ONE>((CalcS)(pi.x - pj.x), (CalcS)(pi.y - pj.y), (CalcS)(pi.z - pj.z), (CalcS)pj.w,
                    valid);
     }
-    // fixed-point neighbour (MIXED): exact integer difference, then FP32
+    // fixed-point neighbour (MIXED): exact integer difference, rounded once to
+    // FP32, in fixed-point units.  The unit inv is a power of two, so scaling
+    // r^2 by inv^2 and the force factor by inv gives bit for bit the values
+    // of scaling each difference, with one multiply fewer per pair.
     template <bool ONE> __device__ __forceinline__ void add(const int4 &pj, bool valid) {
-        const CalcS dx = (CalcS)(float)(int32_t)((uint32_t)pfx.x - (uint32_t)pj.x) * (CalcS)inv;
-        const CalcS dy = (CalcS)(float)(int32_t)((uint32_t)pfx.y - (uint32_t)pj.y) * (CalcS)inv;
-        const CalcS dz = (CalcS)(float)(int32_t)((uint32_t)pfx.z - (uint32_t)pj.z) * (CalcS)inv;
-        add_d<ONE>(dx, dy, dz, (CalcS)__int_as_float(pj.w), valid);
+        const CalcS ux = (CalcS)(float)(int32_t)((uint32_t)pfx.x - (uint32_t)pj.x);
+        const CalcS uy = (CalcS)(float)(int32_t)((uint32_t)pfx.y - (uint32_t)pj.y);
+        const CalcS uz = (CalcS)(float)(int32_t)((uint32_t)pfx.z - (uint32_t)pj.z);
+        add_d<ONE, true>(ux, uy, uz, (CalcS)__int_as_float(pj.w), valid);
     }
-    template <bool ONE>
+    // SCALED: dx, dy, dz are in units of inv (MIXED)
+    template <bool ONE, bool SCALED = false>
     __device__ __forceinline__ void add_d(const CalcS dx, const CalcS dy, const CalcS dz, const CalcS qj,
                                           const bool valid) {
-        const CalcS r2 = dx * dx + dy * dy + dz * dz;
+        CalcS r2 = dx * dx + dy * dy + dz * dz;
+        if constexpr (SCALED) r2 *= (CalcS)inv * (CalcS)inv;
         const CalcS ir = rsq_(r2);
         const CalcS ir2 = ir * ir;
         const CalcS ir6 = ir2 * ir2 * ir2;
@@ -287,14 +292,15 @@ struct PairAcc {
             const CalcS t = (qm * ir + lj) * ir2;
             fs = (in_lj | in_c) ? t : (CalcS)0;
         }
+        const CalcS fd = SCALED ? fs * (CalcS)inv : fs;
         if constexpr (kBatchSums) {
-            bx += fs * dx;
-            by += fs * dy;
-            bz += fs * dz;
+            bx += fd * dx;
+            by += fd * dy;
+            bz += fd * dz;
         } else {
-            fx += fs * dx;
-            fy += fs * dy;
-            fz += fs * dz;
+            fx += fd * dx;
+            fy += fd * dy;
+            fz += fd * dz;
         }
         if (ENERGY) {
             const CalcS elj_all = ir6 * (e12 * ir6 - e6) - eljs;
@@ -309,6 +315,10 @@ struct PairAcc {
 };
 
 constexpr int kPairBatch = 4;
+// Rows of a 64-atom block of the 16-bit list: two entries per row for up to
+// cap entries, plus two rows of sentinel entries (the step kernel's last
+// batch may run up to three entries past the wave's longest list).
+__host__ __device__ __forceinline__ int32_t rows16(int32_t cap) { return (cap + 1) / 2 + 2; }
 // Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format;
 // untyped loads ignore it otherwise).
 constexpr int32_t kBufWord3 = 0x00020000;
@@ -342,7 +352,8 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     const int32_t cnt = i < n ? min(ncount[i], P.cap) : 0;
     int32_t wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
-    const int32_t last = max(__builtin_amdgcn_readfirstlane(wmax) - 1, 0);
+    wmax = __builtin_amdgcn_readfirstlane(wmax);
+    const int32_t last = max(wmax - 1, 0);
     const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
         FX ? (void *)fix_in : (void *)pos_in, 0, n * (int32_t)sizeof(G), kBufWord3);
     // neighbour record j (a sorted index; -1 gives zeros)
@@ -361,14 +372,48 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         return g;
     };
     // this workgroup's neighbourhood table (k_table) staged in LDS; -1: none
-    // (more distinct neighbours than lcap), the lists hold sorted indices
+    // (more distinct neighbours than lcap), the lists hold sorted indices.
+    // Slot U is the sentinel every padding entry names.  FP64: a position so
+    // far away that the cutoff masks it, so its table loop needs no per-pair
+    // count check (measured faster).  FP32 and MIXED keep the count check (for
+    // FP32 dropping it measured slower: the compiler's schedule of the loop
+    // waits more; MIXED's fixed-point differences would wrap) and a zero slot.
     const int32_t U = __builtin_amdgcn_readfirstlane(tcount[blk]);
     G *lt = reinterpret_cast<G *>(s_table);
+    constexpr bool kSentinel = sizeof(G) == 32;
     if (U >= 0) {
         const int32_t *tab = tables + (int64_t)blk * lcap;
-        for (int32_t t = threadIdx.x; t < U; t += blockDim.x) lt[t] = gather(tab[t]);
-        // padding entries read record 0: keep it finite for an empty table
-        if (U == 0 && threadIdx.x == 0) lt[0] = gather(-1);
+        // kStage index loads, then kStage gathers, then the LDS writes: two
+        // memory latencies per kStage slots instead of two per slot
+        constexpr int kStage = 8;
+        for (int32_t t0 = threadIdx.x; t0 < U; t0 += kStage * nd::kBlock) {
+            // unconditional loads (the index clamped inside this table's row,
+            // slots past U gather index -1: zeros, no memory access), so all
+            // of them are in flight together
+            int32_t j[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int32_t t = t0 + u * nd::kBlock;
+                const int32_t v = tab[min(t, lcap - 1)];
+                j[u] = t < U ? v : -1;
+            }
+            G g[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) g[u] = gather(j[u]);
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int32_t t = t0 + u * nd::kBlock;
+                if (t < U) lt[t] = g[u];
+            }
+        }
+        if (threadIdx.x == 0) {
+            G z = gather(-1);
+            if constexpr (kSentinel) {
+                z.x = z.y = z.z = (PosS)1e18;
+                z.w = 0;
+            }
+            lt[U] = z;
+        }
         __syncthreads();
     }
     if (i < n) {
@@ -391,8 +436,9 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         const int32_t wave = __builtin_amdgcn_readfirstlane(i >> 6);
         const uint32_t lane4 = (uint32_t)(i & 63) * 4u;
         // Two register sets (a, b) take turns, so no batch is copied.
-        auto pipeline = [&](auto one, auto entries, auto record) {
+        auto pipeline = [&](auto one, auto entries, auto record, auto checked) {
             constexpr bool ONE = decltype(one)::value;
+            constexpr bool CHK = decltype(checked)::value;
             constexpr int B = kPairBatch;
             int32_t ja[B], jb[B];
             G qa[B], qb[B];
@@ -405,13 +451,13 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 for (int u = 0; u < B; ++u) qb[u] = record(jb[u]);
                 entries(k + 2 * B, ja);
 #pragma unroll
-                for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], k + u < cnt);
+                for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], CHK ? k + u < cnt : true);
                 if (k + B >= cnt) break;
 #pragma unroll
                 for (int u = 0; u < B; ++u) qa[u] = record(ja[u]);
                 entries(k + 3 * B, jb);
 #pragma unroll
-                for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], k + B + u < cnt);
+                for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], CHK ? k + B + u < cnt : true);
                 acc.flush();  // MIXED: FP32 sums over 2 batches, then FP64
                 if (k + 2 * B >= cnt) break;
             }
@@ -419,11 +465,14 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         };
         auto run = [&](auto one) {
             if (U >= 0) {
-                // 16-bit table positions, two per int32 row entry
-                const int32_t cap2 = (P.cap + 1) >> 1;
+                // 16-bit table positions, two per int32 row entry; rows up to
+                // the wave's last, then two sentinel rows, which only the
+                // unchecked (FP64) loop reads: the checked loops re-read the
+                // wave's last row for their look-ahead past it
+                const int32_t cap2 = rows16(P.cap);
                 const __amdgpu_buffer_rsrc_t l16 = __builtin_amdgcn_make_buffer_rsrc(
                     (void *)(nbr16 + (int64_t)wave * cap2 * 64), 0, cap2 * 256, kBufWord3);
-                const int32_t last2 = last >> 1;
+                const int32_t last2 = kSentinel ? max((wmax + 1) >> 1, 1) + 1 : last >> 1;
                 auto entries = [&](int32_t r, int32_t *j) {
                     static_assert(kPairBatch % 2 == 0, "two entries per word");
 #pragma unroll
@@ -435,7 +484,8 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                         j[u + 1] = (int32_t)(v >> 16);
                     }
                 };
-                pipeline(one, entries, [&](int32_t j) -> G { return lt[j]; });
+                pipeline(one, entries, [&](int32_t j) -> G { return lt[j]; },
+                         std::integral_constant<bool, !kSentinel>{});
             } else {
                 const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
                     (void *)(nbr + (int64_t)wave * P.cap * 64), 0, P.cap * 256, kBufWord3);
@@ -447,7 +497,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                         j[u] = r + u < cnt ? v : -1;
                     }
                 };
-                pipeline(one, entries, gather);
+                pipeline(one, entries, gather, std::true_type{});
             }
         };
         // equal cutoffs (uniform over the launch): one mask per pair
@@ -1234,7 +1284,7 @@ constexpr int kRowBatch = 16;
 // Tables from list rows (the rope, one-node and cell walks).
 __global__ void __launch_bounds__(nd::kBlock)
 k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int32_t n, int32_t cap,
-        int32_t cap2, int32_t lcap, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
+        int32_t cap2, int32_t lcap, int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
         int32_t *__restrict__ tcount) {
     __shared__ TableLds T;
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1279,10 +1329,13 @@ k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int
     // waves past the last atom own no list block
     if (((i >> 6) << 6) >= n) return;
     // 16-bit lists, two entries per int32, one row of 64 per step of the
-    // wave; row 0 is written even for a wave without neighbours (the step
-    // kernel's look-ahead reads it)
+    // wave, then two rows of sentinels; row 0 is written even for a wave
+    // without neighbours (the step kernel's look-ahead reads it).  Entries are
+    // table positions << eshift; past an atom's list they name the sentinel
+    // slot U.
     uint32_t *out = nbr16 + ((int64_t)(i >> 6) * cap2) * 64 + lane;
-    const int32_t nrows = max((wmax + 1) >> 1, 1);
+    const int32_t nrows = max((wmax + 1) >> 1, 1) + 2;
+    const uint32_t sent = (uint32_t)U << eshift;
     for (int32_t q0 = 0; q0 < nrows; q0 += kRowBatch / 2) {
         int32_t jj[kRowBatch];
 #pragma unroll
@@ -1294,8 +1347,8 @@ k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int
         for (int u = 0; u < kRowBatch; u += 2) {
             const int32_t q = q0 + u / 2;
             if (q >= nrows) break;
-            const uint32_t a = jj[u] >= 0 ? T.local(jj[u]) : 0u;
-            const uint32_t b = jj[u + 1] >= 0 ? T.local(jj[u + 1]) : 0u;
+            const uint32_t a = jj[u] >= 0 ? T.local(jj[u]) << eshift : sent;
+            const uint32_t b = jj[u + 1] >= 0 ? T.local(jj[u + 1]) << eshift : sent;
             out[(int64_t)q * 64] = a | (b << 16);
         }
     }
@@ -1316,8 +1369,8 @@ constexpr int kRecLds = 3072;  // records per workgroup held in LDS
 __global__ void __launch_bounds__(nd::kBlock)
 k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
-            uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables, int32_t *__restrict__ tcount,
-            int32_t *__restrict__ nbr) {
+            int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
+            int32_t *__restrict__ tcount, int32_t *__restrict__ nbr) {
     constexpr int kW = nd::kBlock / 64;
     __shared__ TableLds T;
     __shared__ uint2 s_rec[kRecLds];     // (hash slot, then group base; mask)
@@ -1391,9 +1444,10 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
         // the lane's entries in record order (every record has a set bit)
         int32_t rk = roff, cb = 0;
         uint32_t bits = 0u, gm = 0u;
-        auto pop = [&]() -> uint32_t {  // table position; 0 past the end
+        const uint32_t sent = (uint32_t)U << eshift;
+        auto pop = [&]() -> uint32_t {  // table entry; the sentinel past the end
             if (bits == 0u) {
-                if (rk >= roff + nr) return 0u;
+                if (rk >= roff + nr) return sent;
                 const uint2 r = s_rec[rk];
                 gm = s_gmask[rk];
                 ++rk;
@@ -1402,13 +1456,13 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             }
             const int32_t b = __builtin_ctz(bits);
             bits &= bits - 1u;
-            return (uint32_t)(cb + __popc(gm & ((1u << b) - 1u)));
+            return (uint32_t)(cb + __popc(gm & ((1u << b) - 1u))) << eshift;
         };
         // 16-bit lists, two entries per int32, one row of 64 per step of the
-        // wave; row 0 is written even for a wave without neighbours (the
-        // step kernel's look-ahead reads it)
+        // wave, then two rows of sentinels; row 0 is written even for a wave
+        // without neighbours (the step kernel's look-ahead reads it)
         uint32_t *out = nbr16 + ((int64_t)(i >> 6) * cap2) * 64 + lane;
-        const int32_t nrows = max((wmax + 1) >> 1, 1);
+        const int32_t nrows = max((wmax + 1) >> 1, 1) + 2;
         for (int32_t r = 0; r < nrows; ++r) {
             const uint32_t a = pop(), b = pop();
             out[(int64_t)r * 64] = a | (b << 16);
@@ -1457,10 +1511,11 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
     };
     if (table) {
         uint32_t *out = nbr16 + ((int64_t)(i >> 6) * cap2) * 64 + lane;
-        const int32_t nrows = max((wmax + 1) >> 1, 1);
+        const int32_t nrows = max((wmax + 1) >> 1, 1) + 2;
+        auto enc = [&](int32_t a) -> uint32_t { return (uint32_t)(a < 0 ? U : a) << eshift; };
         for (int32_t r = 0; r < nrows; ++r) {
             const int32_t a = pop(), b = pop();
-            out[(int64_t)r * 64] = (uint32_t)max(a, 0) | ((uint32_t)max(b, 0) << 16);
+            out[(int64_t)r * 64] = enc(a) | (enc(b) << 16);
         }
     } else {
         int32_t *out = nbr + ((int64_t)(i >> 6) * cap) * 64 + lane;
@@ -1713,7 +1768,7 @@ void alloc_lists(nd_sim *s, int32_t cap) {
     // one block of cap rows x 64 slots per 64 atoms (n rounded up to 64)
     const size_t n64 = ((size_t)s->n + 63) & ~(size_t)63;
     s->nbr.reserve(sizeof(int32_t) * (size_t)cap * n64);
-    s->nbr16.reserve(sizeof(uint32_t) * (size_t)((cap + 1) / 2) * n64);
+    s->nbr16.reserve(sizeof(uint32_t) * (size_t)rows16(cap) * n64);
     s->recs.reserve(sizeof(uint2) * (size_t)rec_stride(cap) * n64);
     drop_graphs(s);
 }
@@ -1739,10 +1794,15 @@ bool resort_due(const nd_sim *s, int64_t k) { return !s->tree_valid || k % resor
 
 void launch_sort_and_build(nd_sim *s);
 
+// 16-bit list entries are table slot numbers (a shift of 4 would make them
+// LDS byte offsets of 16-byte records, one instruction fewer per pair, which
+// measured slower at config 2 and is not used).
+int32_t entry_shift(const nd_sim *) { return 0; }
+
 // Neighbourhood tables and 16-bit lists from the walk's lists.
 void launch_tables(nd_sim *s) {
     k_table<<<nd::grid_for(s->n), nd::kBlock, 0, s->ctx->stream>>>(
-        s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->n, s->cap, (s->cap + 1) / 2, s->lcap,
+        s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->n, s->cap, rows16(s->cap), s->lcap, entry_shift(s),
         s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(), s->tcount.as<int32_t>());
     ND_HIP(hipGetLastError());
 }
@@ -1817,8 +1877,8 @@ void launch_rebuild(nd_sim *s, bool full) {
                 default: k_nlist3<32, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
             }
             ND_HIP(hipGetLastError());
-            k_table_rec<<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, (s->cap + 1) / 2, s->lcap,
-                                                  s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
+            k_table_rec<<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
+                                                  entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
                                                   s->tcount.as<int32_t>(), nb);
             ND_HIP(hipGetLastError());
             return;
@@ -1911,7 +1971,7 @@ void launch_step_t(nd_sim *s, bool check) {
     const unsigned g = nd::grid_for(s->n);
     const int a = s->cur ^ 1;
     using G = std::conditional_t<sizeof(PosS) == 8 && sizeof(CalcS) == 4, int4, P4>;
-    const size_t lds = sizeof(G) * (size_t)s->lcap;
+    const size_t lds = sizeof(G) * (size_t)(s->lcap + 1);  // the table and its sentinel slot
     if (lds > 65536) {
         // dynamic LDS past 64 KiB must be allowed per kernel (once)
         static bool allowed = false;
@@ -2394,17 +2454,23 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->ncount.reserve(sizeof(int32_t) * n);
         s->path.reserve(sizeof(uint16_t) * n);
         s->nrecs.reserve(sizeof(int32_t) * n);
-        // neighbourhood tables: up to 2,560 records per workgroup in the step
-        // kernel's LDS, 40 KiB at 16 B (FP32, MIXED: four workgroups per CU)
-        // or 80 KiB at 32 B (FP64: two), within what the device allows one
-        // workgroup
+        // neighbourhood tables in the step kernel's LDS: records of 16 B
+        // (FP32, MIXED) for four workgroups per CU, of 32 B (FP64) for two;
+        // each workgroup's share of the CU's LDS holds the table, its sentinel
+        // slot and the kernel's static LDS (at most 256 B).  On a 160 KiB CU:
+        // 2,543 records (one record more would drop a workgroup per CU).
         {
-            int lds_max = 65536;
-            ND_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+            int lds_cu = 65536, lds_blk = 65536;
+            ND_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                         ctx->device));
+            ND_HIP(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
             const int rec_bytes = p->precision == ND_PREC_FP64 ? 32 : 16;
-            const int fit = ((lds_max - 1024) / rec_bytes) & ~31;
+            const int per_wg = std::min(lds_cu / (rec_bytes == 16 ? 4 : 2), lds_blk) - 256;
+            const int fit = per_wg / rec_bytes - 1;  // the sentinel slot
             s->lcap = fit < 2560 ? fit : 2560;
             if (s->lcap < 32) s->lcap = 32;
+            // table positions (and the sentinel slot) fit 16-bit entries
+            if (s->lcap + 1 > 0xFFFF) throw nd::Error(ND_ERR_STATE, "simulation: table capacity beyond 16-bit entries");
         }
         s->tables.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n) * (size_t)s->lcap);
         s->tcount.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n));
