@@ -47,14 +47,27 @@ def log(msg):
 # LBVH leaf size per workload, from sweeps over 8 / 16 / 32 with the Hilbert
 # key: 32 on the uniform liquid (ljc-262144), 16 on the dense blobs
 SIM_ATOMS_PER_LEAF = 32
+# atoms re-sorted by key (and the tree rebuilt) every 500 steps; the list
+# rebuilds between (every 10 steps) refit the same tree's boxes.  Sweep at
+# config 2 over 1,000 steps: re-sorting at every rebuild 19,815 steps/s,
+# every 20 / 50 / 100 / 200 steps 21,159 / 22,344 / 22,756 / 23,045, once
+# 24,168; 500 bounds the drift of the atom order over long runs.
+SIM_RESORT_EVERY = 500
+# the blob workload's atoms move faster: re-sort every 100 steps there
+# (500 steps, ms/step at 0 / 100 / 500: FP64 0.418 / 0.406 / 0.409, MIXED
+# 0.273 / 0.259 / 0.261, FP32 0.243 / 0.231 / 0.233); batched replicas
+# every 500 (0.290 / 0.268 / 0.265)
+BLOB_RESORT_EVERY = 100
+BATCH_RESORT_EVERY = 500
 BLOB_ATOMS_PER_LEAF = 16
 
 
-def make_spec(pkg, c, precision, key, energy_every=10, use_graph=True):
+def make_spec(pkg, c, precision, key, energy_every=10, use_graph=True, resort_every=SIM_RESORT_EVERY):
     return pkg.SimSpec(duration=0, stepwidth=c["dt"], rc=c["rc"], rc_coul=c["rc_coul"],
                        skin=c["skin"], rebuild_every=c["rebuild_every"], lo=0.0, hi=c["L"],
                        precision=precision, morton=key, energy_every=energy_every,
-                       use_graph=use_graph, walls=True, atomsperleaf=SIM_ATOMS_PER_LEAF)
+                       use_graph=use_graph, walls=True, atomsperleaf=SIM_ATOMS_PER_LEAF,
+                       resort_every=resort_every)
 
 
 def timed_run(pkg, sim, steps):
@@ -109,7 +122,8 @@ def measured_traffic():
 
 def search_comparison(pkg, c):
     """The one-call LBVH neighbour list (interleaved key, 4 atoms per leaf)
-    against the cell list (nd_cell_neighborlist), N = 262,144, r = r_list,
+    against the cell list (nd_cell_neighborlist) and the point-query list
+    (nd_build_traverse_bvh), N = 262,144, r = r_list,
     unique pairs with s < r^2, on the config-2 initial positions (a jittered
     lattice) and on uniform random positions at the same density (the
     reference benchmarks its searches on uniform points); median wall time of
@@ -135,9 +149,12 @@ def search_comparison(pkg, c):
         d = pkg.DeviceArray.from_host(ctx, x)
         ms_l, n_l = timed(lambda: pkg.neighborlist_device(d, spec, pkg.PRED_SCALAR, pkg.MORTON_INTERLEAVE)[1])
         ms_c, n_c = timed(lambda: pkg.cells.neighborlist_device(d, r, pkg.PRED_SCALAR)[1])
-        log("search %s: LBVH %.3f ms (%d pairs), cell list %.3f ms (%d pairs)" % (name, ms_l, n_l, ms_c, n_c))
+        # the point-query traversal (TreeData + neighbor_traverse), same key and leaves
+        ms_p, n_p = timed(lambda: pkg.bvh.pointquery_device(d, spec, pkg.MORTON_INTERLEAVE)[1])
+        log("search %s: LBVH %.3f ms (%d pairs), cell list %.3f ms (%d pairs), point query %.3f ms (%d pairs)"
+            % (name, ms_l, n_l, ms_c, n_c, ms_p, n_p))
         out[name] = {"lbvh_ms": round(ms_l, 3), "lbvh_pairs": int(n_l), "cell_list_ms": round(ms_c, 3),
-                     "cell_list_pairs": int(n_c)}
+                     "cell_list_pairs": int(n_c), "point_query_ms": round(ms_p, 3), "point_query_pairs": int(n_p)}
     return out
 
 
@@ -153,7 +170,8 @@ def precision_form(pkg, steps=500):
     for prec in ("fp64", "mixed", "fp32"):
         spec = pkg.SimSpec(duration=0, stepwidth=c["dt"], rc=c["rc"], rc_coul=c["rc_coul"], skin=c["skin"],
                            rebuild_every=c["rebuild_every"], lo=0.0, hi=c["L"], precision=prec,
-                           energy_every=10, walls=True, atomsperleaf=BLOB_ATOMS_PER_LEAF)
+                           energy_every=10, walls=True, atomsperleaf=BLOB_ATOMS_PER_LEAF,
+                           resort_every=BLOB_RESORT_EVERY)
         sim = pkg.Simulation(n, spec)
         sim.set_state(c["x"], c["v"], c["q"])
         sim.run(0)
@@ -199,7 +217,8 @@ def batched_form(pkg, steps=200):
     sizes = B.replica_sizes(512, seed=11)
     reps = B.make_replicas(sizes, seed=11)
     spec = pkg.SimSpec(duration=0, stepwidth=0.001, rc=2.5, rc_coul=0.0, skin=0.3, rebuild_every=10,
-                       precision="fp32", energy_every=10, atomsperleaf=SIM_ATOMS_PER_LEAF)
+                       precision="fp32", energy_every=10, atomsperleaf=SIM_ATOMS_PER_LEAF,
+                       resort_every=BATCH_RESORT_EVERY)
     best, table = pkg.autotune(reps, spec, windows=2, log=log)
     b = pkg.BatchedSimulation(reps, best)
     b.set_state()
@@ -210,8 +229,10 @@ def batched_form(pkg, steps=200):
     sec = B.run_one_per_replica(reps, best, steps)
     atoms = int(sizes.sum())
     out = {"workload": "lj-replicas-512", "replicas": 512, "atoms": atoms, "steps": steps,
-           "autotune": [{"leaf": a, "walk": w, "ms_per_step": round(t, 4)} for a, w, t in table],
-           "chosen": {"leaf": best.atomsperleaf, "walk": best.list_walk},
+           "autotune": [{"leaf": a, "walk": w, "walk_front": f or 8, "pair_batch": p or 4,
+                         "ms_per_step": round(t, 4)} for a, w, f, p, t in table],
+           "chosen": {"leaf": best.atomsperleaf, "walk": best.list_walk, "walk_front": best.walk_front or 8,
+                      "pair_batch": best.pair_batch or 4},
            "packed_steps_per_s": round(steps / (ms / 1e3), 2),
            "packed_atom_steps_per_s": float("%.4e" % (atoms * steps / (ms / 1e3))),
            "one_per_replica_atom_steps_per_s": float("%.4e" % (atoms * steps / sec.sum())),
@@ -454,7 +475,8 @@ def main():
         "config": {"workload": "ljc-262144", "n": n, "density": 0.8, "rc_lj": c["rc"],
                    "rc_coul": c["rc_coul"], "skin": c["skin"], "dt": c["dt"],
                    "rebuild_every": c["rebuild_every"], "lbvh_key": "hilbert30",
-                   "atoms_per_leaf": SIM_ATOMS_PER_LEAF, "precision": "fp32", "walls": True},
+                   "atoms_per_leaf": SIM_ATOMS_PER_LEAF, "resort_every": SIM_RESORT_EVERY, "precision": "fp32",
+                   "walls": True},
         "pair_evals_per_s": round(pairs * steps_per_s, 1),
         "pairs_in_cutoff_per_step": pairs,
         "list_pairs": st["list_entries"] // 2,
@@ -503,17 +525,16 @@ def main():
         extra["fp32_masked_key_steps_per_s"] = round(30 / (ms2 / 1e3), 3)
         s2.close()
         log("fp32 masked key: %.3f ms/step" % (ms2 / 30))
-        # full re-sort every 50 steps, box refits of the same tree in between
-        spec = make_spec(pkg, c, "fp32", pkg.MORTON_HILBERT)
-        spec.resort_every = 50
+        # a full re-sort and tree build at every list rebuild (no refits)
+        spec = make_spec(pkg, c, "fp32", pkg.MORTON_HILBERT, resort_every=0)
         s2 = pkg.Simulation(c["n"], spec)
         s2.set_state(c["x"], c["v"], c["q"])
         s2.run(args.warmup)
         s2.prepare()
         ms2, _ = timed_run(pkg, s2, args.steps)
-        extra["fp32_resort50_steps_per_s"] = round(args.steps / (ms2 / 1e3), 3)
+        extra["fp32_resort_every_rebuild_steps_per_s"] = round(args.steps / (ms2 / 1e3), 3)
         s2.close()
-        log("fp32 resort every 50: %.4f ms/step" % (ms2 / args.steps))
+        log("fp32 re-sort at every rebuild: %.4f ms/step" % (ms2 / args.steps))
         # the cell-list comparison point: the same steps with lists from a
         # uniform cell list instead of the LBVH walk
         spec = make_spec(pkg, c, "fp32", pkg.MORTON_HILBERT)

@@ -276,6 +276,10 @@ typedef struct {
                              queued leaves), 1 = rope walk, 2 = one-node child-box walk,
                              3 = uniform cell list over [lo, hi]^3 (no LBVH; the
                              comparison point; needs hi > lo) */
+    int32_t pair_batch;   /* step kernel: neighbours per software-pipeline batch, 2, 4 or
+                             8 (FP32; MIXED and FP64 use 4); 0 = 4 */
+    int32_t walk_front;   /* two-phase walk: tree records fetched per round trip, 8 or
+                             16; 0 = 8 */
     double eps, sigma, rc; /* LJ */
     double k_coul, rc_coul;/* Coulomb (rc_coul = 0 disables) */
     double skin;           /* list radius = max(rc, rc_coul) + skin */

@@ -148,18 +148,22 @@ class BatchedSimulation:
         self.sim.close()
 
 
-def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None):
-    """Pick the leaf size (the traversal / force tile) and list walk of the
-    packed run by timing `windows` rebuild windows of each candidate after one
-    warm-up window: LBVH leaves of 8, 16 or 32 atoms with the two-phase (0)
-    or rope (1) walk, and the cell list (3, no tree).  Returns (best spec,
-    [(leaf, walk, ms_per_step)])."""
+def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None, tiles=True):
+    """Pick the packed run's traversal and force tile shapes by timing
+    `windows` rebuild windows of each candidate after one warm-up window (every
+    window graph captured before timing).  Stage 1: LBVH leaf size (the
+    traversal and force tile: 8, 16 or 32 atoms) with the two-phase (0) or rope
+    (1) walk, and the cell list (3, no tree).  Stage 2 (`tiles`), from the
+    stage-1 winner, one dimension at a time: the two-phase walk's tree records
+    per round trip (`walk_front` 8 or 16) and the step kernel's neighbours per
+    pipeline batch (`pair_batch` 2, 4 or 8, FP32 only).  Returns (best spec,
+    [(leaf, walk, walk_front, pair_batch, ms_per_step)]); 0 in the tile fields
+    means the default (8 records, 4 neighbours)."""
     if candidates is None:
         candidates = [(a, w) for a in (8, 16, 32) for w in (0, 1)] + [(16, 3)]
     table = []
-    best = None
-    for apl, walk in candidates:
-        s2 = replace(spec, atomsperleaf=apl, list_walk=walk)
+
+    def measure(s2):
         b = BatchedSimulation(reps, s2, ctx=ctx)
         b.set_state()
         W = s2.rebuild_every
@@ -170,12 +174,34 @@ def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None):
         b.run(windows * W)
         ms = b.ctx.timer_stop() / (windows * W)
         b.close()
-        table.append((apl, walk, ms))
+        row = (s2.atomsperleaf, s2.list_walk, s2.walk_front, s2.pair_batch, ms)
+        table.append(row)
         if log:
-            log("autotune: leaf %d walk %d: %.4f ms/step" % (apl, walk, ms))
-        if best is None or ms < best[2]:
-            best = (apl, walk, ms)
-    return replace(spec, atomsperleaf=best[0], list_walk=best[1]), table
+            log("autotune: leaf %d walk %d front %d batch %d: %.4f ms/step" % row)
+        return ms
+
+    best, best_ms = None, None
+    for apl, walk in candidates:
+        s2 = replace(spec, atomsperleaf=apl, list_walk=walk)
+        ms = measure(s2)
+        if best is None or ms < best_ms:
+            best, best_ms = s2, ms
+    if tiles:
+        dims = []
+        if best.list_walk == 0:
+            dims.append(("walk_front", (8, 16)))
+        if best.precision == "fp32":
+            dims.append(("pair_batch", (2, 4, 8)))
+        for name, values in dims:
+            cur = getattr(best, name) or (8 if name == "walk_front" else 4)
+            for v in values:
+                if v == cur:
+                    continue
+                s2 = replace(best, **{name: v})
+                ms = measure(s2)
+                if ms < best_ms:
+                    best, best_ms = s2, ms
+    return best, table
 
 
 def run_one_per_replica(reps, spec, nsteps, ctx=None):

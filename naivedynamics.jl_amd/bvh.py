@@ -206,7 +206,8 @@ def tree_export(spec, position, ctx=None):
 
 
 def _pairs_with_capacity(call, what, ctx, cap):
-    """Runs call(out_ptr, cap, count_ref) under the capacity protocol."""
+    """Runs call(out_ptr, cap, count_ref) under the capacity protocol; returns
+    (DeviceArray of pairs, count), the result left on the device."""
     out = _pairbuf.get(ctx, cap)
     count = ctypes.c_int64(0)
     rc = call(out.ptr, out.shape[0], ctypes.byref(count))
@@ -214,14 +215,12 @@ def _pairs_with_capacity(call, what, ctx, cap):
         out = _pairbuf.get(ctx, count.value + count.value // 8 + 1024)
         rc = call(out.ptr, out.shape[0], ctypes.byref(count))
     check(rc, ctx, what, count.value)
-    return out.to_host(count.value)
+    return out, count.value
 
 
-def build_traverse_bvh(position, spec, morton=MORTON_MASKED, query=QUERY_OWN_LEAF, ctx=None):
-    """build_traverse_bvh(position, spec) (BVHTraverse.jl:1416-1421): TreeData's
-    +-r tree, then neighbor_traverse's point queries from each atom's own leaf
-    (query=QUERY_ROOT: the root-started walk of gpuneighbor_traverse!).  Pairs
-    (query, subject, d) with 0-based original indices, query-major."""
+def pointquery_device(position, spec, morton=MORTON_MASKED, query=QUERY_OWN_LEAF, ctx=None):
+    """Point-query list on device (nd_build_traverse_bvh); returns (DeviceArray
+    of pairs, count)."""
     ctx = _ctx(ctx)
     d, _ = _dev_xyz(ctx, position)
     if d.shape[0] != spec.atom_count:
@@ -231,6 +230,15 @@ def build_traverse_bvh(position, spec, morton=MORTON_MASKED, query=QUERY_OWN_LEA
         lambda o, cap, cnt: L.nd_build_traverse_bvh(ctx.handle, d.ptr, ctypes.byref(spec.c), query, morton,
                                                     o, cap, cnt),
         "build_traverse_bvh", ctx, max(1024, 48 * spec.atom_count))
+
+
+def build_traverse_bvh(position, spec, morton=MORTON_MASKED, query=QUERY_OWN_LEAF, ctx=None):
+    """build_traverse_bvh(position, spec) (BVHTraverse.jl:1416-1421): TreeData's
+    +-r tree, then neighbor_traverse's point queries from each atom's own leaf
+    (query=QUERY_ROOT: the root-started walk of gpuneighbor_traverse!).  Pairs
+    (query, subject, d) with 0-based original indices, query-major."""
+    out, n = pointquery_device(position, spec, morton, query, ctx)
+    return out.to_host(n)
 
 
 def tree_store(spec, ctx=None):
@@ -260,10 +268,11 @@ def neighbor_traverse(keys, positions, spec, query=QUERY_OWN_LEAF, ctx=None):
     x = DeviceArray.from_host(ctx, np.ascontiguousarray(positions["position"], np.float32).reshape(-1, 3))
     ix = DeviceArray.from_host(ctx, np.ascontiguousarray(positions["index"], np.int32))
     L = load()
-    return _pairs_with_capacity(
+    out, n = _pairs_with_capacity(
         lambda o, cap, cnt: L.nd_neighbor_traverse(ctx.handle, nodes.ptr, x.ptr, ix.ptr, ctypes.byref(spec.c),
                                                    query, o, cap, cnt),
         "neighbor_traverse", ctx, max(1024, 48 * spec.atom_count))
+    return out.to_host(n)
 
 
 def quantized_positions(position, ctx=None):
@@ -328,5 +337,6 @@ __all__ = ["SpheresBVHSpecs", "binwidth", "mortoncodes", "sortperm", "bounding_v
            "leafneighbor_traverse", "leafbuild_traverse_bvh", "simdbuild_traverse_bvh",
            "exptsimdbuild_traverse_bvh", "leafTreeData", "gpubvh_neighborlist", "TreeData",
            "build_traverse_bvh", "neighbor_traverse", "quantized_positions", "tree_store",
+           "pointquery_device",
            "threshold_unique_pairs", "neighborlist_device", "canonical", "tree_export",
            "CapacityError"]

@@ -314,7 +314,7 @@ struct PairAcc {
     }
 };
 
-constexpr int kPairBatch = 4;
+constexpr int kPairBatch = 4;  // neighbours per pipeline batch (FP32 may pick 2 or 8)
 // Rows of a 64-atom block of the 16-bit list: two entries per row for up to
 // cap entries, plus two rows of sentinel entries (the step kernel's last
 // batch may run up to three entries past the wave's longest list).
@@ -326,7 +326,7 @@ constexpr int32_t kBufWord3 = 0x00020000;
 // PosS: storage scalar of positions/velocities; CalcS: pair arithmetic;
 // AccS: per-atom accumulation.  FP32 = <float,float,float>,
 // MIXED = <double,float,double>, FP64 = <double,double,double>.
-template <class PosS, class CalcS, class AccS, int MODE, bool ENERGY>
+template <class PosS, class CalcS, class AccS, int MODE, bool ENERGY, int PB = kPairBatch>
 __global__ void __launch_bounds__(nd::kBlock)
 k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
        typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
@@ -424,7 +424,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
             acc.inv = P.fx_inv;
         }
         acc.init(P);
-        // Software pipeline over batches of kPairBatch neighbours: while
+        // Software pipeline over batches of PB neighbours: while
         // batch b's pairs are computed, the records of batch b + 1 and the
         // list entries of batch b + 2 are in flight, so one memory latency
         // (not a list load followed by a gather) separates consecutive
@@ -439,7 +439,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         auto pipeline = [&](auto one, auto entries, auto record, auto checked) {
             constexpr bool ONE = decltype(one)::value;
             constexpr bool CHK = decltype(checked)::value;
-            constexpr int B = kPairBatch;
+            constexpr int B = PB;
             int32_t ja[B], jb[B];
             G qa[B], qb[B];
             entries(0, ja);
@@ -474,9 +474,9 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                     (void *)(nbr16 + (int64_t)wave * cap2 * 64), 0, cap2 * 256, kBufWord3);
                 const int32_t last2 = kSentinel ? max((wmax + 1) >> 1, 1) + 1 : last >> 1;
                 auto entries = [&](int32_t r, int32_t *j) {
-                    static_assert(kPairBatch % 2 == 0, "two entries per word");
+                    static_assert(PB % 2 == 0, "two entries per word");
 #pragma unroll
-                    for (int u = 0; u < kPairBatch; u += 2) {
+                    for (int u = 0; u < PB; u += 2) {
                         const int32_t q = (r + u) >> 1;
                         const uint32_t row = (uint32_t)(q < last2 ? q : last2);
                         const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(l16, lane4, row * 256u, 0);
@@ -491,7 +491,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                     (void *)(nbr + (int64_t)wave * P.cap * 64), 0, P.cap * 256, kBufWord3);
                 auto entries = [&](int32_t r, int32_t *j) {
 #pragma unroll
-                    for (int u = 0; u < kPairBatch; ++u) {
+                    for (int u = 0; u < PB; ++u) {
                         const uint32_t row = (uint32_t)(r + u < last ? r + u : last);
                         const int32_t v = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(lrs, lane4, row * 256u, 0);
                         j[u] = r + u < cnt ? v : -1;
@@ -968,6 +968,7 @@ using nd::wave_lds_sync;
 // remain the frontier shrinks to one record (plain depth-first order), which
 // adds at most 3 entries per record level.  Queue entries pack
 // (leaf << 8) | bits, so leaves must number below 2^23.
+// kFront: records per phase-1 round trip (8, or 16 where the autotuner picks it)
 constexpr int kFront = 8, kStack2 = 256, kStackDfs = 128, kGroupQ = 128;
 constexpr int32_t kMaxLeaves3 = 1 << 23;
 
@@ -981,7 +982,7 @@ __host__ __device__ __forceinline__ int32_t rec_stride(int32_t cap) { return (ca
 // are contiguous (recs[i * rec_stride(cap) + k], k < nrecs[i]; blocks of 64 B
 // multiples, so a reader fetches two records per 16-byte load); each holds
 // at least one neighbour, so there are never more records than list entries.
-template <int A, bool REC = false>
+template <int A, bool REC = false, int F = kFront>
 __global__ void __launch_bounds__(nd::kBlock)
 k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
          float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
@@ -989,7 +990,7 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
     constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
     constexpr int kSlot = 4 * A + 4;     // floats per group slot (16 B pad: no bank conflicts)
-    constexpr int kBufF = 8 * kSlot > 32 * kFront ? 8 * kSlot : 32 * kFront;  // records: 8 float4 each
+    constexpr int kBufF = 8 * kSlot > 32 * F ? 8 * kSlot : 32 * F;  // records: 8 float4 each
     constexpr int kW = nd::kBlock / 64;
     __shared__ __align__(16) float s_buf[kW][kBufF];
     __shared__ int32_t s_stk[kW][kStack2];
@@ -1033,7 +1034,7 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     stk[0] = nL + 1;  // the root is internal (nL >= 2)
     wave_lds_sync();
     while (sp > 0 || nqu > 0) {
-        if (nqu > 0 && (sp == 0 || nqu > kGroupQ - 4 * kFront)) {
+        if (nqu > 0 && (sp == 0 || nqu > kGroupQ - 4 * F)) {
             // ---- phase 2: drain the group queues ----
             const int32_t len = __shfl(glen, grp, 64);
             int32_t steps = len;
@@ -1105,27 +1106,27 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
             wave_lds_sync();
             continue;
         }
-        // ---- phase 1: expand up to kFront records of the stack top ----
+        // ---- phase 1: expand up to F records of the stack top ----
         // (each record pushes at most 4 entries: k pops leave room for them)
         const int32_t room = (kStack2 - kStackDfs - sp) / 3;
-        int32_t k = sp < kFront ? sp : kFront;
+        int32_t k = sp < F ? sp : F;
         if (k > room) k = room > 1 ? room : 1;
         fetched += k;
         // more records than the tree has internal nodes, or a stack that could overflow
         if (fetched > nL || sp + 3 * k > kStack2) { bad = true; break; }
         sp -= k;
         {
-            // lane l loads float4 l & 7 of records (l >> 3) + 8 u, u < kFront / 8
-            float4 v[kFront / 8];
+            // lane l loads float4 l & 7 of records (l >> 3) + 8 u, u < F / 8
+            float4 v[F / 8];
 #pragma unroll
-            for (int u = 0; u < kFront / 8; ++u) {
+            for (int u = 0; u < F / 8; ++u) {
                 const int32_t slot = (lane >> 3) + 8 * u;
                 v[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (slot < k) v[u] = wide[8 * (int64_t)(stk[sp + slot] - nL - 1) + (lane & 7)];
             }
             wave_lds_sync();
 #pragma unroll
-            for (int u = 0; u < kFront / 8; ++u) rec[lane + 64 * u] = v[u];
+            for (int u = 0; u < F / 8; ++u) rec[lane + 64 * u] = v[u];
             wave_lds_sync();
         }
         const uint32_t nn = (uint32_t)(2 * nL - 1);  // valid node numbers are 1 .. 2 nL - 1
@@ -1646,6 +1647,8 @@ struct nd_sim {
     nd::DevBuf nbr16, tables, tcount;  // neighbourhood tables and 16-bit lists (k_table)
     nd::DevBuf recs, nrecs;   // the two-phase walk's (group, mask) records per atom
     int32_t lcap = 0;         // table capacity (records) per workgroup
+    int32_t pair_batch = kPairBatch;  // step kernel: neighbours per pipeline batch (FP32)
+    int32_t walk_front = kFront;      // two-phase walk: records per phase-1 round trip
     nd::CellGrid cg{};
     int32_t nrep = 0;  // packed replicas (0: one system)
     nd::SortBufs sort;
@@ -1870,12 +1873,23 @@ void launch_rebuild(nd_sim *s, bool full) {
         uint2 *rc = s->recs.as<uint2>();
         int32_t *nr = s->nrecs.as<int32_t>();
         if (two_phase) {
-            switch (s->apl) {
-                case 4: k_nlist3<4, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
-                case 8: k_nlist3<8, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
-                case 16: k_nlist3<16, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
-                default: k_nlist3<32, true><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr); break;
+#define ND_NLIST3(A, F) k_nlist3<A, true, F><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr)
+            if (s->walk_front == 16) {
+                switch (s->apl) {
+                    case 4: ND_NLIST3(4, 16); break;
+                    case 8: ND_NLIST3(8, 16); break;
+                    case 16: ND_NLIST3(16, 16); break;
+                    default: ND_NLIST3(32, 16); break;
+                }
+            } else {
+                switch (s->apl) {
+                    case 4: ND_NLIST3(4, 8); break;
+                    case 8: ND_NLIST3(8, 8); break;
+                    case 16: ND_NLIST3(16, 8); break;
+                    default: ND_NLIST3(32, 8); break;
+                }
             }
+#undef ND_NLIST3
             ND_HIP(hipGetLastError());
             k_table_rec<<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
                                                   entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
@@ -1964,8 +1978,8 @@ void launch_sort_and_build(nd_sim *s) {
 
 // check: the positions this launch drifts to will be used with the current
 // list (the next step is no rebuild step); a fused launch counts as a drift.
-template <class PosS, class CalcS, class AccS, int MODE, bool E>
-void launch_step_t(nd_sim *s, bool check) {
+template <class PosS, class CalcS, class AccS, int MODE, bool E, int PB>
+void launch_step_pb(nd_sim *s, bool check) {
     using P4 = typename V4<PosS>::T;
     hipStream_t st = s->ctx->stream;
     const unsigned g = nd::grid_for(s->n);
@@ -1976,18 +1990,29 @@ void launch_step_t(nd_sim *s, bool check) {
         // dynamic LDS past 64 KiB must be allowed per kernel (once)
         static bool allowed = false;
         if (!allowed) {
-            ND_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step<PosS, CalcS, AccS, MODE, E>),
+            ND_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step<PosS, CalcS, AccS, MODE, E, PB>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             allowed = true;
         }
     }
-    k_step<PosS, CalcS, AccS, MODE, E><<<g, nd::kBlock, lds, st>>>(
+    k_step<PosS, CalcS, AccS, MODE, E, PB><<<g, nd::kBlock, lds, st>>>(
         s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
         s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s), s->path.as<uint16_t>(),
         s->since == 0 ? 1 : 0, check ? 1 : 0, s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
         s->tcount.as<int32_t>(), s->lcap);
     if (MODE == kModeFused) s->since++;
+}
+
+// Pairs per pipeline batch: FP32 takes the simulation's pair_batch (2, 4 or
+// 8, picked by the batched form's autotuner); MIXED and FP64 use 4.
+template <class PosS, class CalcS, class AccS, int MODE, bool E>
+void launch_step_t(nd_sim *s, bool check) {
+    if constexpr (std::is_same_v<CalcS, float> && std::is_same_v<PosS, float>) {
+        if (s->pair_batch == 2) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 2>(s, check);
+        if (s->pair_batch == 8) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 8>(s, check);
+    }
+    launch_step_pb<PosS, CalcS, AccS, MODE, E, kPairBatch>(s, check);
 }
 
 template <int MODE, bool E>
@@ -2413,6 +2438,10 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     ND_REQUIRE(p->resort_every >= 0 && p->resort_every % p->rebuild_every == 0,
                "nd_sim_create: resort_every must be a multiple of rebuild_every (or 0)");
     ND_REQUIRE(p->list_walk >= 0 && p->list_walk <= 3, "nd_sim_create: list_walk must be 0, 1, 2 or 3");
+    ND_REQUIRE(p->pair_batch == 0 || p->pair_batch == 2 || p->pair_batch == 4 || p->pair_batch == 8,
+               "nd_sim_create: pair_batch must be 0, 2, 4 or 8");
+    ND_REQUIRE(p->walk_front == 0 || p->walk_front == 8 || p->walk_front == 16,
+               "nd_sim_create: walk_front must be 0, 8 or 16");
     ND_REQUIRE(p->list_walk != 3 || p->hi > p->lo, "nd_sim_create: the cell-list walk needs a box (hi > lo)");
     ND_REQUIRE(p->dt > 0.0f && p->rc >= 0.0f && p->rc_coul >= 0.0f && p->skin >= 0.0f,
                "nd_sim_create: dt > 0, cutoffs and skin >= 0");
@@ -2426,6 +2455,8 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
     try {
         s->ctx = ctx;
         s->p = *p;
+        if (p->pair_batch) s->pair_batch = p->pair_batch;
+        if (p->walk_front) s->walk_front = p->walk_front;
         s->n = p->n;
         s->apl = p->atomsperleaf;
         s->nL = p->n / p->atomsperleaf;

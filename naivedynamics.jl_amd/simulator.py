@@ -47,6 +47,8 @@ class SimSpec:
     record_every: int = 0
     resort_every: int = 0
     list_walk: int = 0
+    pair_batch: int = 0   # step kernel pipeline batch (0 = 4; FP32: 2, 4 or 8)
+    walk_front: int = 0   # two-phase walk records per round trip (0 = 8; or 16)
 
 
 @dataclass
@@ -77,6 +79,8 @@ class Simulation:
         p.max_neighbors = int(max_neighbors)
         p.resort_every = int(spec.resort_every)
         p.list_walk = int(spec.list_walk)
+        p.pair_batch = int(spec.pair_batch)
+        p.walk_front = int(spec.walk_front)
         p.eps, p.sigma, p.rc = spec.eps, spec.sigma, spec.rc
         p.k_coul, p.rc_coul = spec.k_coul, spec.rc_coul
         p.skin, p.dt = spec.skin, spec.stepwidth

@@ -73,6 +73,8 @@ def test_autotune_picks_a_measured_candidate(pkg):
     reps = pkg.batched.make_replicas([512, 1024, 700], seed=8)
     best, table = pkg.autotune(reps, _spec(pkg, "fp32"), candidates=[(8, 0), (16, 0), (16, 1), (16, 3)],
                                windows=1)
-    assert len(table) == 4
-    fastest = min(table, key=lambda t: t[2])
-    assert (best.atomsperleaf, best.list_walk) == fastest[:2]
+    # stage 1: the 4 candidates; stage 2: the other walk front (two-phase
+    # winner only) and the other two pair batches (FP32)
+    assert 4 + 2 <= len(table) <= 4 + 3
+    fastest = min(table, key=lambda t: t[4])
+    assert (best.atomsperleaf, best.list_walk, best.walk_front, best.pair_batch) == fastest[:4]

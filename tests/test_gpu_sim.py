@@ -114,6 +114,41 @@ def test_refit_windows_and_walks_vs_oracle(pkg, O, precision, xtol, resort_every
     sim.close()
 
 
+@pytest.mark.parametrize("pair_batch,walk_front", [(2, 8), (8, 8), (4, 16), (8, 16)])
+def test_tile_shapes_vs_oracle(pkg, O, pair_batch, walk_front):
+    """The autotuner's tile shapes: neighbours per pipeline batch (FP32 step
+    kernel) and tree records per round trip (two-phase walk), each run
+    against the FP64 oracle with the default shape's tolerances, and with the
+    same list of the initial positions (entries, longest list) as the default."""
+    x, v, q, L = _cfg(pkg, 4096, 7)
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 100,
+                               walls=True, lo=0.0, hi=L)
+    lists = []
+    for pb, wf in ((0, 0), (pair_batch, walk_front)):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", steps=100, resort_every=50, pair_batch=pb,
+                                           walk_front=wf))
+        sim.set_state(x, v, q)
+        sim.run(0)  # the list of the initial positions
+        st = sim.stats()
+        lists.append((st["list_entries"], st["max_neighbors"]))
+        sim.run(100)
+        xg, vg, fg = sim.get_state()
+        assert sim.stats()["rebuilds"] == 11
+        assert np.abs(xg - xr).max() < 2e-4
+        assert _rel(fg, fr) < 5e-4
+        e = sim.energies()
+        assert np.allclose(e[:, 1], E[:, 0], rtol=1e-4)
+        sim.close()
+    assert lists[0] == lists[1]
+
+
+def test_tile_shapes_rejected(pkg):
+    with pytest.raises(pkg._lib.SpecError):
+        pkg.Simulation(4096, _spec(pkg, 17.0, "fp32", pair_batch=3))
+    with pytest.raises(pkg._lib.SpecError):
+        pkg.Simulation(4096, _spec(pkg, 17.0, "fp32", walk_front=12))
+
+
 @pytest.mark.parametrize("apl", [2, 4, 8, 16, 32])
 def test_walks_give_the_same_list(pkg, apl):
     # 0: two-phase child-box walk (leaf sizes 4..32; others use walk 2),

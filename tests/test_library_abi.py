@@ -5,6 +5,7 @@ import ctypes
 import glob
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -32,11 +33,28 @@ def test_bindings_cover_every_declared_symbol(pkg):
     assert set(declared_symbols()) == set(pkg._lib.SIGNATURES)
 
 
-def test_struct_layouts(pkg):
+def test_struct_layouts(pkg, tmp_path):
+    """The ctypes mirrors agree with the C header field by field: a probe
+    compiled against include/naivedynamics.h prints every offset and size."""
     assert pkg.NODE_DTYPE.itemsize == 32 and pkg.PAIR_DTYPE.itemsize == 12
-    assert ctypes.sizeof(pkg._lib.SimParams) == 12 * 4 + 11 * 8
-    assert ctypes.sizeof(pkg._lib.AdjParams) == 6 * 4 + 10 * 8
-    assert ctypes.sizeof(pkg._lib.BvhSpec) == 20
+    structs = {"nd_sim_params": pkg._lib.SimParams, "nd_adj_params": pkg._lib.AdjParams,
+               "nd_bvh_spec": pkg._lib.BvhSpec, "nd_sim_stats": pkg._lib.SimStats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "naivedynamics.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                              text=True).stdout.splitlines())
+    for cname, cls in structs.items():
+        assert int(out[cname + " size"]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(out[cname + "." + f]) == getattr(cls, f).offset, (cname, f)
 
 
 def test_version(pkg):
