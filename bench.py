@@ -229,10 +229,10 @@ def batched_form(pkg, steps=200):
     sec = B.run_one_per_replica(reps, best, steps)
     atoms = int(sizes.sum())
     out = {"workload": "lj-replicas-512", "replicas": 512, "atoms": atoms, "steps": steps,
-           "autotune": [{"leaf": a, "walk": w, "walk_front": f or 8, "pair_batch": p or 4,
-                         "ms_per_step": round(t, 4)} for a, w, f, p, t in table],
+           "autotune": [{"leaf": a, "walk": w, "walk_front": f or 8, "pair_batch": p or 4, "atom_lanes": ln or 1,
+                         "ms_per_step": round(t, 4)} for a, w, f, p, ln, t in table],
            "chosen": {"leaf": best.atomsperleaf, "walk": best.list_walk, "walk_front": best.walk_front or 8,
-                      "pair_batch": best.pair_batch or 4},
+                      "pair_batch": best.pair_batch or 4, "atom_lanes": best.atom_lanes or 1},
            "packed_steps_per_s": round(steps / (ms / 1e3), 2),
            "packed_atom_steps_per_s": float("%.4e" % (atoms * steps / (ms / 1e3))),
            "one_per_replica_atom_steps_per_s": float("%.4e" % (atoms * steps / sec.sum())),

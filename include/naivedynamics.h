@@ -280,6 +280,8 @@ typedef struct {
                              8 (FP32; MIXED and FP64 use 4); 0 = 4 */
     int32_t walk_front;   /* two-phase walk: tree records fetched per round trip, 8 or
                              16; 0 = 8 */
+    int32_t atom_lanes;   /* step kernel: lanes per atom, 1 or 2 (two lanes split an
+                             atom's list and add their sums in a fixed order); 0 = 1 */
     double eps, sigma, rc; /* LJ */
     double k_coul, rc_coul;/* Coulomb (rc_coul = 0 disables) */
     double skin;           /* list radius = max(rc, rc_coul) + skin */

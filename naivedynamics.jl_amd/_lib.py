@@ -64,6 +64,7 @@ class SimParams(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("max_neighbors", ctypes.c_int32),
                 ("resort_every", ctypes.c_int32), ("list_walk", ctypes.c_int32),
                 ("pair_batch", ctypes.c_int32), ("walk_front", ctypes.c_int32),
+                ("atom_lanes", ctypes.c_int32),
                 ("eps", ctypes.c_double), ("sigma", ctypes.c_double), ("rc", ctypes.c_double),
                 ("k_coul", ctypes.c_double), ("rc_coul", ctypes.c_double),
                 ("skin", ctypes.c_double), ("dt", ctypes.c_double), ("lo", ctypes.c_double),

@@ -156,9 +156,10 @@ def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None, tiles=T
     (1) walk, and the cell list (3, no tree).  Stage 2 (`tiles`), from the
     stage-1 winner, one dimension at a time: the two-phase walk's tree records
     per round trip (`walk_front` 8 or 16) and the step kernel's neighbours per
-    pipeline batch (`pair_batch` 2, 4 or 8, FP32 only).  Returns (best spec,
-    [(leaf, walk, walk_front, pair_batch, ms_per_step)]); 0 in the tile fields
-    means the default (8 records, 4 neighbours)."""
+    pipeline batch (`pair_batch` 2, 4 or 8, FP32 only) and its lanes per atom
+    (`atom_lanes` 1 or 2).  Returns (best spec, [(leaf, walk, walk_front,
+    pair_batch, atom_lanes, ms_per_step)]); 0 in the shape fields means the
+    default (8 records, 4 neighbours, 1 lane)."""
     if candidates is None:
         candidates = [(a, w) for a in (8, 16, 32) for w in (0, 1)] + [(16, 3)]
     table = []
@@ -174,10 +175,10 @@ def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None, tiles=T
         b.run(windows * W)
         ms = b.ctx.timer_stop() / (windows * W)
         b.close()
-        row = (s2.atomsperleaf, s2.list_walk, s2.walk_front, s2.pair_batch, ms)
+        row = (s2.atomsperleaf, s2.list_walk, s2.walk_front, s2.pair_batch, s2.atom_lanes, ms)
         table.append(row)
         if log:
-            log("autotune: leaf %d walk %d front %d batch %d: %.4f ms/step" % row)
+            log("autotune: leaf %d walk %d front %d batch %d lanes %d: %.4f ms/step" % row)
         return ms
 
     best, best_ms = None, None
@@ -192,8 +193,10 @@ def autotune(reps, spec, candidates=None, windows=3, ctx=None, log=None, tiles=T
             dims.append(("walk_front", (8, 16)))
         if best.precision == "fp32":
             dims.append(("pair_batch", (2, 4, 8)))
+        dims.append(("atom_lanes", (1, 2)))
+        defaults = {"walk_front": 8, "pair_batch": 4, "atom_lanes": 1}
         for name, values in dims:
-            cur = getattr(best, name) or (8 if name == "walk_front" else 4)
+            cur = getattr(best, name) or defaults[name]
             for v in values:
                 if v == cur:
                     continue

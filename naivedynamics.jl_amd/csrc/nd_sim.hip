@@ -326,8 +326,15 @@ constexpr int32_t kBufWord3 = 0x00020000;
 // PosS: storage scalar of positions/velocities; CalcS: pair arithmetic;
 // AccS: per-atom accumulation.  FP32 = <float,float,float>,
 // MIXED = <double,float,double>, FP64 = <double,double,double>.
-template <class PosS, class CalcS, class AccS, int MODE, bool ENERGY, int PB = kPairBatch>
-__global__ void __launch_bounds__(nd::kBlock)
+// LPA (lanes per atom) 1: one lane walks an atom's whole list, 256 threads
+// per workgroup.  LPA 2: 512 threads for the same 256 atoms and table; waves
+// 2b and 2b + 1 take the 32 + 32 atoms of list block b, and lanes l and l + 32
+// of a wave split one atom's entries (entries 4m, 4m + 1 and 4m + 2, 4m + 3,
+// i.e. alternate 16-bit list rows), then add their sums in a fixed order:
+// twice the waves in flight for the same work (one lane per atom gives 16
+// waves per CU at 262,144 atoms).
+template <class PosS, class CalcS, class AccS, int MODE, bool ENERGY, int PB = kPairBatch, int LPA = 1>
+__global__ void __launch_bounds__(nd::kBlock * LPA)
 k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
        typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
        const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, StepParams P,
@@ -339,10 +346,14 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     // MIXED gathers the fixed-point copy (16 B per neighbour)
     constexpr bool FX = sizeof(PosS) == 8 && sizeof(CalcS) == 4;
     using G = std::conditional_t<FX, int4, P4>;
-    __shared__ double sh[nd::kBlock / 64][4];
+    constexpr int NT = nd::kBlock * LPA;  // threads per workgroup of 256 atoms
+    __shared__ double sh[NT / 64][4];
     extern __shared__ __align__(16) char s_table[];  // lcap neighbour records (G)
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const int32_t i = blk * blockDim.x + threadIdx.x;
+    const int32_t tid = threadIdx.x;
+    const int32_t half = LPA == 2 ? (tid >> 5) & 1 : 0;
+    const int32_t i = LPA == 2 ? blk * nd::kBlock + ((tid >> 7) << 6) + (((tid >> 6) & 1) << 5) + (tid & 31)
+                               : blk * nd::kBlock + tid;
     const int32_t n = P.n;
     AccS fx = 0, fy = 0, fz = 0;
     double pe = 0.0, vir = 0.0, ke = 0.0, np = 0.0;
@@ -350,6 +361,10 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     // the wave's longest list: rows past it are never needed, so the
     // look-ahead re-reads the last needed row (a cache hit) instead
     const int32_t cnt = i < n ? min(ncount[i], P.cap) : 0;
+    // this lane's share of the list: entries e with (e >> 1) & 1 == half
+    // (LPA 2; local entry k is e = 4 (k >> 1) + 2 half + (k & 1)), or all
+    const int32_t cl = LPA == 1 ? cnt : ((cnt >> 2) << 1) + min(max((cnt & 3) - 2 * half, 0), 2);
+    auto entry_of = [&](int32_t k) -> int32_t { return LPA == 1 ? k : ((k >> 1) << 2) + 2 * half + (k & 1); };
     int32_t wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
     wmax = __builtin_amdgcn_readfirstlane(wmax);
@@ -386,14 +401,14 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         // kStage index loads, then kStage gathers, then the LDS writes: two
         // memory latencies per kStage slots instead of two per slot
         constexpr int kStage = 8;
-        for (int32_t t0 = threadIdx.x; t0 < U; t0 += kStage * nd::kBlock) {
+        for (int32_t t0 = tid; t0 < U; t0 += kStage * NT) {
             // unconditional loads (the index clamped inside this table's row,
             // slots past U gather index -1: zeros, no memory access), so all
             // of them are in flight together
             int32_t j[kStage];
 #pragma unroll
             for (int u = 0; u < kStage; ++u) {
-                const int32_t t = t0 + u * nd::kBlock;
+                const int32_t t = t0 + u * NT;
                 const int32_t v = tab[min(t, lcap - 1)];
                 j[u] = t < U ? v : -1;
             }
@@ -402,11 +417,11 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
             for (int u = 0; u < kStage; ++u) g[u] = gather(j[u]);
 #pragma unroll
             for (int u = 0; u < kStage; ++u) {
-                const int32_t t = t0 + u * nd::kBlock;
+                const int32_t t = t0 + u * NT;
                 if (t < U) lt[t] = g[u];
             }
         }
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             G z = gather(-1);
             if constexpr (kSentinel) {
                 z.x = z.y = z.z = (PosS)1e18;
@@ -451,15 +466,15 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 for (int u = 0; u < B; ++u) qb[u] = record(jb[u]);
                 entries(k + 2 * B, ja);
 #pragma unroll
-                for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], CHK ? k + u < cnt : true);
-                if (k + B >= cnt) break;
+                for (int u = 0; u < B; ++u) acc.template add<ONE>(qa[u], CHK ? k + u < cl : true);
+                if (k + B >= cl) break;
 #pragma unroll
                 for (int u = 0; u < B; ++u) qa[u] = record(ja[u]);
                 entries(k + 3 * B, jb);
 #pragma unroll
-                for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], CHK ? k + B + u < cnt : true);
+                for (int u = 0; u < B; ++u) acc.template add<ONE>(qb[u], CHK ? k + B + u < cl : true);
                 acc.flush();  // MIXED: FP32 sums over 2 batches, then FP64
-                if (k + 2 * B >= cnt) break;
+                if (k + 2 * B >= cl) break;
             }
             acc.flush();  // after an exit at the first batch (a repeat adds zeros)
         };
@@ -477,7 +492,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                     static_assert(PB % 2 == 0, "two entries per word");
 #pragma unroll
                     for (int u = 0; u < PB; u += 2) {
-                        const int32_t q = (r + u) >> 1;
+                        const int32_t q = entry_of(r + u) >> 1;  // both entries of the row are this lane's
                         const uint32_t row = (uint32_t)(q < last2 ? q : last2);
                         const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(l16, lane4, row * 256u, 0);
                         j[u] = (int32_t)(v & 0xFFFFu);
@@ -492,9 +507,10 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                 auto entries = [&](int32_t r, int32_t *j) {
 #pragma unroll
                     for (int u = 0; u < PB; ++u) {
-                        const uint32_t row = (uint32_t)(r + u < last ? r + u : last);
+                        const int32_t e = entry_of(r + u);
+                        const uint32_t row = (uint32_t)(e < last ? e : last);
                         const int32_t v = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(lrs, lane4, row * 256u, 0);
-                        j[u] = r + u < cnt ? v : -1;
+                        j[u] = e < cnt ? v : -1;
                     }
                 };
                 pipeline(one, entries, gather, std::true_type{});
@@ -506,6 +522,13 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         fx = acc.fx;
         fy = acc.fy;
         fz = acc.fz;
+        if constexpr (LPA == 2) {
+            // half 0 adds half 1's sums (a fixed order: deterministic)
+            const AccS ox = __shfl_xor(fx, 32, 64), oy = __shfl_xor(fy, 32, 64), oz = __shfl_xor(fz, 32, 64);
+            fx = fx + ox;
+            fy = fy + oy;
+            fz = fz + oz;
+        }
         const AccS epair = acc.e, vpair = acc.w;
         const int32_t npair = acc.np;
         const P4 pi = acc.pi;
@@ -514,12 +537,15 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         vir = 0.5 * (double)vpair;
         np = 0.5 * (double)npair;
 
-        if (MODE == kModeForce || MODE == kModeFinal) {
+        if (half == 0 && (MODE == kModeForce || MODE == kModeFinal)) {
             P4 f;
             f.x = (PosS)fx; f.y = (PosS)fy; f.z = (PosS)fz; f.w = 0;
             frc[i] = f;
         }
-        if (MODE != kModeForce) {
+        if (half != 0) {
+            // the pair sums (pe, vir, np) of this half are kept; one lane per
+            // atom integrates
+        } else if (MODE != kModeForce) {
             P4 v = vel[i];
             const PosS h = (PosS)0.5 * (sizeof(PosS) == 4 ? (PosS)P.dtf : (PosS)P.dt) * v.w;
             v.x += h * (PosS)fx;
@@ -551,12 +577,12 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     if (ENERGY) {
         double acc[4] = {pe, vir, ke, np};
         block_sum4(acc, sh);
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) partials[4 * (int64_t)blockIdx.x + c] = acc[c];
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && tid == 0) {
         if (ENERGY) misc->partial_step = misc->step;
         if (MODE == kModeFused) misc->step += 1;
     }
@@ -1649,6 +1675,7 @@ struct nd_sim {
     int32_t lcap = 0;         // table capacity (records) per workgroup
     int32_t pair_batch = kPairBatch;  // step kernel: neighbours per pipeline batch (FP32)
     int32_t walk_front = kFront;      // two-phase walk: records per phase-1 round trip
+    int32_t atom_lanes = 1;           // step kernel: lanes per atom (1 or 2)
     nd::CellGrid cg{};
     int32_t nrep = 0;  // packed replicas (0: one system)
     nd::SortBufs sort;
@@ -1978,7 +2005,7 @@ void launch_sort_and_build(nd_sim *s) {
 
 // check: the positions this launch drifts to will be used with the current
 // list (the next step is no rebuild step); a fused launch counts as a drift.
-template <class PosS, class CalcS, class AccS, int MODE, bool E, int PB>
+template <class PosS, class CalcS, class AccS, int MODE, bool E, int PB, int LPA>
 void launch_step_pb(nd_sim *s, bool check) {
     using P4 = typename V4<PosS>::T;
     hipStream_t st = s->ctx->stream;
@@ -1990,12 +2017,12 @@ void launch_step_pb(nd_sim *s, bool check) {
         // dynamic LDS past 64 KiB must be allowed per kernel (once)
         static bool allowed = false;
         if (!allowed) {
-            ND_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step<PosS, CalcS, AccS, MODE, E, PB>),
+            ND_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step<PosS, CalcS, AccS, MODE, E, PB, LPA>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             allowed = true;
         }
     }
-    k_step<PosS, CalcS, AccS, MODE, E, PB><<<g, nd::kBlock, lds, st>>>(
+    k_step<PosS, CalcS, AccS, MODE, E, PB, LPA><<<g, nd::kBlock * LPA, lds, st>>>(
         s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
         s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s), s->path.as<uint16_t>(),
@@ -2009,10 +2036,17 @@ void launch_step_pb(nd_sim *s, bool check) {
 template <class PosS, class CalcS, class AccS, int MODE, bool E>
 void launch_step_t(nd_sim *s, bool check) {
     if constexpr (std::is_same_v<CalcS, float> && std::is_same_v<PosS, float>) {
-        if (s->pair_batch == 2) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 2>(s, check);
-        if (s->pair_batch == 8) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 8>(s, check);
+        if (s->atom_lanes == 2) {
+            if (s->pair_batch == 2) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 2, 2>(s, check);
+            if (s->pair_batch == 8) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 8, 2>(s, check);
+            return launch_step_pb<PosS, CalcS, AccS, MODE, E, kPairBatch, 2>(s, check);
+        }
+        if (s->pair_batch == 2) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 2, 1>(s, check);
+        if (s->pair_batch == 8) return launch_step_pb<PosS, CalcS, AccS, MODE, E, 8, 1>(s, check);
+    } else {
+        if (s->atom_lanes == 2) return launch_step_pb<PosS, CalcS, AccS, MODE, E, kPairBatch, 2>(s, check);
     }
-    launch_step_pb<PosS, CalcS, AccS, MODE, E, kPairBatch>(s, check);
+    launch_step_pb<PosS, CalcS, AccS, MODE, E, kPairBatch, 1>(s, check);
 }
 
 template <int MODE, bool E>
@@ -2442,6 +2476,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
                "nd_sim_create: pair_batch must be 0, 2, 4 or 8");
     ND_REQUIRE(p->walk_front == 0 || p->walk_front == 8 || p->walk_front == 16,
                "nd_sim_create: walk_front must be 0, 8 or 16");
+    ND_REQUIRE(p->atom_lanes >= 0 && p->atom_lanes <= 2, "nd_sim_create: atom_lanes must be 0, 1 or 2");
     ND_REQUIRE(p->list_walk != 3 || p->hi > p->lo, "nd_sim_create: the cell-list walk needs a box (hi > lo)");
     ND_REQUIRE(p->dt > 0.0f && p->rc >= 0.0f && p->rc_coul >= 0.0f && p->skin >= 0.0f,
                "nd_sim_create: dt > 0, cutoffs and skin >= 0");
@@ -2457,6 +2492,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->p = *p;
         if (p->pair_batch) s->pair_batch = p->pair_batch;
         if (p->walk_front) s->walk_front = p->walk_front;
+        if (p->atom_lanes) s->atom_lanes = p->atom_lanes;
         s->n = p->n;
         s->apl = p->atomsperleaf;
         s->nL = p->n / p->atomsperleaf;

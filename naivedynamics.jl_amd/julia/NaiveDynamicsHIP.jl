@@ -41,7 +41,7 @@ struct NDSimParams                 # nd_sim_params
     n::Int32; precision::Int32; morton_mode::Int32; atomsperleaf::Int32
     rebuild_every::Int32; walls::Int32; energy_every::Int32; thermostat_every::Int32
     use_graph::Int32; max_neighbors::Int32; resort_every::Int32; list_walk::Int32
-    pair_batch::Int32; walk_front::Int32
+    pair_batch::Int32; walk_front::Int32; atom_lanes::Int32
     eps::Float64; sigma::Float64; rc::Float64; k_coul::Float64; rc_coul::Float64
     skin::Float64; dt::Float64; lo::Float64; hi::Float64; temperature::Float64; vdamp::Float64
 end
@@ -240,7 +240,7 @@ function NaiveDynamics.simulate!(b::HIPBackend, sys, spec, clct; dt = 0.001, rc 
                                  skin = 0.3, rebuild_every = 10, precision = 0, lo = 0.0, hi = 1.0)
     n = length(sys.position)
     # key 2: Hilbert index (ND_MORTON_HILBERT), 16 atoms per leaf
-    p = Ref(NDSimParams(n, precision, 2, 16, rebuild_every, 1, 10, 0, 1, 0, 0, 0, 0, 0,
+    p = Ref(NDSimParams(n, precision, 2, 16, rebuild_every, 1, 10, 0, 1, 0, 0, 0, 0, 0, 0,
                         1.0, 1.0, rc, 1.0, rc_coul, skin, dt, lo, hi, 1.0, 1.0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(b, ccall((:nd_sim_create, libnaivehip), Cint, (Ptr{Cvoid}, Ref{NDSimParams}, Ref{Ptr{Cvoid}}), b.handle, p, h), "simulate!")

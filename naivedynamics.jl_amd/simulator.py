@@ -49,6 +49,7 @@ class SimSpec:
     list_walk: int = 0
     pair_batch: int = 0   # step kernel pipeline batch (0 = 4; FP32: 2, 4 or 8)
     walk_front: int = 0   # two-phase walk records per round trip (0 = 8; or 16)
+    atom_lanes: int = 0   # step kernel lanes per atom (0 = 1; or 2)
 
 
 @dataclass
@@ -81,6 +82,7 @@ class Simulation:
         p.list_walk = int(spec.list_walk)
         p.pair_batch = int(spec.pair_batch)
         p.walk_front = int(spec.walk_front)
+        p.atom_lanes = int(spec.atom_lanes)
         p.eps, p.sigma, p.rc = spec.eps, spec.sigma, spec.rc
         p.k_coul, p.rc_coul = spec.k_coul, spec.rc_coul
         p.skin, p.dt = spec.skin, spec.stepwidth

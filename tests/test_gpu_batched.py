@@ -74,7 +74,8 @@ def test_autotune_picks_a_measured_candidate(pkg):
     best, table = pkg.autotune(reps, _spec(pkg, "fp32"), candidates=[(8, 0), (16, 0), (16, 1), (16, 3)],
                                windows=1)
     # stage 1: the 4 candidates; stage 2: the other walk front (two-phase
-    # winner only) and the other two pair batches (FP32)
-    assert 4 + 2 <= len(table) <= 4 + 3
-    fastest = min(table, key=lambda t: t[4])
-    assert (best.atomsperleaf, best.list_walk, best.walk_front, best.pair_batch) == fastest[:4]
+    # winner only), the other two pair batches (FP32), two lanes per atom
+    assert 4 + 3 <= len(table) <= 4 + 4
+    fastest = min(table, key=lambda t: t[5])
+    assert (best.atomsperleaf, best.list_walk, best.walk_front, best.pair_batch,
+            best.atom_lanes) == fastest[:5]

@@ -142,11 +142,48 @@ def test_tile_shapes_vs_oracle(pkg, O, pair_batch, walk_front):
     assert lists[0] == lists[1]
 
 
+@pytest.mark.parametrize("precision,xtol,ftol", [("fp32", 2e-4, 5e-4), ("mixed", 1e-7, 1e-5),
+                                                 ("fp64", 1e-10, 1e-10)])
+def test_two_lanes_per_atom_vs_oracle(pkg, O, precision, xtol, ftol):
+    """Step kernel with two lanes per atom (each takes alternate 16-bit list
+    rows; half 0 adds half 1's sums): the trajectory against the FP64 oracle
+    with each precision's tolerances, refit windows included."""
+    x, v, q, L = _cfg(pkg, 4096, 7)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, precision, steps=100, resort_every=50, atom_lanes=2))
+    sim.set_state(x, v, q)
+    sim.run(100)
+    xg, vg, fg = sim.get_state()
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 100,
+                               walls=True, lo=0.0, hi=L)
+    assert np.abs(xg - xr).max() < xtol
+    assert _rel(fg, fr) < ftol
+    e = sim.energies()
+    ktol = {"fp32": 1e-4, "mixed": 1e-6, "fp64": 1e-10}[precision]
+    assert np.allclose(e[:, 1], E[:, 0], rtol=ktol)
+    assert np.allclose(e[:, 2], E[:, 1], rtol=ktol, atol=ktol * abs(E[0, 1]))
+    sim.close()
+
+
+def test_two_lanes_per_atom_deterministic(pkg):
+    x, v, q, L = _cfg(pkg, 16384, 9)
+    out = []
+    for _ in range(2):
+        sim = pkg.Simulation(len(x), _spec(pkg, L, "fp32", steps=30, atom_lanes=2))
+        sim.set_state(x, v, q)
+        sim.run(30)
+        out.append(sim.get_state())
+        sim.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 def test_tile_shapes_rejected(pkg):
     with pytest.raises(pkg._lib.SpecError):
         pkg.Simulation(4096, _spec(pkg, 17.0, "fp32", pair_batch=3))
     with pytest.raises(pkg._lib.SpecError):
         pkg.Simulation(4096, _spec(pkg, 17.0, "fp32", walk_front=12))
+    with pytest.raises(pkg._lib.SpecError):
+        pkg.Simulation(4096, _spec(pkg, 17.0, "fp32", atom_lanes=3))
 
 
 @pytest.mark.parametrize("apl", [2, 4, 8, 16, 32])
