@@ -397,19 +397,20 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     G *lt = reinterpret_cast<G *>(s_table);
     constexpr bool kSentinel = sizeof(G) == 32;
     if (U >= 0) {
-        const int32_t *tab = tables + (int64_t)blk * lcap;
         // kStage index loads, then kStage gathers, then the LDS writes: two
-        // memory latencies per kStage slots instead of two per slot
+        // memory latencies per kStage slots instead of two per slot.  The
+        // loads are unconditional, so all of them are in flight together: a
+        // slot past U reads past the buffer range of the table's used part
+        // (zeros, no memory access) and then gathers index -1 (the same).
+        const __amdgpu_buffer_rsrc_t trs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(tables + (int64_t)blk * lcap), 0, U * 4, kBufWord3);
         constexpr int kStage = 8;
         for (int32_t t0 = tid; t0 < U; t0 += kStage * NT) {
-            // unconditional loads (the index clamped inside this table's row,
-            // slots past U gather index -1: zeros, no memory access), so all
-            // of them are in flight together
             int32_t j[kStage];
 #pragma unroll
             for (int u = 0; u < kStage; ++u) {
                 const int32_t t = t0 + u * NT;
-                const int32_t v = tab[min(t, lcap - 1)];
+                const int32_t v = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(trs, (uint32_t)t * 4u, 0, 0);
                 j[u] = t < U ? v : -1;
             }
             G g[kStage];
