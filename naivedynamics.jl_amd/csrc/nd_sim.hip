@@ -396,6 +396,35 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     const int32_t U = __builtin_amdgcn_readfirstlane(tcount[blk]);
     G *lt = reinterpret_cast<G *>(s_table);
     constexpr bool kSentinel = sizeof(G) == 32;
+    // this lane's list: block i >> 6 (rows of 64 columns), column i & 63
+    const int32_t wave = __builtin_amdgcn_readfirstlane(i >> 6);
+    const uint32_t lane4 = (uint32_t)(i & 63) * 4u;
+    // 16-bit table positions, two per int32 row entry; rows up to the wave's
+    // last, then two sentinel rows, which only the unchecked (FP64) loop
+    // reads: the checked loops re-read the wave's last row for their
+    // look-ahead past it
+    const int32_t cap2 = rows16(P.cap);
+    const __amdgpu_buffer_rsrc_t l16 = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(nbr16 + (int64_t)wave * cap2 * 64), 0, cap2 * 256, kBufWord3);
+    const int32_t last2 = kSentinel ? max((wmax + 1) >> 1, 1) + 1 : last >> 1;
+    auto entries16 = [&](int32_t r, int32_t *j) {
+        static_assert(PB % 2 == 0, "two entries per word");
+#pragma unroll
+        for (int u = 0; u < PB; u += 2) {
+            const int32_t q = entry_of(r + u) >> 1;  // both entries of the row are this lane's
+            const uint32_t row = (uint32_t)(q < last2 ? q : last2);
+            const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(l16, lane4, row * 256u, 0);
+            j[u] = (int32_t)(v & 0xFFFFu);
+            j[u + 1] = (int32_t)(v >> 16);
+        }
+    };
+    // the first two batches of the lane's entries are in flight while the
+    // table is staged (waves whose block holds an atom: uniform per wave)
+    int32_t pja[PB], pjb[PB];
+    if (U >= 0 && (wave << 6) < n) {
+        entries16(0, pja);
+        entries16(PB, pjb);
+    }
     if (U >= 0) {
         // kStage index loads, then kStage gathers, then the LDS writes: two
         // memory latencies per kStage slots instead of two per slot.  The
@@ -449,19 +478,29 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         // index -1, past the end of the buffer, which the hardware range check
         // answers with zeros and no memory access; table lists: they read
         // table record 0.  The pair maths masks them either way.
-        const int32_t wave = __builtin_amdgcn_readfirstlane(i >> 6);
-        const uint32_t lane4 = (uint32_t)(i & 63) * 4u;
-        // Two register sets (a, b) take turns, so no batch is copied.
-        auto pipeline = [&](auto one, auto entries, auto record, auto checked) {
+        // Two register sets (a, b) take turns, so no batch is copied.  pa,
+        // pb: the first two batches of entries already loaded (or null).
+        auto pipeline = [&](auto one, auto entries, auto record, auto checked, const int32_t *pa,
+                            const int32_t *pb) {
             constexpr bool ONE = decltype(one)::value;
             constexpr bool CHK = decltype(checked)::value;
             constexpr int B = PB;
             int32_t ja[B], jb[B];
             G qa[B], qb[B];
-            entries(0, ja);
+            if (pa) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) ja[u] = pa[u];
+            } else {
+                entries(0, ja);
+            }
 #pragma unroll
             for (int u = 0; u < B; ++u) qa[u] = record(ja[u]);
-            entries(B, jb);
+            if (pb) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) jb[u] = pb[u];
+            } else {
+                entries(B, jb);
+            }
             for (int32_t k = 0;; k += 2 * B) {
 #pragma unroll
                 for (int u = 0; u < B; ++u) qb[u] = record(jb[u]);
@@ -481,27 +520,8 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
         };
         auto run = [&](auto one) {
             if (U >= 0) {
-                // 16-bit table positions, two per int32 row entry; rows up to
-                // the wave's last, then two sentinel rows, which only the
-                // unchecked (FP64) loop reads: the checked loops re-read the
-                // wave's last row for their look-ahead past it
-                const int32_t cap2 = rows16(P.cap);
-                const __amdgpu_buffer_rsrc_t l16 = __builtin_amdgcn_make_buffer_rsrc(
-                    (void *)(nbr16 + (int64_t)wave * cap2 * 64), 0, cap2 * 256, kBufWord3);
-                const int32_t last2 = kSentinel ? max((wmax + 1) >> 1, 1) + 1 : last >> 1;
-                auto entries = [&](int32_t r, int32_t *j) {
-                    static_assert(PB % 2 == 0, "two entries per word");
-#pragma unroll
-                    for (int u = 0; u < PB; u += 2) {
-                        const int32_t q = entry_of(r + u) >> 1;  // both entries of the row are this lane's
-                        const uint32_t row = (uint32_t)(q < last2 ? q : last2);
-                        const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(l16, lane4, row * 256u, 0);
-                        j[u] = (int32_t)(v & 0xFFFFu);
-                        j[u + 1] = (int32_t)(v >> 16);
-                    }
-                };
-                pipeline(one, entries, [&](int32_t j) -> G { return lt[j]; },
-                         std::integral_constant<bool, !kSentinel>{});
+                pipeline(one, entries16, [&](int32_t j) -> G { return lt[j]; },
+                         std::integral_constant<bool, !kSentinel>{}, pja, pjb);
             } else {
                 const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
                     (void *)(nbr + (int64_t)wave * P.cap * 64), 0, P.cap * 256, kBufWord3);
@@ -514,7 +534,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
                         j[u] = e < cnt ? v : -1;
                     }
                 };
-                pipeline(one, entries, gather, std::true_type{});
+                pipeline(one, entries, gather, std::true_type{}, nullptr, nullptr);
             }
         };
         // equal cutoffs (uniform over the launch): one mask per pair
