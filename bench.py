@@ -498,7 +498,7 @@ def main():
     flops_launch = FLOPS_PER_PAIR_LJC * pairs
     result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                          "traffic": measured_traffic(), "kernel": "k_step<float,float,float,1,0>",
+                          "traffic": measured_traffic(), "kernel": "k_step<float, float, float, 1, false, 4, 1>",
                           "avg_launch_ms": round(avg_ms, 5), "launches": launches,
                           "algorithmic_bytes_per_launch": int(bytes_launch),
                           "fp32_tflops": round(flops_launch / (avg_ms * 1e-3) / 1e12, 2),
