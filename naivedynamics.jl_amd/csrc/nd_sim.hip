@@ -999,7 +999,7 @@ using nd::wave_lds_sync;
 // numbers.  Phase 1 pops up to kFront records from the wave's LDS stack and
 // fetches them together (lane l loads float4 l & 7 of records (l >> 3) + 8 u),
 // so one memory latency covers up to kFront records (two tree levels each); every lane
-// tests its box against each entry, overlapped internal entries are pushed,
+// tests its sphere of radius r_list against each entry's box, overlapped internal entries are pushed,
 // and an overlapped leaf is queued for each group of 8 lanes (8 consecutive
 // atoms) with an overlapping lane, together with the group's 8 overlap bits.
 // Phase 2 drains the 8 group queues side by side: per step each group stages
@@ -1059,11 +1059,16 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
     uint32_t pend_g = ~0u, pend_b = 0u;    // REC: the record being built (leaves of one group merge)
     uint2 *myrec = recs + (REC ? (int64_t)i * rec_stride(cap) : 0);
     const float4 p = active ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    // lanes without an atom get an empty query box (it overlaps nothing)
-    const float qmn0 = active ? p.x - h : INFINITY, qmn1 = active ? p.y - h : INFINITY,
-                qmn2 = active ? p.z - h : INFINITY;
-    const float qmx0 = active ? p.x + h : -INFINITY, qmx1 = active ? p.y + h : -INFINITY,
-                qmx2 = active ? p.z + h : -INFINITY;
+    // A lane overlaps a node when its sphere of radius r_list meets the
+    // node's atoms' box (the stored box less the padding h = r_list / 2 on
+    // each side), not merely the padded box: at 32 atoms per leaf about a
+    // quarter fewer leaves are queued.  The distance to the unpadded box is
+    // under-estimated by a slack covering the rounding of the padding and of
+    // the differences (8 ulp of the largest magnitude involved) and compared
+    // with rl2 plus 8 ulp, so every leaf holding a neighbour still passes;
+    // lanes without an atom overlap nothing.
+    const float slack = (fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)) + 4.0f * h) * 0x1p-20f;
+    const float hc = h - slack, rlb2 = rl2 * (1.0f + 0x1p-20f);
     bool bad = false;
 
     int32_t glen = 0;  // lanes 0..7: length of queue `lane`
@@ -1181,8 +1186,10 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
             const int32_t c = __builtin_amdgcn_readfirstlane(__float_as_int(idf));
             if (c == 0) return;
             if ((uint32_t)(c - 1) >= nn) { bad = true; return; }
-            const bool ov = (qmn0 < mxx) & (qmx0 > mnx) & (qmn1 < mxy) & (qmx1 > mny) & (qmn2 < mxz) &
-                            (qmx2 > mnz);
+            const float ex = fmaxf(fmaxf(mnx - p.x, p.x - mxx), -hc) + hc;
+            const float ey = fmaxf(fmaxf(mny - p.y, p.y - mxy), -hc) + hc;
+            const float ez = fmaxf(fmaxf(mnz - p.z, p.z - mxz), -hc) + hc;
+            const bool ov = active & (ex * ex + ey * ey + ez * ez < rlb2);
             const uint64_t mk = __builtin_amdgcn_ballot_w64(ov);
             if (mk) {
                 if (c <= nL) enqueue(c, mk);

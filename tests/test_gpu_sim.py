@@ -186,6 +186,63 @@ def test_tile_shapes_rejected(pkg):
         pkg.Simulation(4096, _spec(pkg, 17.0, "fp32", atom_lanes=3))
 
 
+def _radius_pairs(L, rl2f, seed):
+    """Sites 12 apart, each with two 4-atom clusters: atom a0 is cluster A's
+    corner towards B along a diagonal (random signs) and b0 cluster B's
+    corner towards A, |a0 - b0| a few ulp inside or outside the list radius
+    in float32 (both with and without a fused multiply-add); returns the
+    positions and the number of boundary pairs inside."""
+    rng = np.random.default_rng(seed)
+    f = np.float32
+    xs, inside = [], 0
+    for k, site in enumerate(np.stack(np.meshgrid(*[np.arange(8)] * 3, indexing="ij"), -1).reshape(-1, 3)):
+        sg = rng.choice([-1.0, 1.0], 3)
+        a0 = (f(6.0) + f(12.0) * site.astype(f) + rng.random(3).astype(f)).astype(f)
+        want_in = k % 2 == 0
+        t = np.sqrt(float(rl2f) / 3.0)
+        b0 = (a0 + (sg * t).astype(f)).astype(f)
+        for _ in range(200):  # nudge b0 by ulps until s is on the wanted side in both forms
+            d = (a0 - b0).astype(f)
+            s1 = f(f(f(d[0] * d[0]) + f(d[1] * d[1])) + f(d[2] * d[2]))
+            s2 = f(np.float64(d[2]) * d[2] + np.float64(f(np.float64(d[1]) * d[1] + np.float64(f(d[0] * d[0])))))
+            ok = (s1 < rl2f and s2 < rl2f) if want_in else (s1 >= rl2f and s2 >= rl2f)
+            if ok:
+                break
+            ax = k % 3
+            b0[ax] = np.nextafter(b0[ax], (a0[ax] if want_in else f(1e9) * sg[ax] + a0[ax]), dtype=f)
+        assert ok
+        inside += want_in
+        offs = np.array([[0.0, 0.0, 0.0], [0.35, 0.1, 0.2], [0.15, 0.4, 0.05], [0.3, 0.3, 0.45]])
+        xs.append(a0 - (offs * sg).astype(f))
+        xs.append(b0 + (offs * sg).astype(f))
+    return np.concatenate(xs).astype(np.float64), inside
+
+
+@pytest.mark.parametrize("apl", [4, 32])
+def test_walk_keeps_pairs_at_the_list_radius(pkg, apl):
+    """The two-phase walk prunes nodes by the distance from the atom to the
+    node's unpadded box (a sphere test with a rounding slack): a neighbour
+    that is the nearest corner of its leaf's box, a few ulp inside the list
+    radius along a diagonal, is still listed, and one a few ulp outside is
+    not; the rope walk (box test) and the cell list give the same count."""
+    L = 100.0
+    rl = 2.5 + 0.3
+    rlist = np.float32(rl + (L + rl) * 9.5367431640625e-07)
+    rl2f = np.float32(rlist * rlist)
+    x, inside = _radius_pairs(L, rl2f, 17)
+    n = len(x)
+    # every other pair: the 6 within each cluster, none between sites
+    expect = 2 * (inside + 6 * (n // 4))
+    counts = []
+    for walk in (0, 1, 3):
+        sim = pkg.Simulation(n, _spec(pkg, L, "fp32", coul=False, list_walk=walk, atomsperleaf=apl))
+        sim.set_state(x, np.zeros_like(x), None)
+        sim.run(0)
+        counts.append(sim.stats()["list_entries"])
+        sim.close()
+    assert counts == [expect] * 3, (counts, expect)
+
+
 @pytest.mark.parametrize("apl", [2, 4, 8, 16, 32])
 def test_walks_give_the_same_list(pkg, apl):
     # 0: two-phase child-box walk (leaf sizes 4..32; others use walk 2),
