@@ -1412,11 +1412,13 @@ k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int
 // Tables from the two-phase walk's records (k_nlist3<A, true>): a record per
 // (atom, leaf with neighbours), ~9 per atom at density 0.8, instead of a list
 // entry per neighbour (~76), so the walk writes and this kernel reads ~4x
-// fewer bytes.  While the records go into the hash they are copied to LDS;
-// once the table is laid out each LDS record gets its group's table base and
-// mask in use, and every lane expands its own records into its list with one
-// LDS read per record and a population count per entry (a lane reading its
-// records from memory instead would stall its whole wave at each one).
+// fewer bytes.  While the records go into the hash they are copied to LDS
+// with their hash slot; once the table is laid out every lane expands its own
+// records into its list, reading per record the slot's table base and mask
+// in use from LDS and per entry a population count (a lane reading its
+// records from memory instead would stall its whole wave at each one).  The
+// LDS of a workgroup (36.9 KB) lets four of them share a CU, so the 1,024
+// workgroups of 262,144 atoms run in one round on 256 CUs.
 // A workgroup whose table does not fit (lcap) gets list rows of sorted
 // indices (nbr) instead, as does one whose records do not fit in LDS.
 constexpr int kRecLds = 3072;  // records per workgroup held in LDS
@@ -1428,8 +1430,7 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             int32_t *__restrict__ tcount, int32_t *__restrict__ nbr) {
     constexpr int kW = nd::kBlock / 64;
     __shared__ TableLds T;
-    __shared__ uint2 s_rec[kRecLds];     // (hash slot, then group base; mask)
-    __shared__ uint32_t s_gmask[kRecLds];  // the record's group mask in use
+    __shared__ uint2 s_rec[kRecLds];  // (hash slot, mask)
     __shared__ int32_t s_wsum[kW];
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1481,16 +1482,7 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
     const int32_t U = T.finish(lcap);  // ends with a barrier
     const bool lds_ok = in_lds && U >= 0;
     if (tid == 0) tcount[blk] = U;
-    if (lds_ok) {
-        T.write(tables + (int64_t)blk * lcap);
-        for (int32_t t = tid; t < rtot; t += nd::kBlock) {
-            const uint32_t h = s_rec[t].x;
-            s_rec[t].x = (uint32_t)T.base[h];
-            s_gmask[t] = T.mask[h];
-        }
-        __syncthreads();
-    }
-    if (U >= 0 && !in_lds) T.write(tables + (int64_t)blk * lcap);
+    if (U >= 0) T.write(tables + (int64_t)blk * lcap);
     if (((i >> 6) << 6) >= n) return;  // waves past the last atom own no list block
     int32_t wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
@@ -1504,9 +1496,9 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             if (bits == 0u) {
                 if (rk >= roff + nr) return sent;
                 const uint2 r = s_rec[rk];
-                gm = s_gmask[rk];
                 ++rk;
-                cb = (int32_t)r.x;
+                cb = T.base[r.x];
+                gm = T.mask[r.x];
                 bits = r.y;
             }
             const int32_t b = __builtin_ctz(bits);
