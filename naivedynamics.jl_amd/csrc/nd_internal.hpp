@@ -129,6 +129,10 @@ void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_strid
 void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *wide);
 // Four-wide records of the internal nodes (8 float4 per internal node).
 void launch_lbvh_wide4(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *rec);
+// launch_lbvh_refit followed by launch_lbvh_wide4, fused: leaf boxes, chunk
+// boxes and the records (internal node boxes in nodes are not updated).
+void launch_lbvh_refit_wide4(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride, int32_t nL,
+                             int32_t apl, float r, nd_node *nodes, int32_t *store, float4 *rec);
 // slots_cleared: the caller has already zeroed store[0 .. nL-2] (the CAS slots);
 // leaves_done: the caller has already written the leaf boxes (+- r/2).
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,
@@ -329,18 +333,21 @@ __device__ __forceinline__ uint32_t cell10(float x, float lo, float s) {
 
 }  // namespace nd
 
-// Julia's Float32 min/max (ordering with -0 < +0; NaN propagates).
+// Julia's Float32 min/max (ordering with -0 < +0; NaN propagates), branch
+// free: for non-NaN floats the integer key bits ^ ((bits >> 31) & 0x7FFFFFFF)
+// orders like the floats with -0 below +0, so one signed compare and a select
+// decide; an unordered pair gives x - y (a NaN), as Julia's min/max do.
+__device__ __forceinline__ int32_t jl_order_key(float x) {
+    const int32_t b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
 __device__ __forceinline__ float jl_min(float x, float y) {
-    if (__builtin_isnan(x) || __builtin_isnan(y)) return x - y;
-    if (x < y) return x;
-    if (y < x) return y;
-    return __builtin_signbit(x) ? x : y;
+    const float m = jl_order_key(y) < jl_order_key(x) ? y : x;
+    return __builtin_isunordered(x, y) ? x - y : m;
 }
 __device__ __forceinline__ float jl_max(float x, float y) {
-    if (__builtin_isnan(x) || __builtin_isnan(y)) return x - y;
-    if (x > y) return x;
-    if (y > x) return y;
-    return __builtin_signbit(x) ? y : x;
+    const float m = jl_order_key(y) > jl_order_key(x) ? y : x;
+    return __builtin_isunordered(x, y) ? x - y : m;
 }
 
 #define ND_API_BEGIN try {

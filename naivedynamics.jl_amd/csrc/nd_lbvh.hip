@@ -162,23 +162,17 @@ __device__ __forceinline__ float group_max(float v) {
     return v;
 }
 
-// Eight lanes per internal node: leaves before the first whole chunk, whole
-// chunks, leaves after the last whole chunk (at most 63 + nL/64 + 63 reads).
-__global__ void __launch_bounds__(nd::kBlock)
-k_lbvh_refit(nd_node *__restrict__ nodes, const int2 *__restrict__ ranges,
-             const float4 *__restrict__ chunks, int32_t nL, int32_t override_root) {
-    const int32_t q = 1 + (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kGroup);
-    const int lane = threadIdx.x & (kGroup - 1);
-    if (q > nL - 1) return;  // the whole 8-lane group exits together
-    const int2 rg = ranges[q - 1];
-    // only reachable with an incomplete topology (status already flagged)
-    if (rg.x < 1 || rg.y > nL || rg.x > rg.y) return;
+// Box of the leaves of rg (1-based, inclusive) over the kGroup lanes of an
+// aligned group (all of them call it): leaves before the first whole chunk,
+// whole chunks, leaves after the last whole chunk (at most 63 + nL/64 + 63
+// reads); every lane of the group returns the box.
+__device__ __forceinline__ Box6 range_box(const nd_node *__restrict__ leaves, const float4 *__restrict__ chunks,
+                                          int2 rg, int lane) {
     const int32_t l0 = rg.x - 1, r0 = rg.y - 1;           // 0-based inclusive leaf range
     const int32_t c0 = (l0 + kChunk - 1) / kChunk;         // first whole chunk
     const int32_t c1 = (r0 + 1) / kChunk;                  // one past the last whole chunk
     Box6 b;
     b.init();
-    const nd_node *leaves = nodes;
     if (c0 < c1) {
         for (int32_t l = l0 + lane; l < c0 * kChunk; l += kGroup) {
             const float4 *p = reinterpret_cast<const float4 *>(leaves + l);
@@ -195,8 +189,23 @@ k_lbvh_refit(nd_node *__restrict__ nodes, const int2 *__restrict__ ranges,
             b.merge(p[0], p[1]);
         }
     }
-    float mn0 = group_min(b.mn0), mn1 = group_min(b.mn1), mn2 = group_min(b.mn2);
-    float mx0 = group_max(b.mx0), mx1 = group_max(b.mx1), mx2 = group_max(b.mx2);
+    b.mn0 = group_min(b.mn0); b.mn1 = group_min(b.mn1); b.mn2 = group_min(b.mn2);
+    b.mx0 = group_max(b.mx0); b.mx1 = group_max(b.mx1); b.mx2 = group_max(b.mx2);
+    return b;
+}
+
+// Eight lanes per internal node.
+__global__ void __launch_bounds__(nd::kBlock)
+k_lbvh_refit(nd_node *__restrict__ nodes, const int2 *__restrict__ ranges,
+             const float4 *__restrict__ chunks, int32_t nL, int32_t override_root) {
+    const int32_t q = 1 + (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kGroup);
+    const int lane = threadIdx.x & (kGroup - 1);
+    if (q > nL - 1) return;  // the whole 8-lane group exits together
+    const int2 rg = ranges[q - 1];
+    // only reachable with an incomplete topology (status already flagged)
+    if (rg.x < 1 || rg.y > nL || rg.x > rg.y) return;
+    const Box6 b = range_box(nodes, chunks, rg, lane);
+    float mn0 = b.mn0, mn1 = b.mn1, mn2 = b.mn2, mx0 = b.mx0, mx1 = b.mx1, mx2 = b.mx2;
     if (lane == 0) {
         if (q == 1 && override_root) {  // BVHTraverse.jl:856-858
             mn0 = mn1 = mn2 = 0.0f;
@@ -335,6 +344,130 @@ k_lbvh_wide4(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__
     r[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Leaf boxes and the box of every aligned chunk of 64 leaves in one pass,
+// for a tree whose topology is kept: one workgroup per chunk (64 SEG atoms,
+// SEG = atoms per leaf, a power of two up to 64), each pass of 256 atoms
+// reduced per leaf by the butterfly of k_leaf_boxes_seg and over the chunk
+// through LDS.  Leaf and chunk boxes equal those of k_leaf_boxes_seg and
+// k_lbvh_chunks bit for bit (min/max are associative and commutative).
+template <int SEG>
+__global__ void __launch_bounds__(nd::kBlock)
+k_leaf_chunk_boxes(const float *__restrict__ xyz, int32_t stride, int32_t natoms, float h,
+                   nd_node *__restrict__ nodes, float4 *__restrict__ chunks) {
+    constexpr int kPer = kChunk * SEG;
+    constexpr int kPass = (kPer + nd::kBlock - 1) / nd::kBlock;
+    __shared__ float s_box[nd::kBlock / 64][6];
+    const int32_t c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const float inf = __int_as_float(0x7F800000);
+    float px[kPass], py[kPass], pz[kPass];
+    bool act[kPass];
+#pragma unroll
+    for (int ps = 0; ps < kPass; ++ps) {  // every load in flight before the first use
+        const int32_t o = ps * nd::kBlock + tid, a = c * kPer + o;
+        act[ps] = o < kPer && a < natoms;
+        // unconditional loads (lanes past the end read the last atom and are
+        // masked below), so that no branch separates them
+        const float *p = xyz + (int64_t)stride * (act[ps] ? a : natoms - 1);
+        px[ps] = p[0]; py[ps] = p[1]; pz[ps] = p[2];
+    }
+    Box6 cb;
+    cb.init();
+#pragma unroll
+    for (int ps = 0; ps < kPass; ++ps) {
+        const int32_t a = c * kPer + ps * nd::kBlock + tid;
+        float mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;
+        if (act[ps]) {
+            mn0 = px[ps] - h; mn1 = py[ps] - h; mn2 = pz[ps] - h;
+            mx0 = px[ps] + h; mx1 = py[ps] + h; mx2 = pz[ps] + h;
+        }
+#pragma unroll
+        for (int off = 1; off < SEG; off <<= 1) {
+            mn0 = jl_min(mn0, __shfl_xor(mn0, off, 64));
+            mn1 = jl_min(mn1, __shfl_xor(mn1, off, 64));
+            mn2 = jl_min(mn2, __shfl_xor(mn2, off, 64));
+            mx0 = jl_max(mx0, __shfl_xor(mx0, off, 64));
+            mx1 = jl_max(mx1, __shfl_xor(mx1, off, 64));
+            mx2 = jl_max(mx2, __shfl_xor(mx2, off, 64));
+        }
+        if (act[ps] && (a & (SEG - 1)) == 0) {
+            float *q = reinterpret_cast<float *>(nodes + a / SEG);
+            *reinterpret_cast<float4 *>(q) = make_float4(mn0, mn1, mn2, mx0);
+            *reinterpret_cast<float2 *>(q + 4) = make_float2(mx1, mx2);
+        }
+        cb.mn0 = jl_min(cb.mn0, mn0); cb.mn1 = jl_min(cb.mn1, mn1); cb.mn2 = jl_min(cb.mn2, mn2);
+        cb.mx0 = jl_max(cb.mx0, mx0); cb.mx1 = jl_max(cb.mx1, mx1); cb.mx2 = jl_max(cb.mx2, mx2);
+    }
+    cb.wave_reduce();
+    const int w = tid >> 6;
+    if ((tid & 63) == 0) {
+        s_box[w][0] = cb.mn0; s_box[w][1] = cb.mn1; s_box[w][2] = cb.mn2;
+        s_box[w][3] = cb.mx0; s_box[w][4] = cb.mx1; s_box[w][5] = cb.mx2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        Box6 b;
+        b.init();
+#pragma unroll
+        for (int k = 0; k < nd::kBlock / 64; ++k)
+            b.merge(make_float4(s_box[k][0], s_box[k][1], s_box[k][2], s_box[k][3]),
+                    make_float4(s_box[k][4], s_box[k][5], 0.0f, 0.0f));
+        chunks[2 * c] = make_float4(b.mn0, b.mn1, b.mn2, b.mx0);
+        chunks[2 * c + 1] = make_float4(b.mx1, b.mx2, 0.0f, 0.0f);
+    }
+}
+
+// The refit fused with the four-wide records, for a tree whose topology is
+// kept: 32 lanes per record, 8 per entry; an internal entry's box is the box
+// of its leaf range (range_box, as k_lbvh_refit computes it), a leaf entry's
+// its leaf box, an empty entry's the empty box.  The records equal those
+// k_lbvh_wide4 makes after k_lbvh_refit bit for bit; the internal nodes'
+// boxes in nodes are left as they were (the records are what the walk reads).
+__global__ void __launch_bounds__(nd::kBlock)
+k_refit_wide4(const nd_node *__restrict__ nodes, const int2 *__restrict__ ranges,
+              const float4 *__restrict__ chunks, int32_t nL, float4 *__restrict__ rec) {
+    const int32_t k = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 5);
+    const int e = (threadIdx.x >> 3) & 3, lane = threadIdx.x & (kGroup - 1);
+    if (k >= nL - 1) return;  // whole 32-lane groups exit together
+    int32_t id[4] = {0, 0, 0, 0};
+    int32_t m = 0;
+    const int32_t nn = 2 * nL - 1;
+    const int32_t L = nodes[nL + k].left;
+    const int32_t R = (L >= 1 && L <= nn) ? nodes[L - 1].skip : 0;
+    const int32_t ch[2] = {L, R};
+    for (int c = 0; c < 2; ++c) {
+        const int32_t x = ch[c];
+        if (x < 1 || x > nn) continue;  // malformed tree: the walk flags the empty record
+        if (x <= nL) {
+            id[m++] = x;
+        } else {
+            const int32_t g0 = nodes[x - 1].left;
+            const int32_t g1 = (g0 >= 1 && g0 <= nn) ? nodes[g0 - 1].skip : 0;
+            if (g0 >= 1 && g0 <= nn) id[m++] = g0;
+            if (g1 >= 1 && g1 <= nn) id[m++] = g1;
+        }
+    }
+    const int32_t x = id[e];
+    Box6 b;
+    b.init();
+    if (x > nL) {
+        const int2 rg = ranges[x - nL - 1];
+        // an invalid range only follows an incomplete topology (status flagged)
+        if (rg.x >= 1 && rg.y <= nL && rg.x <= rg.y) b = range_box(nodes, chunks, rg, lane);
+    } else if (x >= 1) {
+        const nd_node &q = nodes[x - 1];
+        b.mn0 = q.min[0]; b.mn1 = q.min[1]; b.mn2 = q.min[2];
+        b.mx0 = q.max[0]; b.mx1 = q.max[1]; b.mx2 = q.max[2];
+    }
+    if (lane == 0) {
+        float *r = reinterpret_cast<float *>(rec + 8 * (int64_t)k);
+        r[e] = b.mn0; r[4 + e] = b.mn1; r[8 + e] = b.mn2;
+        r[12 + e] = b.mx0; r[16 + e] = b.mx1; r[20 + e] = b.mx2;
+        r[24 + e] = __int_as_float(x);
+        r[28 + e] = 0.0f;
+    }
+}
+
 }  // namespace
 
 namespace nd {
@@ -372,6 +505,30 @@ void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_strid
     const int64_t nchunks = (nL + kChunk - 1) / kChunk;
     k_lbvh_chunks<<<grid_for(nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);
     k_lbvh_refit<<<grid_for((int64_t)(nL - 1) * kGroup), kBlock, 0, s>>>(nodes, ranges, chunks, nL, 0);
+    ND_HIP(hipGetLastError());
+}
+
+void launch_lbvh_refit_wide4(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride, int32_t nL,
+                             int32_t apl, float r, nd_node *nodes, int32_t *store, float4 *rec) {
+    const size_t off_r = ((size_t)nL + 3) & ~(size_t)3;
+    const size_t off_c = (off_r + 2 * (size_t)nL + 3) & ~(size_t)3;
+    const int2 *ranges = reinterpret_cast<const int2 *>(store + off_r);
+    float4 *chunks = reinterpret_cast<float4 *>(store + off_c);
+    const float h = r / 2.0f;
+    const unsigned nchunks = (unsigned)((nL + kChunk - 1) / kChunk);
+    const int32_t na = nL * apl;
+    switch (apl) {
+        case 2: k_leaf_chunk_boxes<2><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
+        case 4: k_leaf_chunk_boxes<4><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
+        case 8: k_leaf_chunk_boxes<8><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
+        case 16: k_leaf_chunk_boxes<16><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
+        case 32: k_leaf_chunk_boxes<32><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
+        case 64: k_leaf_chunk_boxes<64><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
+        default:
+            launch_leaf_boxes(s, xyz_sorted, xyz_stride, nL, apl, h, nodes);
+            k_lbvh_chunks<<<grid_for((int64_t)nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);
+    }
+    k_refit_wide4<<<grid_for((int64_t)(nL - 1) * 32), kBlock, 0, s>>>(nodes, ranges, chunks, nL, rec);
     ND_HIP(hipGetLastError());
 }
 

@@ -1245,6 +1245,13 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
 // consecutive in it.
 constexpr int kHashSlots = 1024;
 
+// The list walks' last kernel (a table build) also copies the stale flag the
+// integrators raise to pinned host memory, where the run loop looks for it
+// (one thread; the flag only goes from 0 to 1 between rebuilds).
+__device__ __forceinline__ void stale_to_host(const SimMisc *__restrict__ misc, volatile int32_t *hstale) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && misc->stale) *hstale = 1;
+}
+
 __device__ __forceinline__ uint32_t group_hash(int32_t g) {
     return ((uint32_t)g * 2654435761u) >> (32 - 10);  // kHashSlots = 2^10
 }
@@ -1340,7 +1347,8 @@ constexpr int kRowBatch = 16;
 __global__ void __launch_bounds__(nd::kBlock)
 k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int32_t n, int32_t cap,
         int32_t cap2, int32_t lcap, int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
-        int32_t *__restrict__ tcount) {
+        int32_t *__restrict__ tcount, const SimMisc *__restrict__ misc, volatile int32_t *hstale) {
+    stale_to_host(misc, hstale);
     __shared__ TableLds T;
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t lane = threadIdx.x & 63;
@@ -1427,7 +1435,9 @@ __global__ void __launch_bounds__(nd::kBlock)
 k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
             int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
-            int32_t *__restrict__ tcount, int32_t *__restrict__ nbr) {
+            int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, const SimMisc *__restrict__ misc,
+            volatile int32_t *hstale) {
+    stale_to_host(misc, hstale);
     constexpr int kW = nd::kBlock / 64;
     __shared__ TableLds T;
     __shared__ uint2 s_rec[kRecLds];  // (hash slot, mask)
@@ -1648,10 +1658,6 @@ k_pos32(const double4 *__restrict__ pos, int32_t n, float4 *__restrict__ out) {
 
 // Copies a raised stale flag to host memory (the first kernel of every
 // rebuild, so a run on the host sees a stale list a window or two later).
-__global__ void k_stale_to_host(const SimMisc *__restrict__ misc, volatile int32_t *hflag) {
-    if (threadIdx.x == 0 && misc->stale) *hflag = 1;
-}
-
 __global__ void k_misc_reset_counters(SimMisc *misc) {
     if (threadIdx.x == 0) { misc->overflow = 0; misc->status = 0; misc->maxcnt = 0; }
 }
@@ -1707,8 +1713,9 @@ struct nd_sim {
     int32_t since = 0;  // drifts launched since the last list build
     int64_t stale_reruns = 0, forced_rebuilds = 0;
     int64_t graph_recaptures = 0;  // graphs dropped because a buffer moved (expected 0)
-    // stale flag in pinned host memory (k_stale_to_host) and one event per
-    // window in flight, so a run can stop early when a list went stale
+    // stale flag in pinned host memory (written by the table kernel that ends
+    // each rebuild) and one event per window in flight, so a run can stop
+    // early when a list went stale
     volatile int32_t *hstale = nullptr;
     static constexpr int kInFlight = 3;
     hipEvent_t wev[kInFlight + 1] = {};
@@ -1853,7 +1860,8 @@ int32_t entry_shift(const nd_sim *) { return 0; }
 void launch_tables(nd_sim *s) {
     k_table<<<nd::grid_for(s->n), nd::kBlock, 0, s->ctx->stream>>>(
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->n, s->cap, rows16(s->cap), s->lcap, entry_shift(s),
-        s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(), s->tcount.as<int32_t>());
+        s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(), s->tcount.as<int32_t>(), s->misc.as<SimMisc>(),
+        s->hstale);
     ND_HIP(hipGetLastError());
 }
 
@@ -1862,7 +1870,6 @@ void launch_tables(nd_sim *s) {
 void launch_rebuild(nd_sim *s, bool full) {
     s->since = 0;  // the next drift starts every path length at 0
     hipStream_t st = s->ctx->stream;
-    k_stale_to_host<<<1, 64, 0, st>>>(s->misc.as<SimMisc>(), const_cast<int32_t *>(s->hstale));
     const int32_t n = s->n;
     const unsigned g = nd::grid_for(n);
     SimMisc *misc = s->misc.as<SimMisc>();
@@ -1887,6 +1894,9 @@ void launch_rebuild(nd_sim *s, bool full) {
         launch_tables(s);
         return;
     }
+    const bool two_phase = s->p.list_walk == 0 && s->nL < kMaxLeaves3 &&
+                           (s->apl == 4 || s->apl == 8 || s->apl == 16 || s->apl == 32);
+    bool records_done = false;  // the four-wide records already hold the refitted boxes
     if (full) {
         launch_sort_and_build(s);
         s->tree_valid = true;
@@ -1895,8 +1905,14 @@ void launch_rebuild(nd_sim *s, bool full) {
             k_pos32<<<g, nd::kBlock, 0, st>>>(s->pos[s->cur].as<double4>(), n, s->pos32.as<float4>());
         const float *p32 = s->wide ? reinterpret_cast<const float *>(s->pos32.p)
                                    : reinterpret_cast<const float *>(s->pos[s->cur].p);
-        nd::launch_lbvh_refit(st, p32, 4, s->nL, s->apl, s->rlist, s->nodes.as<nd_node>(),
-                              s->store.as<int32_t>());
+        if (two_phase) {
+            nd::launch_lbvh_refit_wide4(st, p32, 4, s->nL, s->apl, s->rlist, s->nodes.as<nd_node>(),
+                                        s->store.as<int32_t>(), s->kids.as<float4>());
+            records_done = true;
+        } else {
+            nd::launch_lbvh_refit(st, p32, 4, s->nL, s->apl, s->rlist, s->nodes.as<nd_node>(),
+                                  s->store.as<int32_t>());
+        }
     }
     const float4 *p4 = s->wide ? s->pos32.as<float4>() : s->pos[s->cur].as<float4>();
     const float hl = s->rlist / 2.0f, rl2 = s->rlist * s->rlist;
@@ -1913,10 +1929,11 @@ void launch_rebuild(nd_sim *s, bool full) {
         }
     } else {
         float4 *kids = s->kids.as<float4>();
-        const bool two_phase = s->p.list_walk == 0 && s->nL < kMaxLeaves3 &&
-                               (s->apl == 4 || s->apl == 8 || s->apl == 16 || s->apl == 32);
-        if (two_phase) nd::launch_lbvh_wide4(st, s->nodes.as<nd_node>(), s->nL, kids);
-        else nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
+        if (two_phase) {
+            if (!records_done) nd::launch_lbvh_wide4(st, s->nodes.as<nd_node>(), s->nL, kids);
+        } else {
+            nd::launch_lbvh_wide(st, s->nodes.as<nd_node>(), s->nL, kids);
+        }
         uint2 *rc = s->recs.as<uint2>();
         int32_t *nr = s->nrecs.as<int32_t>();
         if (two_phase) {
@@ -1940,7 +1957,7 @@ void launch_rebuild(nd_sim *s, bool full) {
             ND_HIP(hipGetLastError());
             k_table_rec<<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
                                                   entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
-                                                  s->tcount.as<int32_t>(), nb);
+                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale);
             ND_HIP(hipGetLastError());
             return;
         }
