@@ -346,74 +346,80 @@ k_lbvh_wide4(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__
 
 // Leaf boxes and the box of every aligned chunk of 64 leaves in one pass,
 // for a tree whose topology is kept: one workgroup per chunk (64 SEG atoms,
-// SEG = atoms per leaf, a power of two up to 64), each pass of 256 atoms
-// reduced per leaf by the butterfly of k_leaf_boxes_seg and over the chunk
-// through LDS.  Leaf and chunk boxes equal those of k_leaf_boxes_seg and
-// k_lbvh_chunks bit for bit (min/max are associative and commutative).
+// SEG = atoms per leaf, a power of two from 4 to 64), each lane reducing
+// SEG / 4 consecutive atoms, four lanes a leaf, the workgroup the chunk
+// (shuffles within the wave, then LDS).  Leaf and chunk boxes equal those of
+// k_leaf_boxes_seg and k_lbvh_chunks bit for bit (Julia's min/max are
+// associative and commutative).
 template <int SEG>
 __global__ void __launch_bounds__(nd::kBlock)
 k_leaf_chunk_boxes(const float *__restrict__ xyz, int32_t stride, int32_t natoms, float h,
                    nd_node *__restrict__ nodes, float4 *__restrict__ chunks) {
-    constexpr int kPer = kChunk * SEG;
-    constexpr int kPass = (kPer + nd::kBlock - 1) / nd::kBlock;
+    static_assert(SEG >= 4 && SEG <= 64 && (SEG & (SEG - 1)) == 0, "leaf size");
+    static_assert(kChunk * 4 == nd::kBlock, "four lanes per leaf, one workgroup per chunk");
+    constexpr int kA = SEG / 4;  // atoms per lane
     __shared__ float s_box[nd::kBlock / 64][6];
     const int32_t c = blockIdx.x;
     const int tid = threadIdx.x;
     const float inf = __int_as_float(0x7F800000);
-    float px[kPass], py[kPass], pz[kPass];
-    bool act[kPass];
+    const int32_t a0 = c * (kChunk * SEG) + tid * kA;
+    float px[kA], py[kA], pz[kA];
 #pragma unroll
-    for (int ps = 0; ps < kPass; ++ps) {  // every load in flight before the first use
-        const int32_t o = ps * nd::kBlock + tid, a = c * kPer + o;
-        act[ps] = o < kPer && a < natoms;
-        // unconditional loads (lanes past the end read the last atom and are
-        // masked below), so that no branch separates them
-        const float *p = xyz + (int64_t)stride * (act[ps] ? a : natoms - 1);
-        px[ps] = p[0]; py[ps] = p[1]; pz[ps] = p[2];
+    for (int u = 0; u < kA; ++u) {  // every load in flight before the first use
+        // lanes past the end read the last atom and are masked below
+        const int32_t a = a0 + u < natoms ? a0 + u : natoms - 1;
+        const float *p = xyz + (int64_t)stride * a;
+        px[u] = p[0]; py[u] = p[1]; pz[u] = p[2];
     }
-    Box6 cb;
-    cb.init();
+    Box6 b;
+    b.init();
 #pragma unroll
-    for (int ps = 0; ps < kPass; ++ps) {
-        const int32_t a = c * kPer + ps * nd::kBlock + tid;
-        float mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;
-        if (act[ps]) {
-            mn0 = px[ps] - h; mn1 = py[ps] - h; mn2 = pz[ps] - h;
-            mx0 = px[ps] + h; mx1 = py[ps] + h; mx2 = pz[ps] + h;
+    for (int u = 0; u < kA; ++u) {
+        if (a0 + u < natoms) {
+            b.mn0 = jl_min(b.mn0, px[u] - h); b.mn1 = jl_min(b.mn1, py[u] - h); b.mn2 = jl_min(b.mn2, pz[u] - h);
+            b.mx0 = jl_max(b.mx0, px[u] + h); b.mx1 = jl_max(b.mx1, py[u] + h); b.mx2 = jl_max(b.mx2, pz[u] + h);
         }
-#pragma unroll
-        for (int off = 1; off < SEG; off <<= 1) {
-            mn0 = jl_min(mn0, __shfl_xor(mn0, off, 64));
-            mn1 = jl_min(mn1, __shfl_xor(mn1, off, 64));
-            mn2 = jl_min(mn2, __shfl_xor(mn2, off, 64));
-            mx0 = jl_max(mx0, __shfl_xor(mx0, off, 64));
-            mx1 = jl_max(mx1, __shfl_xor(mx1, off, 64));
-            mx2 = jl_max(mx2, __shfl_xor(mx2, off, 64));
-        }
-        if (act[ps] && (a & (SEG - 1)) == 0) {
-            float *q = reinterpret_cast<float *>(nodes + a / SEG);
-            *reinterpret_cast<float4 *>(q) = make_float4(mn0, mn1, mn2, mx0);
-            *reinterpret_cast<float2 *>(q + 4) = make_float2(mx1, mx2);
-        }
-        cb.mn0 = jl_min(cb.mn0, mn0); cb.mn1 = jl_min(cb.mn1, mn1); cb.mn2 = jl_min(cb.mn2, mn2);
-        cb.mx0 = jl_max(cb.mx0, mx0); cb.mx1 = jl_max(cb.mx1, mx1); cb.mx2 = jl_max(cb.mx2, mx2);
     }
-    cb.wave_reduce();
+    // the leaf: four lanes
+#pragma unroll
+    for (int off = 1; off < 4; off <<= 1) {
+        b.mn0 = jl_min(b.mn0, __shfl_xor(b.mn0, off, 64));
+        b.mn1 = jl_min(b.mn1, __shfl_xor(b.mn1, off, 64));
+        b.mn2 = jl_min(b.mn2, __shfl_xor(b.mn2, off, 64));
+        b.mx0 = jl_max(b.mx0, __shfl_xor(b.mx0, off, 64));
+        b.mx1 = jl_max(b.mx1, __shfl_xor(b.mx1, off, 64));
+        b.mx2 = jl_max(b.mx2, __shfl_xor(b.mx2, off, 64));
+    }
+    if ((tid & 3) == 0 && a0 < natoms) {
+        float *q = reinterpret_cast<float *>(nodes + a0 / SEG);
+        *reinterpret_cast<float4 *>(q) = make_float4(b.mn0, b.mn1, b.mn2, b.mx0);
+        *reinterpret_cast<float2 *>(q + 4) = make_float2(b.mx1, b.mx2);
+    }
+    // the chunk: the rest of the wave, then the four waves
+#pragma unroll
+    for (int off = 4; off < 64; off <<= 1) {
+        b.mn0 = jl_min(b.mn0, __shfl_xor(b.mn0, off, 64));
+        b.mn1 = jl_min(b.mn1, __shfl_xor(b.mn1, off, 64));
+        b.mn2 = jl_min(b.mn2, __shfl_xor(b.mn2, off, 64));
+        b.mx0 = jl_max(b.mx0, __shfl_xor(b.mx0, off, 64));
+        b.mx1 = jl_max(b.mx1, __shfl_xor(b.mx1, off, 64));
+        b.mx2 = jl_max(b.mx2, __shfl_xor(b.mx2, off, 64));
+    }
     const int w = tid >> 6;
     if ((tid & 63) == 0) {
-        s_box[w][0] = cb.mn0; s_box[w][1] = cb.mn1; s_box[w][2] = cb.mn2;
-        s_box[w][3] = cb.mx0; s_box[w][4] = cb.mx1; s_box[w][5] = cb.mx2;
+        s_box[w][0] = b.mn0; s_box[w][1] = b.mn1; s_box[w][2] = b.mn2;
+        s_box[w][3] = b.mx0; s_box[w][4] = b.mx1; s_box[w][5] = b.mx2;
     }
     __syncthreads();
     if (tid == 0) {
-        Box6 b;
-        b.init();
+        Box6 cb;
+        cb.init();
 #pragma unroll
         for (int k = 0; k < nd::kBlock / 64; ++k)
-            b.merge(make_float4(s_box[k][0], s_box[k][1], s_box[k][2], s_box[k][3]),
-                    make_float4(s_box[k][4], s_box[k][5], 0.0f, 0.0f));
-        chunks[2 * c] = make_float4(b.mn0, b.mn1, b.mn2, b.mx0);
-        chunks[2 * c + 1] = make_float4(b.mx1, b.mx2, 0.0f, 0.0f);
+            cb.merge(make_float4(s_box[k][0], s_box[k][1], s_box[k][2], s_box[k][3]),
+                     make_float4(s_box[k][4], s_box[k][5], 0.0f, 0.0f));
+        chunks[2 * c] = make_float4(cb.mn0, cb.mn1, cb.mn2, cb.mx0);
+        chunks[2 * c + 1] = make_float4(cb.mx1, cb.mx2, 0.0f, 0.0f);
     }
 }
 
@@ -518,7 +524,6 @@ void launch_lbvh_refit_wide4(hipStream_t s, const float *xyz_sorted, int32_t xyz
     const unsigned nchunks = (unsigned)((nL + kChunk - 1) / kChunk);
     const int32_t na = nL * apl;
     switch (apl) {
-        case 2: k_leaf_chunk_boxes<2><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
         case 4: k_leaf_chunk_boxes<4><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
         case 8: k_leaf_chunk_boxes<8><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
         case 16: k_leaf_chunk_boxes<16><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;

@@ -114,6 +114,25 @@ def test_refit_windows_and_walks_vs_oracle(pkg, O, precision, xtol, resort_every
     sim.close()
 
 
+@pytest.mark.parametrize("apl", [8, 16, 32])
+def test_refit_window_leaf_sizes_vs_oracle(pkg, O, apl):
+    """Refit windows of the two-phase walk (leaf and chunk boxes in one
+    kernel, the refit fused with the four-wide records) at every leaf size it
+    takes: the FP64 trajectory stays within 1e-10 of the oracle's, which a
+    single missing pair would break."""
+    x, v, q, L = _cfg(pkg, 4096, 7)
+    sim = pkg.Simulation(len(x), _spec(pkg, L, "fp64", steps=60, resort_every=50, list_walk=0,
+                                       atomsperleaf=apl))
+    sim.set_state(x, v, q)
+    sim.run(60)
+    xg, vg, fg = sim.get_state()
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 60,
+                               walls=True, lo=0.0, hi=L)
+    assert np.abs(xg - xr).max() < 1e-10
+    assert _rel(fg, fr) < 1e-10
+    sim.close()
+
+
 @pytest.mark.parametrize("pair_batch,walk_front", [(2, 8), (8, 8), (4, 16), (8, 16)])
 def test_tile_shapes_vs_oracle(pkg, O, pair_batch, walk_front):
     """The autotuner's tile shapes: neighbours per pipeline batch (FP32 step
