@@ -1313,18 +1313,32 @@ struct TableLds {
         __syncthreads();
         return bad ? -1 : U;
     }
-    // the table: sorted indices of the groups' atoms in use, in slot order
+    // the table: sorted indices of the groups' atoms in use, in slot order.
+    // A wave writes the runs of its lanes' non-empty slots two at a time,
+    // a half-wave per run (lane b of the half writes the entry of bit b), so
+    // the stores of a run are consecutive rather than one lane's serial loop.
     __device__ void write(int32_t *tab) const {
         constexpr int kPer = kHashSlots / nd::kBlock;
+        const int lane = threadIdx.x & 63, bit = lane & 31;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int32_t sl = kPer * threadIdx.x + k;
-            uint32_t m = mask[sl];
-            int32_t b = base[sl];
-            const int32_t g32 = key[sl] << 5;
-            while (m) {
-                tab[b++] = g32 + __builtin_ctz(m);
-                m &= m - 1u;
+            const uint32_t m = mask[sl];
+            const int32_t b = base[sl], g32 = key[sl] << 5;
+            uint64_t todo = __builtin_amdgcn_ballot_w64(m != 0u);
+            while (todo) {  // wave-uniform
+                const int s0 = (int)__builtin_ctzll(todo);
+                todo &= todo - 1u;
+                const int s1 = todo ? (int)__builtin_ctzll(todo) : s0;
+                const bool two = todo != 0u;
+                todo &= todo - 1u;
+                const uint32_t m0 = (uint32_t)__builtin_amdgcn_readlane((int)m, s0);
+                const uint32_t m1 = two ? (uint32_t)__builtin_amdgcn_readlane((int)m, s1) : 0u;
+                const int32_t b0 = __builtin_amdgcn_readlane(b, s0), b1 = __builtin_amdgcn_readlane(b, s1);
+                const int32_t g0 = __builtin_amdgcn_readlane(g32, s0), g1 = __builtin_amdgcn_readlane(g32, s1);
+                const uint32_t mm = lane < 32 ? m0 : m1;
+                if ((mm >> bit) & 1u)
+                    tab[(lane < 32 ? b0 : b1) + __popc(mm & ((1u << bit) - 1u))] = (lane < 32 ? g0 : g1) + bit;
             }
         }
     }
