@@ -305,6 +305,11 @@ typedef struct {
                                  have served) */
     int64_t graph_recaptures; /* window graphs captured again because a buffer they use had
                                  moved since capture (a guard: expected 0) */
+    int32_t table_spills;     /* table builds' workgroups (summed over builds) whose
+                                 records exceeded the LDS record buffer (expanded from
+                                 memory instead) */
+    int32_t table_records_big; /* 1: table builds hold up to 5,120 records per workgroup in
+                                  LDS (chosen when the first checked build spilled), else 3,072 */
 } nd_sim_stats;
 
 typedef struct nd_sim nd_sim;

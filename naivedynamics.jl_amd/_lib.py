@@ -86,7 +86,8 @@ class SimStats(ctypes.Structure):
                 ("list_entries", ctypes.c_int64), ("max_neighbors", ctypes.c_int32),
                 ("capacity", ctypes.c_int32), ("pairs_in_cutoff", ctypes.c_int64),
                 ("stale_reruns", ctypes.c_int64), ("forced_rebuilds", ctypes.c_int64),
-                ("graph_recaptures", ctypes.c_int64)]
+                ("graph_recaptures", ctypes.c_int64), ("table_spills", ctypes.c_int32),
+                ("table_records_big", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p

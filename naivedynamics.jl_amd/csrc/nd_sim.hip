@@ -55,7 +55,9 @@ struct SimMisc {
     // set by a drift when an atom's path length since the last list build
     // reaches half the skin while the list is still to be used (see path_add)
     int32_t stale;
-    int32_t pad0;
+    // workgroups of a table build whose records did not fit in LDS (the
+    // slower global-records path); never reset, read at a checked rebuild
+    int32_t rec_spill;
     double pad[2];
 };
 static_assert(offsetof(SimMisc, maxcnt) == 0 && offsetof(SimMisc, overflow) == 4 &&
@@ -1442,19 +1444,24 @@ k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int
 // LDS of a workgroup (36.9 KB) lets four of them share a CU, so the 1,024
 // workgroups of 262,144 atoms run in one round on 256 CUs.
 // A workgroup whose table does not fit (lcap) gets list rows of sorted
-// indices (nbr) instead, as does one whose records do not fit in LDS.
-constexpr int kRecLds = 3072;  // records per workgroup held in LDS
+// indices (nbr) instead; one whose records do not fit in LDS (RL records)
+// expands them from memory.  RL = 3072 (36.9 KB of LDS: four workgroups per
+// CU) unless the first checked build of a simulation saw more than 1/64 of
+// its workgroups spill, as on dense blobs (about 13 records per atom), which
+// then take RL = 5120 (53.3 KB: three per CU).
+constexpr int kRecLds = 3072, kRecLdsBig = 5120;
 
+template <int RL>
 __global__ void __launch_bounds__(nd::kBlock)
 k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
             int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
-            int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, const SimMisc *__restrict__ misc,
+            int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, SimMisc *__restrict__ misc,
             volatile int32_t *hstale) {
     stale_to_host(misc, hstale);
     constexpr int kW = nd::kBlock / 64;
     __shared__ TableLds T;
-    __shared__ uint2 s_rec[kRecLds];  // (hash slot, mask)
+    __shared__ uint2 s_rec[RL];  // (hash slot, mask)
     __shared__ int32_t s_wsum[kW];
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1478,7 +1485,8 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
         roff += k < w ? s_wsum[k] : 0;
         rtot += s_wsum[k];
     }
-    const bool in_lds = rtot <= kRecLds;  // uniform over the workgroup
+    const bool in_lds = rtot <= RL;  // uniform over the workgroup
+    if (!in_lds && tid == 0) atomicAdd(&misc->rec_spill, 1);
     // records are fetched kRecBatch at a time, two per 16-byte load (the
     // atom's block is 64-byte aligned; slots past nr are read and ignored)
     constexpr int kRecBatch = 8;
@@ -1736,6 +1744,9 @@ struct nd_sim {
     bool have_force = false, list_valid = false, have_state = false;
     // the tree in nodes/store matches the current atom order (a refit may reuse it)
     bool tree_valid = false;
+    // table builds hold up to kRecLdsBig records per workgroup in LDS (set
+    // when a checked build saw many workgroups spill at kRecLds)
+    bool rec_big = false;
     float rlist = 0.0f;
     StepParams sp{};
     // graphs of one rebuild window, per buffer parity (bits 0-1) and kind
@@ -1969,7 +1980,12 @@ void launch_rebuild(nd_sim *s, bool full) {
             }
 #undef ND_NLIST3
             ND_HIP(hipGetLastError());
-            k_table_rec<<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
+            if (s->rec_big)
+                k_table_rec<kRecLdsBig><<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
+                                                  entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
+                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale);
+            else
+                k_table_rec<kRecLds><<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
                                                   entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
                                                   s->tcount.as<int32_t>(), nb, misc, s->hstale);
             ND_HIP(hipGetLastError());
@@ -2196,6 +2212,12 @@ void rebuild_checked(nd_sim *s) {
         if (m.status & nd::kStatusTree)
             throw nd::Error(ND_ERR_STATE, "simulation: malformed neighbour tree");
         s->last_maxcnt = m.maxcnt;
+        if (!s->rec_big && (int64_t)m.rec_spill * 64 > (int64_t)nd::grid_for(s->n)) {
+            // later builds hold more records per workgroup in LDS; captured
+            // windows used the smaller form
+            s->rec_big = true;
+            drop_graphs(s);
+        }
         if (!m.overflow) {
             s->list_valid = true;
             s->rebuilds++;
@@ -2874,6 +2896,10 @@ int nd_sim_stats_get(nd_sim *s, nd_sim_stats *out) {
     out->stale_reruns = s->stale_reruns;
     out->forced_rebuilds = s->forced_rebuilds;
     out->graph_recaptures = s->graph_recaptures;
+    SimMisc m;
+    read_misc(s, m);
+    out->table_spills = m.rec_spill;
+    out->table_records_big = s->rec_big ? 1 : 0;
     ND_API_END(ctx)
 }
 

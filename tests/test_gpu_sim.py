@@ -413,6 +413,10 @@ def test_full_size_forces_vs_oracle(pkg, O):
         assert _rel(fg, F) < tol, (precision, _rel(fg, F))
         e = sim.energies()
         assert int(e[-1, 4]) == npairs
+        # the uniform liquid fits every workgroup's records in the smaller
+        # LDS buffer (2,664 at most by a CPU count): no spill, no switch
+        st = sim.stats()
+        assert st["table_spills"] == 0 and st["table_records_big"] == 0
         sim.close()
 
 
