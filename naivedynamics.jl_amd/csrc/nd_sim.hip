@@ -343,7 +343,7 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
        double *__restrict__ partials, SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in,
        int4 *__restrict__ fix_out, Walls W, uint16_t *__restrict__ path, int32_t first, int32_t check,
        const uint32_t *__restrict__ nbr16, const int32_t *__restrict__ tables,
-       const int32_t *__restrict__ tcount, int32_t lcap) {
+       const int32_t *__restrict__ tcount, int32_t lcap, const uint32_t *__restrict__ slot_info) {
     using P4 = typename V4<PosS>::T;
     // MIXED gathers the fixed-point copy (16 B per neighbour)
     constexpr bool FX = sizeof(PosS) == 8 && sizeof(CalcS) == 4;
@@ -354,15 +354,21 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t tid = threadIdx.x;
     const int32_t half = LPA == 2 ? (tid >> 5) & 1 : 0;
-    const int32_t i = LPA == 2 ? blk * nd::kBlock + ((tid >> 7) << 6) + (((tid >> 6) & 1) << 5) + (tid & 31)
-                               : blk * nd::kBlock + tid;
+    // the list column (slot) of this lane, and the atom whose list the table
+    // build put there: slot_info = (list length << 8) | atom within the
+    // workgroup (the table build orders a workgroup's lists by length, so
+    // that each wave's lists are of similar length)
+    const int32_t is = LPA == 2 ? blk * nd::kBlock + ((tid >> 7) << 6) + (((tid >> 6) & 1) << 5) + (tid & 31)
+                                : blk * nd::kBlock + tid;
+    const uint32_t sinfo = slot_info[is];
+    const int32_t i = blk * nd::kBlock + (int32_t)(sinfo & 0xFFu);
     const int32_t n = P.n;
     AccS fx = 0, fy = 0, fz = 0;
     double pe = 0.0, vir = 0.0, ke = 0.0, np = 0.0;
     // list length (the wave's block of the list holds rows of 64 int32), and
     // the wave's longest list: rows past it are never needed, so the
     // look-ahead re-reads the last needed row (a cache hit) instead
-    const int32_t cnt = i < n ? min(ncount[i], P.cap) : 0;
+    const int32_t cnt = i < n ? min((int32_t)(sinfo >> 8), P.cap) : 0;
     // this lane's share of the list: entries e with (e >> 1) & 1 == half
     // (LPA 2; local entry k is e = 4 (k >> 1) + 2 half + (k & 1)), or all
     const int32_t cl = LPA == 1 ? cnt : ((cnt >> 2) << 1) + min(max((cnt & 3) - 2 * half, 0), 2);
@@ -398,9 +404,9 @@ k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__
     const int32_t U = __builtin_amdgcn_readfirstlane(tcount[blk]);
     G *lt = reinterpret_cast<G *>(s_table);
     constexpr bool kSentinel = sizeof(G) == 32;
-    // this lane's list: block i >> 6 (rows of 64 columns), column i & 63
-    const int32_t wave = __builtin_amdgcn_readfirstlane(i >> 6);
-    const uint32_t lane4 = (uint32_t)(i & 63) * 4u;
+    // this lane's list: block is >> 6 (rows of 64 columns), column is & 63
+    const int32_t wave = __builtin_amdgcn_readfirstlane(is >> 6);
+    const uint32_t lane4 = (uint32_t)(is & 63) * 4u;
     // 16-bit table positions, two per int32 row entry; rows up to the wave's
     // last, then two sentinel rows, which only the unchecked (FP64) loop
     // reads: the checked loops re-read the wave's last row for their
@@ -1363,7 +1369,8 @@ constexpr int kRowBatch = 16;
 __global__ void __launch_bounds__(nd::kBlock)
 k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int32_t n, int32_t cap,
         int32_t cap2, int32_t lcap, int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
-        int32_t *__restrict__ tcount, const SimMisc *__restrict__ misc, volatile int32_t *hstale) {
+        int32_t *__restrict__ tcount, const SimMisc *__restrict__ misc, volatile int32_t *hstale,
+        uint32_t *__restrict__ slot_info) {
     stale_to_host(misc, hstale);
     __shared__ TableLds T;
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1372,6 +1379,7 @@ k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int
     T.init();
     __syncthreads();
     const int32_t cnt = i < n ? min(ncount[i], cap) : 0;
+    slot_info[i] = ((uint32_t)cnt << 8) | (uint32_t)threadIdx.x;  // lists stay in atom order
     const int32_t *row = nbr + ((int64_t)(i >> 6) * cap) * 64 + lane;
     int32_t wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
@@ -1457,7 +1465,7 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
             int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
             int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, SimMisc *__restrict__ misc,
-            volatile int32_t *hstale) {
+            volatile int32_t *hstale, uint32_t *__restrict__ slot_info) {
     stale_to_host(misc, hstale);
     constexpr int kW = nd::kBlock / 64;
     __shared__ TableLds T;
@@ -1515,18 +1523,58 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
     const bool lds_ok = in_lds && U >= 0;
     if (tid == 0) tcount[blk] = U;
     if (U >= 0) T.write(tables + (int64_t)blk * lcap);
+    // Slot order.  A wave of the step kernel runs as long as its longest
+    // list, so a workgroup expanded from LDS lays its 256 lists out by
+    // decreasing length (ties by atom): column t of the workgroup's list
+    // blocks holds the list of atom `a`.  A bitonic sort of the keys
+    // (0xFFFF - length) << 8 | atom, one per lane: shuffles within a wave,
+    // LDS (the hash keys, dead after the table write) across waves.  Other
+    // workgroups keep atom order.
+    int32_t a = tid, my_cnt = cnt, my_roff = roff, my_nr = nr;
+    if (lds_ok) {  // uniform over the workgroup
+        uint32_t key = ((uint32_t)(0xFFFF - min(cnt, 0xFFFF)) << 8) | (uint32_t)tid;
+        uint32_t *sk = reinterpret_cast<uint32_t *>(T.key);
+        int32_t *s_ro = T.key + nd::kBlock, *s_nr = T.key + 2 * nd::kBlock, *s_cn = T.key + 3 * nd::kBlock;
+        __syncthreads();  // the table write's reads of the hash keys are done
+        s_ro[tid] = roff;
+        s_nr[tid] = nr;
+        s_cn[tid] = cnt;
+#pragma unroll
+        for (int k = 2; k <= nd::kBlock; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                uint32_t other;
+                if (j >= 64) {
+                    __syncthreads();
+                    sk[tid] = key;
+                    __syncthreads();
+                    other = sk[tid ^ j];
+                } else {
+                    other = (uint32_t)__shfl_xor((int)key, j, 64);
+                }
+                const bool up = (tid & k) == 0, lower = (tid & j) == 0;
+                key = (lower == up) ? min(key, other) : max(key, other);
+            }
+        }
+        __syncthreads();  // s_ro / s_nr / s_cn are written
+        a = (int32_t)(key & 0xFFu);
+        my_roff = s_ro[a];
+        my_nr = s_nr[a];
+        my_cnt = s_cn[a];
+    }
+    slot_info[i] = ((uint32_t)my_cnt << 8) | (uint32_t)a;
     if (((i >> 6) << 6) >= n) return;  // waves past the last atom own no list block
-    int32_t wmax = cnt;
+    int32_t wmax = my_cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
     wmax = __builtin_amdgcn_readfirstlane(wmax);
     if (lds_ok) {
-        // the lane's entries in record order (every record has a set bit)
-        int32_t rk = roff, cb = 0;
+        // the entries of atom a in record order (every record has a set bit)
+        int32_t rk = my_roff, cb = 0;
         uint32_t bits = 0u, gm = 0u;
         const uint32_t sent = (uint32_t)U << eshift;
         auto pop = [&]() -> uint32_t {  // table entry; the sentinel past the end
             if (bits == 0u) {
-                if (rk >= roff + nr) return sent;
+                if (rk >= my_roff + my_nr) return sent;
                 const uint2 r = s_rec[rk];
                 ++rk;
                 cb = T.base[r.x];
@@ -1719,6 +1767,7 @@ struct nd_sim {
     nd::DevBuf cpos, cstart;  // cell-list walk: cell-sorted positions, cell starts
     nd::DevBuf path;          // uint16 path length of each atom since the last build
     nd::DevBuf nbr16, tables, tcount;  // neighbourhood tables and 16-bit lists (k_table)
+    nd::DevBuf slot_info;              // per list column: (length << 8) | atom in the workgroup
     nd::DevBuf recs, nrecs;   // the two-phase walk's (group, mask) records per atom
     int32_t lcap = 0;         // table capacity (records) per workgroup
     int32_t pair_batch = kPairBatch;  // step kernel: neighbours per pipeline batch (FP32)
@@ -1790,7 +1839,7 @@ void for_each_buf(nd_sim *s, F &&f) {
                           &s->store, &s->kids, &s->rep, &s->boxes, &s->nbr, &s->ncount, &s->misc,
                           &s->partials, &s->rows, &s->snap_pos, &s->snap_vel, &s->snap_frc, &s->snap_orig,
                           &s->sort.keys_alt, &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos,
-                          &s->cstart, &s->path, &s->nbr16, &s->tables, &s->tcount, &s->recs, &s->nrecs,
+                          &s->cstart, &s->path, &s->nbr16, &s->tables, &s->tcount, &s->slot_info, &s->recs, &s->nrecs,
                           &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig};
     for (auto *b : bufs) f(*b);
 }
@@ -1886,7 +1935,7 @@ void launch_tables(nd_sim *s) {
     k_table<<<nd::grid_for(s->n), nd::kBlock, 0, s->ctx->stream>>>(
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->n, s->cap, rows16(s->cap), s->lcap, entry_shift(s),
         s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(), s->tcount.as<int32_t>(), s->misc.as<SimMisc>(),
-        s->hstale);
+        s->hstale, s->slot_info.as<uint32_t>());
     ND_HIP(hipGetLastError());
 }
 
@@ -1983,11 +2032,11 @@ void launch_rebuild(nd_sim *s, bool full) {
             if (s->rec_big)
                 k_table_rec<kRecLdsBig><<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
                                                   entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
-                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale);
+                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale, s->slot_info.as<uint32_t>());
             else
                 k_table_rec<kRecLds><<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
                                                   entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
-                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale);
+                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale, s->slot_info.as<uint32_t>());
             ND_HIP(hipGetLastError());
             return;
         }
@@ -2094,7 +2143,7 @@ void launch_step_pb(nd_sim *s, bool check) {
         s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
         s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s), s->path.as<uint16_t>(),
         s->since == 0 ? 1 : 0, check ? 1 : 0, s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
-        s->tcount.as<int32_t>(), s->lcap);
+        s->tcount.as<int32_t>(), s->lcap, s->slot_info.as<uint32_t>());
     if (MODE == kModeFused) s->since++;
 }
 
@@ -2614,6 +2663,7 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         }
         s->tables.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n) * (size_t)s->lcap);
         s->tcount.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n));
+        s->slot_info.reserve(sizeof(uint32_t) * (size_t)nd::grid_for(s->n) * nd::kBlock);
         s->misc.reserve(sizeof(SimMisc));
         s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
         s->rows.reserve(sizeof(double) * 5 * (size_t)s->max_rows);
