@@ -18,7 +18,10 @@
 //           its atoms (sorted indices, at most lcap of them) and their count
 //           (tcount; -1: too many, the workgroup reads nbr directly)
 //   nbr16   the same lists as 16-bit positions in the workgroup's table, two
-//           per int32: one block of [ceil(cap/2)][64] per 64 atoms
+//           per int32: one block of [ceil(cap/2)][64] per 64 list columns
+//   slot_info  uint32 per list column: (list length << 8) | atom within the
+//           workgroup; a workgroup built from LDS records orders its columns
+//           by decreasing list length, others keep atom order
 // Rebuild (every rebuild_every steps): key -> radix sort -> permute state ->
 // LBVH (leaf boxes +-r_list/2, nd_lbvh.hip) -> one lane per atom walks the
 // tree from the root and appends every slot j != i with |x_i - x_j|^2 <
@@ -339,9 +342,8 @@ template <class PosS, class CalcS, class AccS, int MODE, bool ENERGY, int PB = k
 __global__ void __launch_bounds__(nd::kBlock * LPA)
 k_step(const typename V4<PosS>::T *__restrict__ pos_in, typename V4<PosS>::T *__restrict__ pos_out,
        typename V4<PosS>::T *__restrict__ vel, typename V4<PosS>::T *__restrict__ frc,
-       const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, StepParams P,
-       double *__restrict__ partials, SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in,
-       int4 *__restrict__ fix_out, Walls W, uint16_t *__restrict__ path, int32_t first, int32_t check,
+       const int32_t *__restrict__ nbr, StepParams P, double *__restrict__ partials,
+       SimMisc *__restrict__ misc, const int4 *__restrict__ fix_in, int4 *__restrict__ fix_out, Walls W, uint16_t *__restrict__ path, int32_t first, int32_t check,
        const uint32_t *__restrict__ nbr16, const int32_t *__restrict__ tables,
        const int32_t *__restrict__ tcount, int32_t lcap, const uint32_t *__restrict__ slot_info) {
     using P4 = typename V4<PosS>::T;
@@ -2140,7 +2142,7 @@ void launch_step_pb(nd_sim *s, bool check) {
     }
     k_step<PosS, CalcS, AccS, MODE, E, PB, LPA><<<g, nd::kBlock * LPA, lds, st>>>(
         s->pos[s->cur].as<P4>(), s->pos[a].as<P4>(), s->vel[s->vcur].as<P4>(), s->frc.as<P4>(),
-        s->nbr.as<int32_t>(), s->ncount.as<int32_t>(), s->sp, s->partials.as<double>(),
+        s->nbr.as<int32_t>(), s->sp, s->partials.as<double>(),
         s->misc.as<SimMisc>(), fix_ptr(s, s->cur), fix_ptr(s, a), walls_of(s), s->path.as<uint16_t>(),
         s->since == 0 ? 1 : 0, check ? 1 : 0, s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
         s->tcount.as<int32_t>(), s->lcap, s->slot_info.as<uint32_t>());
