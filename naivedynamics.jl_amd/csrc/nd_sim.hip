@@ -1284,8 +1284,12 @@ struct TableLds {
     __device__ int32_t insert(int32_t g, uint32_t bits) {
         uint32_t h = group_hash(g);
         for (int probe = 0; probe < kHashSlots; ++probe) {
-            const int32_t old = atomicCAS(&key[h], -1, g);
-            if (old == -1 || old == g) {
+            // a slot's key, once set, never changes: a plain read settles
+            // most inserts (the group is there, or another group is), and
+            // only an empty slot takes the compare-and-swap
+            int32_t cur = __hip_atomic_load(&key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cur == -1) cur = atomicCAS(&key[h], -1, g);
+            if (cur == -1 || cur == g) {
                 atomicOr(&mask[h], bits);
                 return (int32_t)h;
             }
