@@ -47,12 +47,15 @@ def log(msg):
 # LBVH leaf size per workload, from sweeps over 8 / 16 / 32 with the Hilbert
 # key: 32 on the uniform liquid (ljc-262144), 16 on the dense blobs
 SIM_ATOMS_PER_LEAF = 32
-# atoms re-sorted by key (and the tree rebuilt) every 500 steps; the list
+# atoms re-sorted by key (and the tree rebuilt) every 200 steps; the list
 # rebuilds between (every 10 steps) refit the same tree's boxes.  Sweep at
-# config 2 over 1,000 steps: re-sorting at every rebuild 19,815 steps/s,
-# every 20 / 50 / 100 / 200 steps 21,159 / 22,344 / 22,756 / 23,045, once
-# 24,168; 500 bounds the drift of the atom order over long runs.
-SIM_RESORT_EVERY = 500
+# config 2 over 8,000 steps (steps/s, first 1,000-step segment / mean): re-sort
+# every 100: 26,603 / 25,860, 150: 26,498 / 25,882, 200: 26,778-27,210 /
+# 25,910-25,935, 300: 26,747 / 25,728, 500: 26,462-27,022 / 24,772-25,073,
+# 1,000: 28,368 / 23,813, 2,000: 28,532 / 22,782, never: 28,466 / 19,449 (the
+# refitted tree of the initial lattice order is the best order while the
+# lattice melts, and degrades steadily after it).
+SIM_RESORT_EVERY = 200
 # the blob workload's atoms move faster: re-sort every 100 steps there
 # (500 steps, ms/step at 0 / 100 / 500: FP64 0.418 / 0.406 / 0.409, MIXED
 # 0.273 / 0.259 / 0.261, FP32 0.243 / 0.231 / 0.233); batched replicas
