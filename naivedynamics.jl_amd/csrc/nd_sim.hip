@@ -1039,24 +1039,31 @@ __host__ __device__ __forceinline__ int32_t rec_stride(int32_t cap) { return (ca
 // are contiguous (recs[i * rec_stride(cap) + k], k < nrecs[i]; blocks of 64 B
 // multiples, so a reader fetches two records per 16-byte load); each holds
 // at least one neighbour, so there are never more records than list entries.
-template <int A, bool REC = false, int F = kFront>
-__global__ void __launch_bounds__(nd::kBlock)
-k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
-         float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
-         int32_t *__restrict__ ctr, uint2 *__restrict__ recs = nullptr, int32_t *__restrict__ nrecs = nullptr) {
+// LDS of the walk: per wave a staging buffer (8 group slots of A atoms, or
+// F fetched records), a node stack and 8 group queues with dummy slots.
+template <int A, int F> struct Walk3Lds {
+    static constexpr int kSlot = 4 * A + 4;  // floats per group slot (16 B pad: no bank conflicts)
+    static constexpr int kBufF = 8 * kSlot > 32 * F ? 8 * kSlot : 32 * F;  // records: 8 float4 each
+    static constexpr int kW = nd::kBlock / 64;
+    __align__(16) float buf[kW][kBufF];
+    int32_t stk[kW][kStack2];
+    int32_t q[kW][8 * kGroupQ + 64];
+};
+
+template <int A, bool REC, int F>
+__device__ __forceinline__ void
+walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL, float h,
+           float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
+           int32_t *__restrict__ ctr, uint2 *__restrict__ recs, int32_t *__restrict__ nrecs,
+           Walk3Lds<A, F> &L) {
     static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
     constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
-    constexpr int kSlot = 4 * A + 4;     // floats per group slot (16 B pad: no bank conflicts)
-    constexpr int kBufF = 8 * kSlot > 32 * F ? 8 * kSlot : 32 * F;  // records: 8 float4 each
-    constexpr int kW = nd::kBlock / 64;
-    __shared__ __align__(16) float s_buf[kW][kBufF];
-    __shared__ int32_t s_stk[kW][kStack2];
-    __shared__ int32_t s_q[kW][8 * kGroupQ + 64];  // 8 group queues + dummy slots
+    constexpr int kSlot = Walk3Lds<A, F>::kSlot;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int grp = lane >> 3, gl = lane & 7;
-    float *buf = s_buf[w];
+    float *buf = L.buf[w];
     float4 *rec = reinterpret_cast<float4 *>(buf);
-    int32_t *stk = s_stk[w], *gq = s_q[w];
+    int32_t *stk = L.stk[w], *gq = L.q[w];
 
     const int32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = i < n;
@@ -1239,6 +1246,15 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
         if (m > __hip_atomic_load(&ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&ctr[0], m);
         if (m > cap) atomicOr(&ctr[1], 1);
     }
+}
+
+template <int A, bool REC = false, int F = kFront>
+__global__ void __launch_bounds__(nd::kBlock)
+k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
+         float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
+         int32_t *__restrict__ ctr, uint2 *__restrict__ recs = nullptr, int32_t *__restrict__ nrecs = nullptr) {
+    __shared__ Walk3Lds<A, F> L;
+    walk3_body<A, REC, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L);
 }
 
 // ---- neighbourhood tables ----
@@ -1465,18 +1481,25 @@ k_table(const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncount, int
 // then take RL = 5120 (53.3 KB: three per CU).
 constexpr int kRecLds = 3072, kRecLdsBig = 5120;
 
+// LDS of a table build from records: the hash and RL records (hash slot, mask)
+template <int RL> struct TableRecLds {
+    TableLds T;
+    uint2 rec[RL];
+    int32_t wsum[nd::kBlock / 64];
+};
+
 template <int RL>
-__global__ void __launch_bounds__(nd::kBlock)
-k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
-            const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
-            int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
-            int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, SimMisc *__restrict__ misc,
-            volatile int32_t *hstale, uint32_t *__restrict__ slot_info) {
+__device__ __forceinline__ void
+table_rec_body(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
+               const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
+               int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
+               int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, SimMisc *__restrict__ misc,
+               volatile int32_t *hstale, uint32_t *__restrict__ slot_info, TableRecLds<RL> &S) {
     stale_to_host(misc, hstale);
     constexpr int kW = nd::kBlock / 64;
-    __shared__ TableLds T;
-    __shared__ uint2 s_rec[RL];  // (hash slot, mask)
-    __shared__ int32_t s_wsum[kW];
+    TableLds &T = S.T;
+    uint2 *s_rec = S.rec;        // (hash slot, mask)
+    int32_t *s_wsum = S.wsum;
     const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int32_t i = blk * nd::kBlock + tid;
@@ -1655,6 +1678,45 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
         for (int32_t r = 0; r < wmax; ++r) out[(int64_t)r * 64] = pop();
     }
 }
+
+template <int RL>
+__global__ void __launch_bounds__(nd::kBlock)
+k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
+            const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
+            int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
+            int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, SimMisc *__restrict__ misc,
+            volatile int32_t *hstale, uint32_t *__restrict__ slot_info) {
+    __shared__ TableRecLds<RL> S;
+    table_rec_body<RL>(recs, nrecs, ncount, n, cap, cap2, lcap, eshift, nbr16, tables, tcount, nbr, misc,
+                       hstale, slot_info, S);
+}
+
+// The two-phase walk and the table build in one kernel: a workgroup walks
+// for its 256 atoms (the same 256 atoms as a step-kernel workgroup), then,
+// after a barrier, builds their table in the same LDS.  A workgroup that
+// finishes its walk early builds its table while slower ones still walk,
+// and one launch replaces two.  Each lane reads back only the records and
+// counts it wrote itself, so the barrier (for the LDS) is the only
+// synchronisation needed.
+template <int A, int F>
+__global__ void __launch_bounds__(nd::kBlock)
+k_walk_table(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL, float h,
+             float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
+             int32_t *__restrict__ ctr, uint2 *__restrict__ recs, int32_t *__restrict__ nrecs, int32_t cap2,
+             int32_t lcap, int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
+             int32_t *__restrict__ tcount, SimMisc *__restrict__ misc, volatile int32_t *hstale,
+             uint32_t *__restrict__ slot_info) {
+    union Lds {
+        Walk3Lds<A, F> walk;
+        TableRecLds<kRecLds> table;
+    };
+    __shared__ Lds L;
+    walk3_body<A, true, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L.walk);
+    __syncthreads();  // every wave's walk is done with the LDS
+    table_rec_body<kRecLds>(recs, nrecs, ncount, n, cap, cap2, lcap, eshift, nbr16, tables, tcount, nbr, misc,
+                            hstale, slot_info, L.table);
+}
+
 
 // Cell-list walk (list_walk 3), the comparison point for the LBVH walks: the
 // same list (every other atom with r^2 < r_list^2, as sorted indices) from the
@@ -2017,6 +2079,36 @@ void launch_rebuild(nd_sim *s, bool full) {
         uint2 *rc = s->recs.as<uint2>();
         int32_t *nr = s->nrecs.as<int32_t>();
         if (two_phase) {
+            uint32_t *si = s->slot_info.as<uint32_t>();
+            uint32_t *n16 = s->nbr16.as<uint32_t>();
+            int32_t *tb = s->tables.as<int32_t>(), *tc = s->tcount.as<int32_t>();
+            const int32_t c2 = rows16(s->cap), es = entry_shift(s);
+            if (!s->rec_big) {
+                // walk and table build in one kernel
+#define ND_WALKTAB(A, F)                                                                                  \
+    k_walk_table<A, F><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr, c2, \
+                                                 s->lcap, es, n16, tb, tc, misc, s->hstale, si)
+                if (s->walk_front == 16) {
+                    switch (s->apl) {
+                        case 4: ND_WALKTAB(4, 16); break;
+                        case 8: ND_WALKTAB(8, 16); break;
+                        case 16: ND_WALKTAB(16, 16); break;
+                        default: ND_WALKTAB(32, 16); break;
+                    }
+                } else {
+                    switch (s->apl) {
+                        case 4: ND_WALKTAB(4, 8); break;
+                        case 8: ND_WALKTAB(8, 8); break;
+                        case 16: ND_WALKTAB(16, 8); break;
+                        default: ND_WALKTAB(32, 8); break;
+                    }
+                }
+#undef ND_WALKTAB
+                ND_HIP(hipGetLastError());
+                return;
+            }
+            // dense workloads: the walk, then the table build with the larger
+            // record buffer (its LDS would lower the walk's occupancy)
 #define ND_NLIST3(A, F) k_nlist3<A, true, F><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr)
             if (s->walk_front == 16) {
                 switch (s->apl) {
@@ -2035,14 +2127,8 @@ void launch_rebuild(nd_sim *s, bool full) {
             }
 #undef ND_NLIST3
             ND_HIP(hipGetLastError());
-            if (s->rec_big)
-                k_table_rec<kRecLdsBig><<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
-                                                  entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
-                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale, s->slot_info.as<uint32_t>());
-            else
-                k_table_rec<kRecLds><<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, rows16(s->cap), s->lcap,
-                                                  entry_shift(s), s->nbr16.as<uint32_t>(), s->tables.as<int32_t>(),
-                                                  s->tcount.as<int32_t>(), nb, misc, s->hstale, s->slot_info.as<uint32_t>());
+            k_table_rec<kRecLdsBig><<<g, nd::kBlock, 0, st>>>(rc, nr, nc, n, s->cap, c2, s->lcap, es, n16, tb, tc,
+                                                              nb, misc, s->hstale, si);
             ND_HIP(hipGetLastError());
             return;
         }
