@@ -684,9 +684,55 @@ __device__ __forceinline__ void two_sum_merge(double &s, double &c, double s2, d
     c = c + c2 + err;
 }
 
+// Largest XCD slab the walk order places (blocks of 256 atoms per XCD).
+constexpr int kOrderSlab = 1024;
+
+// The next walk's block order for XCD slab x, by one workgroup (the blocks
+// xcd_block gives that XCD: slab x starts at x per + min(x, rem) and holds
+// per + (x < rem) blocks).  A block's weight is its four waves' work
+// estimates from the walk that just ran; blocks are ranked by weight (ties
+// by index, so the ranks are a permutation) and rank r goes to slab position
+// snake(r): position k is the workgroup the XCD's dispatcher places k-th,
+// on CU k mod cus in its (k / cus)-th slot, and odd slots take their ranks in
+// reverse, so CU c of a 4-slot slab gets ranks c, 2 cus - 1 - c, 2 cus + c
+// and 4 cus - 1 - c: equal rank sums, the heaviest block in the oldest slot.
+__device__ __forceinline__ void walk_order_slab(const int32_t *__restrict__ work, int32_t nblocks, int32_t cus,
+                                                int32_t *__restrict__ order, int32_t x, int32_t *wt) {
+    constexpr int32_t kXcd = 8;
+    const int32_t per = nblocks / kXcd, rem = nblocks % kXcd;
+    const int32_t base = x * per + (x < rem ? x : rem), S = per + (x < rem ? 1 : 0);
+    if (S > kOrderSlab || cus < 1) return;  // uniform: the order stays as it is
+    for (int32_t t = threadIdx.x; t < S; t += blockDim.x) {
+        const int32_t *q = work + 4 * (int64_t)(base + t);
+        wt[t] = q[0] + q[1] + q[2] + q[3];
+    }
+    __syncthreads();
+    for (int32_t t = threadIdx.x; t < S; t += blockDim.x) {
+        const int32_t v = wt[t];
+        int32_t r = 0;
+        for (int32_t j = 0; j < S; ++j) {
+            const int32_t u = wt[j];
+            r += (u > v) | ((u == v) & (j < t));
+        }
+        const int32_t slot = r / cus, c = r - slot * cus;
+        const int32_t m = min(cus, S - slot * cus);
+        const int32_t k = (slot & 1) ? slot * cus + (m - 1 - c) : r;
+        order[base + k] = base + t;
+    }
+}
+
+// Block 0: the energy row.  Blocks 1-8 (launched after a fused walk): the
+// next walk's block order, one XCD slab each (walk_order_slab), so the order
+// costs no launch of its own.
 __global__ void __launch_bounds__(nd::kBlock)
 k_energy_finalize(const double *__restrict__ partials, int32_t nblocks, double *__restrict__ rows,
-                  int32_t max_rows, SimMisc *__restrict__ misc) {
+                  int32_t max_rows, SimMisc *__restrict__ misc, const int32_t *__restrict__ work,
+                  int32_t wblocks, int32_t cus, int32_t *__restrict__ order) {
+    if (blockIdx.x > 0) {
+        __shared__ int32_t wt[kOrderSlab];
+        walk_order_slab(work, wblocks, cus, order, (int32_t)blockIdx.x - 1, wt);
+        return;
+    }
     __shared__ double ss[4][nd::kBlock], cc[4][nd::kBlock];
     double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
     for (int32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
@@ -1055,7 +1101,7 @@ __device__ __forceinline__ void
 walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL, float h,
            float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
            int32_t *__restrict__ ctr, uint2 *__restrict__ recs, int32_t *__restrict__ nrecs,
-           Walk3Lds<A, F> &L) {
+           Walk3Lds<A, F> &L, int32_t blk, int32_t *__restrict__ work) {
     static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
     constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
     constexpr int kSlot = Walk3Lds<A, F>::kSlot;
@@ -1065,7 +1111,7 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
     float4 *rec = reinterpret_cast<float4 *>(buf);
     int32_t *stk = L.stk[w], *gq = L.q[w];
 
-    const int32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int32_t i = blk * nd::kBlock + threadIdx.x;
     const bool active = i < n;
     // this wave's block of the list as a buffer resource: a row at or past cap
     // lies outside it, and the hardware drops such a store (no branch per hit)
@@ -1100,6 +1146,7 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
     };
 
     int32_t sp = 1, fetched = 0;
+    int32_t rounds = 0, leaf_steps = 0;  // wave-uniform work counters (phase 1, phase 2)
     stk[0] = nL + 1;  // the root is internal (nL >= 2)
     wave_lds_sync();
     while (sp > 0 || nqu > 0) {
@@ -1109,6 +1156,7 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
             int32_t steps = len;
             for (int off = 32; off > 0; off >>= 1) steps = max(steps, __shfl_xor(steps, off, 64));
             steps = __builtin_amdgcn_readfirstlane(steps);
+            leaf_steps += steps;
             float *slot = buf + grp * kSlot;
             // atom gl + 8 r of the group's leaf k; lanes past the leaf size load atom 0
             // (named registers, not an array: the loads stay in flight across the step)
@@ -1176,6 +1224,7 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
             continue;
         }
         // ---- phase 1: expand up to F records of the stack top ----
+        ++rounds;
         // (each record pushes at most 4 entries: k pops leave room for them)
         const int32_t room = (kStack2 - kStackDfs - sp) / 3;
         int32_t k = sp < F ? sp : F;
@@ -1229,6 +1278,10 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
         wave_lds_sync();
     }
     if (bad && lane == 0) atomicOr(&ctr[2], nd::kStatusTree);
+    // the wave's work estimate for the next walk's block placement: a CU's
+    // walk time follows its waves' leaf steps and round trips (~0.12 and
+    // ~0.28 us each when four workgroups share the CU)
+    if (work != nullptr && lane == 0) work[4 * blk + w] = 5 * leaf_steps + 12 * rounds;
     if (REC && pend_g != ~0u) {
         if (nrec < cap) myrec[nrec] = make_uint2(pend_g, pend_b);
         ++nrec;
@@ -1254,7 +1307,8 @@ k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_
          float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
          int32_t *__restrict__ ctr, uint2 *__restrict__ recs = nullptr, int32_t *__restrict__ nrecs = nullptr) {
     __shared__ Walk3Lds<A, F> L;
-    walk3_body<A, REC, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L);
+    walk3_body<A, REC, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L,
+                          xcd_block(blockIdx.x, gridDim.x), nullptr);
 }
 
 // ---- neighbourhood tables ----
@@ -1494,13 +1548,12 @@ table_rec_body(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs
                const int32_t *__restrict__ ncount, int32_t n, int32_t cap, int32_t cap2, int32_t lcap,
                int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
                int32_t *__restrict__ tcount, int32_t *__restrict__ nbr, SimMisc *__restrict__ misc,
-               volatile int32_t *hstale, uint32_t *__restrict__ slot_info, TableRecLds<RL> &S) {
+               volatile int32_t *hstale, uint32_t *__restrict__ slot_info, TableRecLds<RL> &S, int32_t blk) {
     stale_to_host(misc, hstale);
     constexpr int kW = nd::kBlock / 64;
     TableLds &T = S.T;
     uint2 *s_rec = S.rec;        // (hash slot, mask)
     int32_t *s_wsum = S.wsum;
-    const int32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int32_t i = blk * nd::kBlock + tid;
     T.init();
@@ -1688,7 +1741,7 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
             volatile int32_t *hstale, uint32_t *__restrict__ slot_info) {
     __shared__ TableRecLds<RL> S;
     table_rec_body<RL>(recs, nrecs, ncount, n, cap, cap2, lcap, eshift, nbr16, tables, tcount, nbr, misc,
-                       hstale, slot_info, S);
+                       hstale, slot_info, S, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // The two-phase walk and the table build in one kernel: a workgroup walks
@@ -1698,6 +1751,14 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
 // and one launch replaces two.  Each lane reads back only the records and
 // counts it wrote itself, so the barrier (for the LDS) is the only
 // synchronisation needed.
+//
+// Block placement: workgroup b walks atom block order[xcd_block(b)], where
+// order is a permutation of each XCD's slab of blocks made (walk_order_slab,
+// after the force step that follows a walk) from the previous walk's work
+// estimates (the identity at first).
+// All workgroups of a launch are resident at once (four per CU), so the
+// kernel ends with its busiest CU; the order gives every CU of an XCD a
+// similar total.  Any permutation gives the same lists and tables.
 template <int A, int F>
 __global__ void __launch_bounds__(nd::kBlock)
 k_walk_table(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL, float h,
@@ -1705,16 +1766,25 @@ k_walk_table(const float4 *__restrict__ pos, const float4 *__restrict__ wide, in
              int32_t *__restrict__ ctr, uint2 *__restrict__ recs, int32_t *__restrict__ nrecs, int32_t cap2,
              int32_t lcap, int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
              int32_t *__restrict__ tcount, SimMisc *__restrict__ misc, volatile int32_t *hstale,
-             uint32_t *__restrict__ slot_info) {
+             uint32_t *__restrict__ slot_info, const int32_t *__restrict__ order, int32_t *__restrict__ work) {
     union Lds {
         Walk3Lds<A, F> walk;
         TableRecLds<kRecLds> table;
     };
     __shared__ Lds L;
-    walk3_body<A, true, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L.walk);
+    const int32_t blk = __builtin_amdgcn_readfirstlane(order[xcd_block(blockIdx.x, gridDim.x)]);
+    walk3_body<A, true, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L.walk, blk, work);
     __syncthreads();  // every wave's walk is done with the LDS
     table_rec_body<kRecLds>(recs, nrecs, ncount, n, cap, cap2, lcap, eshift, nbr16, tables, tcount, nbr, misc,
-                            hstale, slot_info, L.table);
+                            hstale, slot_info, L.table, blk);
+}
+
+// The walk order alone (when the step after a walk writes no energy row;
+// otherwise k_energy_finalize's blocks 1-8 compute it).
+__global__ void __launch_bounds__(nd::kBlock)
+k_walk_order(const int32_t *__restrict__ work, int32_t nblocks, int32_t cus, int32_t *__restrict__ order) {
+    __shared__ int32_t wt[kOrderSlab];
+    walk_order_slab(work, nblocks, cus, order, blockIdx.x, wt);
 }
 
 
@@ -1836,6 +1906,10 @@ struct nd_sim {
     nd::DevBuf path;          // uint16 path length of each atom since the last build
     nd::DevBuf nbr16, tables, tcount;  // neighbourhood tables and 16-bit lists (k_table)
     nd::DevBuf slot_info;              // per list column: (length << 8) | atom in the workgroup
+    nd::DevBuf walk_order;             // fused walk: atom block of each slab position
+    nd::DevBuf walk_work;              // fused walk: per wave work estimate of the last walk
+    int32_t cus_per_xcd = 32;          // compute units per XCD (the walk order's slot width)
+    bool order_pending = false;        // a fused walk ran; its work estimates are not yet ordered
     nd::DevBuf recs, nrecs;   // the two-phase walk's (group, mask) records per atom
     int32_t lcap = 0;         // table capacity (records) per workgroup
     int32_t pair_batch = kPairBatch;  // step kernel: neighbours per pipeline batch (FP32)
@@ -1908,7 +1982,7 @@ void for_each_buf(nd_sim *s, F &&f) {
                           &s->partials, &s->rows, &s->snap_pos, &s->snap_vel, &s->snap_frc, &s->snap_orig,
                           &s->sort.keys_alt, &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos,
                           &s->cstart, &s->path, &s->nbr16, &s->tables, &s->tcount, &s->slot_info, &s->recs, &s->nrecs,
-                          &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig};
+                          &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig, &s->walk_order, &s->walk_work};
     for (auto *b : bufs) f(*b);
 }
 
@@ -2085,9 +2159,11 @@ void launch_rebuild(nd_sim *s, bool full) {
             const int32_t c2 = rows16(s->cap), es = entry_shift(s);
             if (!s->rec_big) {
                 // walk and table build in one kernel
+                const int32_t *ord = s->walk_order.as<int32_t>();
+                int32_t *wk = s->walk_work.as<int32_t>();
 #define ND_WALKTAB(A, F)                                                                                  \
     k_walk_table<A, F><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr, c2, \
-                                                 s->lcap, es, n16, tb, tc, misc, s->hstale, si)
+                                                 s->lcap, es, n16, tb, tc, misc, s->hstale, si, ord, wk)
                 if (s->walk_front == 16) {
                     switch (s->apl) {
                         case 4: ND_WALKTAB(4, 16); break;
@@ -2105,6 +2181,7 @@ void launch_rebuild(nd_sim *s, bool full) {
                 }
 #undef ND_WALKTAB
                 ND_HIP(hipGetLastError());
+                s->order_pending = true;  // the next force step's tail orders the next walk
                 return;
             }
             // dense workloads: the walk, then the table build with the larger
@@ -2286,11 +2363,18 @@ void launch_force_step(nd_sim *s, int mode, int64_t k) {
     ND_HIP(hipGetLastError());
     if (s->timing) ND_HIP(hipEventRecord(e1, s->ctx->stream));
     if (mode == kModeFused) s->cur ^= 1;
+    // the energy row and, after a fused walk, the next walk's block order
+    const bool ord = s->order_pending;
+    s->order_pending = false;
+    const int32_t nb = (int32_t)nd::grid_for(s->n);
     if (e) {
-        k_energy_finalize<<<1, nd::kBlock, 0, s->ctx->stream>>>(s->partials.as<double>(),
-                                                                (int32_t)nd::grid_for(s->n),
-                                                                s->rows.as<double>(), s->max_rows,
-                                                                s->misc.as<SimMisc>());
+        k_energy_finalize<<<ord ? 9 : 1, nd::kBlock, 0, s->ctx->stream>>>(
+            s->partials.as<double>(), nb, s->rows.as<double>(), s->max_rows, s->misc.as<SimMisc>(),
+            s->walk_work.as<int32_t>(), nb, s->cus_per_xcd, s->walk_order.as<int32_t>());
+        ND_HIP(hipGetLastError());
+    } else if (ord) {
+        k_walk_order<<<8, nd::kBlock, 0, s->ctx->stream>>>(s->walk_work.as<int32_t>(), nb, s->cus_per_xcd,
+                                                           s->walk_order.as<int32_t>());
         ND_HIP(hipGetLastError());
     }
 }
@@ -2429,7 +2513,7 @@ hipGraphExec_t window_graph(nd_sim *s, int64_t k0, int cur, int vcur, bool full)
     hipGraph_t graph;
     const int cur0 = s->cur, vcur0 = s->vcur, since0 = s->since;
     const int64_t rb0 = s->rebuilds, lb0 = s->last_build_step;
-    const bool tv0 = s->tree_valid;
+    const bool tv0 = s->tree_valid, op0 = s->order_pending;
     s->cur = cur;
     s->vcur = vcur;
     s->tree_valid = true;  // a refit window is captured only for a valid tree
@@ -2444,6 +2528,7 @@ hipGraphExec_t window_graph(nd_sim *s, int64_t k0, int cur, int vcur, bool full)
         s->rebuilds = rb0;
         s->last_build_step = lb0;
         s->tree_valid = tv0;
+        s->order_pending = op0;
         throw;
     }
     s->cur = cur0;
@@ -2452,6 +2537,7 @@ hipGraphExec_t window_graph(nd_sim *s, int64_t k0, int cur, int vcur, bool full)
     s->rebuilds = rb0;
     s->last_build_step = lb0;
     s->tree_valid = tv0;
+    s->order_pending = op0;
     ND_HIP(hipStreamEndCapture(st, &graph));
     const hipError_t e = hipGraphInstantiate(&s->gexec[key], graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -2469,6 +2555,7 @@ void run_window_graph(nd_sim *s, int64_t k0) {
     s->rebuilds++;
     s->last_build_step = k0;
     s->since = s->p.rebuild_every;  // one drift per step of the window
+    s->order_pending = false;       // the window's first force step ordered its walk
     if (full) {
         s->cur ^= (1 + s->p.rebuild_every) & 1;
         s->vcur ^= 1;
@@ -2756,6 +2843,21 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
         s->tables.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n) * (size_t)s->lcap);
         s->tcount.reserve(sizeof(int32_t) * (size_t)nd::grid_for(s->n));
         s->slot_info.reserve(sizeof(uint32_t) * (size_t)nd::grid_for(s->n) * nd::kBlock);
+        {
+            // block placement of the fused walk: the identity order, no estimates yet
+            const int32_t nb = (int32_t)nd::grid_for(s->n);
+            std::vector<int32_t> iota(nb);
+            for (int32_t b = 0; b < nb; ++b) iota[b] = b;
+            s->walk_order.reserve(sizeof(int32_t) * (size_t)nb);
+            s->walk_work.reserve(sizeof(int32_t) * 4 * (size_t)nb);
+            ND_HIP(hipMemcpyAsync(s->walk_order.p, iota.data(), sizeof(int32_t) * (size_t)nb,
+                                  hipMemcpyHostToDevice, ctx->stream));
+            ND_HIP(hipMemsetAsync(s->walk_work.p, 0, sizeof(int32_t) * 4 * (size_t)nb, ctx->stream));
+            ND_HIP(hipStreamSynchronize(ctx->stream));
+            int cus = 256;
+            ND_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+            s->cus_per_xcd = cus >= 8 ? cus / 8 : 1;
+        }
         s->misc.reserve(sizeof(SimMisc));
         s->partials.reserve(sizeof(double) * 4 * (size_t)nd::grid_for(s->n));
         s->rows.reserve(sizeof(double) * 5 * (size_t)s->max_rows);
