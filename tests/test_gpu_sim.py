@@ -133,6 +133,31 @@ def test_refit_window_leaf_sizes_vs_oracle(pkg, O, apl):
     sim.close()
 
 
+@pytest.mark.parametrize("energy_every", [0, 10])
+def test_walk_block_order_vs_oracle(pkg, O, energy_every):
+    """The fused walk places its blocks by the previous walk's work estimates.
+    70,016 atoms (2,188 leaves of 32) make 274 blocks of 256 (the last one
+    half full): XCD slabs of 35 and 34 blocks, so each order has a full
+    first slot and a partial, reversed second one.  The order comes from
+    the energy finalize launch (energy_every 10) or from a launch of its
+    own (0).  Any permutation gives the same lists, so the FP64 trajectory
+    stays within 1e-10 of the oracle's; a block walked twice or skipped
+    would break that."""
+    n = 70016
+    x, v, q, L = _cfg(pkg, n, 11)
+    sim = pkg.Simulation(n, _spec(pkg, L, "fp64", steps=30, resort_every=20, list_walk=0,
+                                  atomsperleaf=32, energy_every=energy_every))
+    sim.set_state(x, v, q)
+    sim.run(30)
+    xg, vg, fg = sim.get_state()
+    xr, vr, fr, E = O.vv_run64(x, v, q, None, O.o64_params(rc_lj=2.5, rc_coul=2.5), 0.001, 30,
+                               walls=True, lo=0.0, hi=L)
+    assert np.abs(xg - xr).max() < 1e-10
+    assert _rel(fg, fr) < 1e-10
+    assert sim.stats()["rebuilds"] == 4
+    sim.close()
+
+
 @pytest.mark.parametrize("pair_batch,walk_front", [(2, 8), (8, 8), (4, 16), (8, 16)])
 def test_tile_shapes_vs_oracle(pkg, O, pair_batch, walk_front):
     """The autotuner's tile shapes: neighbours per pipeline batch (FP32 step
