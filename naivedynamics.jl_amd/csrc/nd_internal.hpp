@@ -131,8 +131,11 @@ void launch_lbvh_wide(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *w
 void launch_lbvh_wide4(hipStream_t s, const nd_node *nodes, int32_t nL, float4 *rec);
 // launch_lbvh_refit followed by launch_lbvh_wide4, fused: leaf boxes, chunk
 // boxes and the records (internal node boxes in nodes are not updated).
+// tq, lc (optional, leaf sizes 4 to 64): per atom (-2 q', |q'|^2) with q' the
+// atom's position less its leaf box's centre c, per leaf (c, max |q'|^2).
 void launch_lbvh_refit_wide4(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride, int32_t nL,
-                             int32_t apl, float r, nd_node *nodes, int32_t *store, float4 *rec);
+                             int32_t apl, float r, nd_node *nodes, int32_t *store, float4 *rec,
+                             float4 *tq = nullptr, float4 *lc = nullptr);
 // slots_cleared: the caller has already zeroed store[0 .. nL-2] (the CAS slots);
 // leaves_done: the caller has already written the leaf boxes (+- r/2).
 void launch_lbvh(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride,

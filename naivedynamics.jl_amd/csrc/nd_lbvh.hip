@@ -354,7 +354,8 @@ k_lbvh_wide4(const nd_node *__restrict__ nodes, int32_t nL, float4 *__restrict__
 template <int SEG>
 __global__ void __launch_bounds__(nd::kBlock)
 k_leaf_chunk_boxes(const float *__restrict__ xyz, int32_t stride, int32_t natoms, float h,
-                   nd_node *__restrict__ nodes, float4 *__restrict__ chunks) {
+                   nd_node *__restrict__ nodes, float4 *__restrict__ chunks, float4 *__restrict__ tq,
+                   float4 *__restrict__ lc) {
     static_assert(SEG >= 4 && SEG <= 64 && (SEG & (SEG - 1)) == 0, "leaf size");
     static_assert(kChunk * 4 == nd::kBlock, "four lanes per leaf, one workgroup per chunk");
     constexpr int kA = SEG / 4;  // atoms per lane
@@ -394,6 +395,24 @@ k_leaf_chunk_boxes(const float *__restrict__ xyz, int32_t stride, int32_t natoms
         float *q = reinterpret_cast<float *>(nodes + a0 / SEG);
         *reinterpret_cast<float4 *>(q) = make_float4(b.mn0, b.mn1, b.mn2, b.mx0);
         *reinterpret_cast<float2 *>(q + 4) = make_float2(b.mx1, b.mx2);
+    }
+    if (tq != nullptr) {
+        // leaf-local copies (uniform over the launch): per atom (-2 q', |q'|^2)
+        // with q' = q - c about the leaf box's centre c, per leaf (c, max |q'|^2)
+        const float cx = 0.5f * (b.mn0 + b.mx0), cy = 0.5f * (b.mn1 + b.mx1), cz = 0.5f * (b.mn2 + b.mx2);
+        float r2 = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kA; ++u) {
+            const float qx = px[u] - cx, qy = py[u] - cy, qz = pz[u] - cz;
+            const float q2 = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));
+            if (a0 + u < natoms) {
+                tq[a0 + u] = make_float4(-2.0f * qx, -2.0f * qy, -2.0f * qz, q2);
+                r2 = fmaxf(r2, q2);
+            }
+        }
+#pragma unroll
+        for (int off = 1; off < 4; off <<= 1) r2 = fmaxf(r2, __shfl_xor(r2, off, 64));
+        if ((tid & 3) == 0 && a0 < natoms) lc[a0 / SEG] = make_float4(cx, cy, cz, r2);
     }
     // the chunk: the rest of the wave, then the four waves
 #pragma unroll
@@ -515,7 +534,8 @@ void launch_lbvh_refit(hipStream_t s, const float *xyz_sorted, int32_t xyz_strid
 }
 
 void launch_lbvh_refit_wide4(hipStream_t s, const float *xyz_sorted, int32_t xyz_stride, int32_t nL,
-                             int32_t apl, float r, nd_node *nodes, int32_t *store, float4 *rec) {
+                             int32_t apl, float r, nd_node *nodes, int32_t *store, float4 *rec, float4 *tq,
+                             float4 *lc) {
     const size_t off_r = ((size_t)nL + 3) & ~(size_t)3;
     const size_t off_c = (off_r + 2 * (size_t)nL + 3) & ~(size_t)3;
     const int2 *ranges = reinterpret_cast<const int2 *>(store + off_r);
@@ -524,12 +544,13 @@ void launch_lbvh_refit_wide4(hipStream_t s, const float *xyz_sorted, int32_t xyz
     const unsigned nchunks = (unsigned)((nL + kChunk - 1) / kChunk);
     const int32_t na = nL * apl;
     switch (apl) {
-        case 4: k_leaf_chunk_boxes<4><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
-        case 8: k_leaf_chunk_boxes<8><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
-        case 16: k_leaf_chunk_boxes<16><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
-        case 32: k_leaf_chunk_boxes<32><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
-        case 64: k_leaf_chunk_boxes<64><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks); break;
+        case 4: k_leaf_chunk_boxes<4><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks, tq, lc); break;
+        case 8: k_leaf_chunk_boxes<8><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks, tq, lc); break;
+        case 16: k_leaf_chunk_boxes<16><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks, tq, lc); break;
+        case 32: k_leaf_chunk_boxes<32><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks, tq, lc); break;
+        case 64: k_leaf_chunk_boxes<64><<<nchunks, kBlock, 0, s>>>(xyz_sorted, xyz_stride, na, h, nodes, chunks, tq, lc); break;
         default:
+            if (tq != nullptr) throw Error(ND_ERR_ARG, "refit: leaf-local copies need a leaf size of 4 to 64");
             launch_leaf_boxes(s, xyz_sorted, xyz_stride, nL, apl, h, nodes);
             k_lbvh_chunks<<<grid_for((int64_t)nchunks * 64), kBlock, 0, s>>>(nodes, nL, chunks);
     }

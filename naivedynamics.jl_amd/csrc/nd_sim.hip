@@ -815,18 +815,20 @@ k_permute(const int32_t *__restrict__ perm, int32_t n, const typename V4<PosS>::
           typename V4<PosS>::T *__restrict__ pos_out, const typename V4<PosS>::T *__restrict__ vel_in,
           typename V4<PosS>::T *__restrict__ vel_out, const int32_t *__restrict__ orig_in,
           int32_t *__restrict__ orig_out, float4 *__restrict__ pos32, int4 *__restrict__ fix_out,
-          StepParams P, SimMisc *__restrict__ misc, float h, nd_node *__restrict__ nodes) {
+          StepParams P, SimMisc *__restrict__ misc, float h, nd_node *__restrict__ nodes,
+          float4 *__restrict__ tq, float4 *__restrict__ lc) {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = s < n;
     const float inf = __int_as_float(0x7F800000);
     float mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
     if (act) {
         const int32_t o = perm[s];
         const auto p = pos_in[o];
         pos_out[s] = p;
         vel_out[s] = vel_in[o];
         orig_out[s] = orig_in[o];
-        const float px = (float)p.x, py = (float)p.y, pz = (float)p.z;
+        px = (float)p.x; py = (float)p.y; pz = (float)p.z;
         if (pos32) pos32[s] = make_float4(px, py, pz, (float)p.w);
         if (fix_out)
             fix_out[s] = make_int4(to_fix((double)p.x, P, misc), to_fix((double)p.y, P, misc),
@@ -848,6 +850,18 @@ k_permute(const int32_t *__restrict__ perm, int32_t n, const typename V4<PosS>::
             float *q = reinterpret_cast<float *>(nodes + s / SEG);
             *reinterpret_cast<float4 *>(q) = make_float4(mn0, mn1, mn2, mx0);
             *reinterpret_cast<float2 *>(q + 4) = make_float2(mx1, mx2);
+        }
+        if (tq != nullptr) {
+            // leaf-local copies for the two-phase walk (as k_leaf_chunk_boxes
+            // writes them in a refit)
+            const float cx = 0.5f * (mn0 + mx0), cy = 0.5f * (mn1 + mx1), cz = 0.5f * (mn2 + mx2);
+            const float qx = px - cx, qy = py - cy, qz = pz - cz;
+            float r2 = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));
+            if (act) tq[s] = make_float4(-2.0f * qx, -2.0f * qy, -2.0f * qz, r2);
+            if (!act) r2 = 0.0f;
+#pragma unroll
+            for (int off = 1; off < SEG; off <<= 1) r2 = fmaxf(r2, __shfl_xor(r2, off, 64));
+            if (act && (s & (SEG - 1)) == 0) lc[s / SEG] = make_float4(cx, cy, cz, r2);
         }
     }
 }
@@ -1096,12 +1110,13 @@ template <int A, int F> struct Walk3Lds {
     int32_t q[kW][8 * kGroupQ + 64];
 };
 
-template <int A, bool REC, int F>
+template <int A, bool REC, int F, bool LOCAL = false>
 __device__ __forceinline__ void
 walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL, float h,
            float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
            int32_t *__restrict__ ctr, uint2 *__restrict__ recs, int32_t *__restrict__ nrecs,
-           Walk3Lds<A, F> &L, int32_t blk, int32_t *__restrict__ work) {
+           Walk3Lds<A, F> &L, int32_t blk, int32_t *__restrict__ work,
+           const float4 *__restrict__ tq = nullptr, const float4 *__restrict__ lc = nullptr) {
     static_assert(A == 4 || A == 8 || A == 16 || A == 32, "leaf size");
     constexpr int NV = (A + 7) / 8;      // atoms a lane stages per leaf (1, 2 or 4)
     constexpr int kSlot = Walk3Lds<A, F>::kSlot;
@@ -1160,14 +1175,17 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
             float *slot = buf + grp * kSlot;
             // atom gl + 8 r of the group's leaf k; lanes past the leaf size load atom 0
             // (named registers, not an array: the loads stay in flight across the step)
-            const float4 *src = pos + (gl < A ? gl : 0);
+            // LOCAL: the leaf-local copies (-2 q', |q'|^2) and the leaf's
+            // (centre, R^2), loaded with the atoms
+            const float4 *src = (LOCAL ? tq : pos) + (gl < A ? gl : 0);
             const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            float4 v0 = zero, v1 = zero, v2 = zero, v3 = zero;
+            float4 v0 = zero, v1 = zero, v2 = zero, v3 = zero, cn = zero;
             auto load4 = [&](int32_t e) {
                 const float4 *lp = src + ((e >> 8) - 1) * A;
                 v0 = lp[0];
                 if constexpr (NV > 1) v1 = lp[8];
                 if constexpr (NV > 2) { v2 = lp[16]; v3 = lp[24]; }
+                if constexpr (LOCAL) cn = lc[(e >> 8) - 1];
             };
             if (len > 0) load4(gq[grp * kGroupQ]);
             for (int32_t k = 0; k < steps; ++k) {
@@ -1179,17 +1197,49 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
                     if constexpr (NV > 2) { dst[16] = v2; dst[24] = v3; }
                 }
                 wave_lds_sync();
+                const float4 cc = cn;  // the staged leaf's centre and R^2 (LOCAL)
                 const int32_t e = k < len ? gq[grp * kGroupQ + k] : 0;
                 if (k + 1 < len) load4(gq[grp * kGroupQ + k + 1]);
                 uint32_t hits = 0u;
                 const int32_t base = ((e >> 8) - 1) * A;
+                // the list's pair test: |p - q|^2 < rl2, in this order of operations
+                auto exact = [&](const float4 &q) -> bool {
+                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    const float s = dx * dx + dy * dy + dz * dz;
+                    return s < rl2;
+                };
                 if ((e >> gl) & 1) {
+                    if constexpr (LOCAL) {
+                        // |p - q|^2 - rl2 = |a|^2 - rl2 + (-2 q').a + |q'|^2 with a = p - c:
+                        // three FMAs per atom against the staged copy.  Its distance
+                        // from the exact test's value is below 16 u (|a| + R + r)^2
+                        // (u = 2^-24: the roundings of a, q', |q'|^2, the FMAs and
+                        // of the exact test itself), which E bounds from above
+                        // (3 (x^2 + y^2 + z^2) >= (x + y + z)^2); a wave with any
+                        // test within E of the radius redoes the leaf exactly from
+                        // the positions, so the list is the exact test's.
+                        const float ax = p.x - cc.x, ay = p.y - cc.y, az = p.z - cc.z;
+                        const float cp = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, __builtin_fmaf(ax, ax, -rl2)));
+                        const float E = 48.0f * 0x1p-24f * (cp + 2.0f * rl2 + cc.w);
+                        uint64_t band = 0;
 #pragma unroll
-                    for (int m = 0; m < A; ++m) {
-                        const float4 q = *reinterpret_cast<const float4 *>(slot + 4 * m);
-                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                        const float s = dx * dx + dy * dy + dz * dz;
-                        hits |= (s < rl2 ? 1u : 0u) << m;
+                        for (int m = 0; m < A; ++m) {
+                            const float4 q = *reinterpret_cast<const float4 *>(slot + 4 * m);
+                            const float t =
+                                __builtin_fmaf(ax, q.x, __builtin_fmaf(ay, q.y, __builtin_fmaf(az, q.z, q.w))) + cp;
+                            // the sign bit (a zero t is within E: the exact test decides)
+                            hits |= (__float_as_uint(t) >> (31 - m)) & (1u << m);
+                            band |= __builtin_amdgcn_ballot_w64(fabsf(t) < E);
+                        }
+                        if (band != 0) {  // rare: one leaf step in a few hundred
+                            hits = 0u;
+#pragma unroll 1
+                            for (int m = 0; m < A; ++m) hits |= (exact(pos[base + m]) ? 1u : 0u) << m;
+                        }
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < A; ++m)
+                            hits |= (exact(*reinterpret_cast<const float4 *>(slot + 4 * m)) ? 1u : 0u) << m;
                     }
                     const uint32_t self = (uint32_t)(i - base);
                     if (self < (uint32_t)A) hits &= ~(1u << self);
@@ -1301,14 +1351,15 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
     }
 }
 
-template <int A, bool REC = false, int F = kFront>
+template <int A, bool REC = false, int F = kFront, bool LOCAL = false>
 __global__ void __launch_bounds__(nd::kBlock)
 k_nlist3(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL,
          float h, float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
-         int32_t *__restrict__ ctr, uint2 *__restrict__ recs = nullptr, int32_t *__restrict__ nrecs = nullptr) {
+         int32_t *__restrict__ ctr, uint2 *__restrict__ recs = nullptr, int32_t *__restrict__ nrecs = nullptr,
+         const float4 *__restrict__ tq = nullptr, const float4 *__restrict__ lc = nullptr) {
     __shared__ Walk3Lds<A, F> L;
-    walk3_body<A, REC, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L,
-                          xcd_block(blockIdx.x, gridDim.x), nullptr);
+    walk3_body<A, REC, F, LOCAL>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L,
+                                 xcd_block(blockIdx.x, gridDim.x), nullptr, tq, lc);
 }
 
 // ---- neighbourhood tables ----
@@ -1760,20 +1811,22 @@ k_table_rec(const uint2 *__restrict__ recs, const int32_t *__restrict__ nrecs,
 // kernel ends with its busiest CU; the order gives every CU of an XCD a
 // similar total.  Any permutation gives the same lists and tables.
 template <int A, int F>
-__global__ void __launch_bounds__(nd::kBlock)
+__global__ void __launch_bounds__(nd::kBlock, 4)  // four waves per SIMD: all 4 workgroups per CU resident
 k_walk_table(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int32_t n, int32_t nL, float h,
              float rl2, int32_t cap, int32_t *__restrict__ nbr, int32_t *__restrict__ ncount,
              int32_t *__restrict__ ctr, uint2 *__restrict__ recs, int32_t *__restrict__ nrecs, int32_t cap2,
              int32_t lcap, int32_t eshift, uint32_t *__restrict__ nbr16, int32_t *__restrict__ tables,
              int32_t *__restrict__ tcount, SimMisc *__restrict__ misc, volatile int32_t *hstale,
-             uint32_t *__restrict__ slot_info, const int32_t *__restrict__ order, int32_t *__restrict__ work) {
+             uint32_t *__restrict__ slot_info, const int32_t *__restrict__ order, int32_t *__restrict__ work,
+             const float4 *__restrict__ tq, const float4 *__restrict__ lc) {
     union Lds {
         Walk3Lds<A, F> walk;
         TableRecLds<kRecLds> table;
     };
     __shared__ Lds L;
     const int32_t blk = __builtin_amdgcn_readfirstlane(order[xcd_block(blockIdx.x, gridDim.x)]);
-    walk3_body<A, true, F>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L.walk, blk, work);
+    walk3_body<A, true, F, true>(pos, wide, n, nL, h, rl2, cap, nbr, ncount, ctr, recs, nrecs, L.walk, blk, work,
+                                 tq, lc);
     __syncthreads();  // every wave's walk is done with the LDS
     table_rec_body<kRecLds>(recs, nrecs, ncount, n, cap, cap2, lcap, eshift, nbr16, tables, tcount, nbr, misc,
                             hstale, slot_info, L.table, blk);
@@ -1908,6 +1961,8 @@ struct nd_sim {
     nd::DevBuf slot_info;              // per list column: (length << 8) | atom in the workgroup
     nd::DevBuf walk_order;             // fused walk: atom block of each slab position
     nd::DevBuf walk_work;              // fused walk: per wave work estimate of the last walk
+    nd::DevBuf leaf_q;                 // two-phase walk: per atom (-2 q', |q'|^2) about its leaf's centre
+    nd::DevBuf leaf_c;                 // two-phase walk: per leaf (centre, largest |q'|^2)
     int32_t cus_per_xcd = 32;          // compute units per XCD (the walk order's slot width)
     bool order_pending = false;        // a fused walk ran; its work estimates are not yet ordered
     nd::DevBuf recs, nrecs;   // the two-phase walk's (group, mask) records per atom
@@ -1982,7 +2037,8 @@ void for_each_buf(nd_sim *s, F &&f) {
                           &s->partials, &s->rows, &s->snap_pos, &s->snap_vel, &s->snap_frc, &s->snap_orig,
                           &s->sort.keys_alt, &s->sort.vals_alt, &s->sort.hist, &s->sort.flags, &s->cpos,
                           &s->cstart, &s->path, &s->nbr16, &s->tables, &s->tcount, &s->slot_info, &s->recs, &s->nrecs,
-                          &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig, &s->walk_order, &s->walk_work};
+                          &s->wsnap_pos, &s->wsnap_vel, &s->wsnap_orig, &s->walk_order, &s->walk_work,
+                          &s->leaf_q, &s->leaf_c};
     for (auto *b : bufs) f(*b);
 }
 
@@ -2061,6 +2117,13 @@ int32_t resort_period(const nd_sim *s) {
     return s->p.resort_every > 0 ? s->p.resort_every : s->p.rebuild_every;
 }
 
+// The two-phase walk (records of four-wide nodes, leaf-local copies) runs
+// the simulation's list when asked for and the leaf size and count allow it.
+bool two_phase_walk(const nd_sim *s) {
+    return s->p.list_walk == 0 && s->nL < kMaxLeaves3 &&
+           (s->apl == 4 || s->apl == 8 || s->apl == 16 || s->apl == 32);
+}
+
 // A rebuild at step k re-sorts (new keys, order and topology) when the
 // period says so or when no tree matches the current order.
 bool resort_due(const nd_sim *s, int64_t k) { return !s->tree_valid || k % resort_period(s) == 0; }
@@ -2110,8 +2173,7 @@ void launch_rebuild(nd_sim *s, bool full) {
         launch_tables(s);
         return;
     }
-    const bool two_phase = s->p.list_walk == 0 && s->nL < kMaxLeaves3 &&
-                           (s->apl == 4 || s->apl == 8 || s->apl == 16 || s->apl == 32);
+    const bool two_phase = two_phase_walk(s);
     bool records_done = false;  // the four-wide records already hold the refitted boxes
     if (full) {
         launch_sort_and_build(s);
@@ -2123,7 +2185,8 @@ void launch_rebuild(nd_sim *s, bool full) {
                                    : reinterpret_cast<const float *>(s->pos[s->cur].p);
         if (two_phase) {
             nd::launch_lbvh_refit_wide4(st, p32, 4, s->nL, s->apl, s->rlist, s->nodes.as<nd_node>(),
-                                        s->store.as<int32_t>(), s->kids.as<float4>());
+                                        s->store.as<int32_t>(), s->kids.as<float4>(), s->leaf_q.as<float4>(),
+                                        s->leaf_c.as<float4>());
             records_done = true;
         } else {
             nd::launch_lbvh_refit(st, p32, 4, s->nL, s->apl, s->rlist, s->nodes.as<nd_node>(),
@@ -2155,6 +2218,9 @@ void launch_rebuild(nd_sim *s, bool full) {
         if (two_phase) {
             uint32_t *si = s->slot_info.as<uint32_t>();
             uint32_t *n16 = s->nbr16.as<uint32_t>();
+            // leaf-local copies of the atoms for the walk's leaf tests (written
+            // with the leaf boxes, by the permutation or the refit)
+            const float4 *tql = s->leaf_q.as<float4>(), *lcl = s->leaf_c.as<float4>();
             int32_t *tb = s->tables.as<int32_t>(), *tc = s->tcount.as<int32_t>();
             const int32_t c2 = rows16(s->cap), es = entry_shift(s);
             if (!s->rec_big) {
@@ -2163,7 +2229,7 @@ void launch_rebuild(nd_sim *s, bool full) {
                 int32_t *wk = s->walk_work.as<int32_t>();
 #define ND_WALKTAB(A, F)                                                                                  \
     k_walk_table<A, F><<<g, nd::kBlock, 0, st>>>(p4, kids, n, s->nL, hl, rl2, s->cap, nb, nc, ctr, rc, nr, c2, \
-                                                 s->lcap, es, n16, tb, tc, misc, s->hstale, si, ord, wk)
+                                                 s->lcap, es, n16, tb, tc, misc, s->hstale, si, ord, wk, tql, lcl)
                 if (s->walk_front == 16) {
                     switch (s->apl) {
                         case 4: ND_WALKTAB(4, 16); break;
@@ -2250,13 +2316,17 @@ void launch_sort_and_build(nd_sim *s) {
     // leaf boxes in the permutation for power-of-two leaves up to 64 (the
     // cell walk needs no tree, so none there)
     const bool boxes = s->p.list_walk != 3 && s->apl <= 64 && (s->apl & (s->apl - 1)) == 0;
+    // the two-phase walk's leaf-local copies, written with the leaf boxes
+    float4 *lq = two_phase_walk(s) ? s->leaf_q.as<float4>() : nullptr;
+    float4 *lcn = two_phase_walk(s) ? s->leaf_c.as<float4>() : nullptr;
     const float h = s->rlist / 2.0f;  // as launch_lbvh computes it
     nd_node *nodes = s->nodes.as<nd_node>();
 #define ND_PERMUTE(PT, SEG)                                                                              \
     k_permute<PT, SEG><<<g, nd::kBlock, 0, st>>>(                                                        \
         perm, n, s->pos[s->cur].as<V4<PT>::T>(), s->pos[a].as<V4<PT>::T>(), s->vel[s->vcur].as<V4<PT>::T>(), \
         s->vel[va].as<V4<PT>::T>(), s->orig[s->vcur].as<int32_t>(), s->orig[va].as<int32_t>(),           \
-        s->wide ? s->pos32.as<float4>() : nullptr, s->wide ? fix_ptr(s, a) : nullptr, s->sp, misc, h, nodes)
+        s->wide ? s->pos32.as<float4>() : nullptr, s->wide ? fix_ptr(s, a) : nullptr, s->sp, misc, h, nodes, \
+        lq, lcn)
 #define ND_PERMUTE_SEG(PT)                                        \
     switch (boxes ? s->apl : 0) {                                \
         case 1: ND_PERMUTE(PT, 1); break;                         \
@@ -2850,6 +2920,8 @@ int nd_sim_create(nd_ctx *ctx, const nd_sim_params *p, nd_sim **out) {
             for (int32_t b = 0; b < nb; ++b) iota[b] = b;
             s->walk_order.reserve(sizeof(int32_t) * (size_t)nb);
             s->walk_work.reserve(sizeof(int32_t) * 4 * (size_t)nb);
+            s->leaf_q.reserve(sizeof(float4) * (size_t)s->n);
+            s->leaf_c.reserve(sizeof(float4) * (size_t)(s->nL > 0 ? s->nL : 1));
             ND_HIP(hipMemcpyAsync(s->walk_order.p, iota.data(), sizeof(int32_t) * (size_t)nb,
                                   hipMemcpyHostToDevice, ctx->stream));
             ND_HIP(hipMemsetAsync(s->walk_work.p, 0, sizeof(int32_t) * 4 * (size_t)nb, ctx->stream));
