@@ -1231,7 +1231,7 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
                             hits |= (__float_as_uint(t) >> (31 - m)) & (1u << m);
                             band |= __builtin_amdgcn_ballot_w64(fabsf(t) < E);
                         }
-                        if (band != 0) {  // rare: one leaf step in a few hundred
+                        if (band != 0) {  // rare: a test within E of the radius
                             hits = 0u;
 #pragma unroll 1
                             for (int m = 0; m < A; ++m) hits |= (exact(pos[base + m]) ? 1u : 0u) << m;
