@@ -1223,12 +1223,14 @@ walk3_body(const float4 *__restrict__ pos, const float4 *__restrict__ wide, int3
                         const float E = 48.0f * 0x1p-24f * (cp + 2.0f * rl2 + cc.w);
                         uint64_t band = 0;
 #pragma unroll
-                        for (int m = 0; m < A; ++m) {
+                        for (int m = A - 1; m >= 0; --m) {
                             const float4 q = *reinterpret_cast<const float4 *>(slot + 4 * m);
                             const float t =
                                 __builtin_fmaf(ax, q.x, __builtin_fmaf(ay, q.y, __builtin_fmaf(az, q.z, q.w))) + cp;
-                            // the sign bit (a zero t is within E: the exact test decides)
-                            hits |= (__float_as_uint(t) >> (31 - m)) & (1u << m);
+                            // t's sign bit (a zero t is within E: the exact test decides),
+                            // shifted in below the bits of atoms m + 1 .. A - 1 by one
+                            // funnel shift, so atom m ends at bit m
+                            hits = __builtin_amdgcn_alignbit(hits, __float_as_uint(t), 31);
                             band |= __builtin_amdgcn_ballot_w64(fabsf(t) < E);
                         }
                         if (band != 0) {  // rare: a test within E of the radius
